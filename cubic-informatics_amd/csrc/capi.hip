@@ -1,0 +1,496 @@
+// capi.hip -- the C-ABI of include/hdog.h: context, argument checks, scratch management
+// and stage sequencing.  No process-global state: everything lives in hdog_ctx.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "hdog_internal.hpp"
+#include "hdog.h"
+
+using namespace hdog;
+
+namespace {
+struct Buf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+}  // namespace
+
+struct hdog_ctx {
+  int device = 0;
+  std::string err;
+  Buf tmp1, tmp2, rid, bcnt, dR, acc, weights, blobs;
+  Buf o_label, o_size, o_vox, o_cen, o_eig, o_nrm, o_hess, o_isum, o_imax;
+  i64 out_cap = 0;
+  Buf img, norm, dog, hess, pd, labels;  // substack buffers of hdog_process_substack
+  bool gauss_ready = false;
+  float sigma[4] = {0, 0, 0, 0};
+  double truncate = 0.0;
+  int r[4] = {0, 0, 0, 0};
+  i64 R = 0;
+  uint32_t* h_R = nullptr;  // pinned
+  bool prof = false;        // record an event after every kernel of run_device
+  Prof pool;
+  bool ev_valid = false;
+};
+
+thread_local hdog::Prof* hdog::tl_prof = nullptr;
+
+namespace {
+
+int fail(hdog_ctx* c, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  c->err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return fail(ctx, HDOG_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));  \
+  } while (0)
+
+hipError_t ensure(Buf& b, size_t bytes) {
+  if (bytes <= b.bytes) return hipSuccess;
+  if (b.p) {
+    hipError_t e = hipFree(b.p);
+    if (e != hipSuccess) return e;
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  hipError_t e = hipMalloc(&b.p, bytes);
+  if (e == hipSuccess) b.bytes = bytes;
+  return e;
+}
+
+void release(Buf& b) {
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+int check_shape(hdog_ctx* ctx, int W, int H, int D) {
+  if (W < 1 || H < 1 || D < 1)
+    return fail(ctx, HDOG_ERR_ARG, "invalid shape %dx%dx%d (width x height x depth)", W, H, D);
+  const i64 N = (i64)W * H * D;
+  if (N >= (i64)0xFFFFFFFFll)
+    return fail(ctx, HDOG_ERR_ARG, "volume of %lld voxels exceeds the uint32 label range",
+                (long long)N);
+  return HDOG_OK;
+}
+
+// calcGaussianWeight, src/3D/DoG3DFilter.cu:372-392, truncation as a parameter
+int gaussian_weights(float sigma, double truncate, float* w) {
+  const int lw = (int)(truncate * (double)sigma + 0.5);
+  if (lw > kMaxGaussRadius) return -1;
+  const float s2 = sigma * sigma;
+  float sum = 1.0f;
+  w[lw] = 1.0f;
+  for (int i = 1; i <= lw; ++i) {
+    const float p = (float)std::exp(-0.5 * i * i / (double)s2);
+    w[lw - i] = p;
+    w[lw + i] = p;
+    sum += p * 2;
+  }
+  for (int i = 0; i < 2 * lw + 1; ++i) w[i] /= sum;
+  return lw;
+}
+
+int ensure_regions(hdog_ctx* ctx, i64 R) {
+  HIPCHK(ensure(ctx->acc, (size_t)R * sizeof(RegionAcc)));
+  if (R > ctx->out_cap) {
+    const i64 cap = std::max<i64>(R, ctx->out_cap * 3 / 2);
+    HIPCHK(ensure(ctx->o_label, cap * 4));
+    HIPCHK(ensure(ctx->o_size, cap * 2));
+    HIPCHK(ensure(ctx->o_vox, cap * 8));
+    HIPCHK(ensure(ctx->o_cen, cap * 3 * 4));
+    HIPCHK(ensure(ctx->o_eig, cap * 2 * 4));
+    HIPCHK(ensure(ctx->o_nrm, cap * 4));
+    HIPCHK(ensure(ctx->o_hess, cap * 6 * 4));
+    HIPCHK(ensure(ctx->o_isum, cap * 8));
+    HIPCHK(ensure(ctx->o_imax, cap * 2));
+    ctx->out_cap = cap;
+  }
+  return HDOG_OK;
+}
+
+template <typename TS>
+int regional_features_t(hdog_ctx* ctx, hipStream_t s, const uint32_t* labels, const TS* hess,
+                        const TS* norm, const uint16_t* img, int W, int H, int D, int is_ave,
+                        int64_t* num_regions) {
+  const i64 N = (i64)W * H * D;
+  HIPCHK(ensure(ctx->rid, (size_t)N * 4));
+  const i64 nb = (N + 4095) / 4096;
+  HIPCHK(ensure(ctx->bcnt, (size_t)nb * 4));
+  HIPCHK(ensure(ctx->dR, 64));
+  i64 nbo = 0;
+  HIPCHK(launch_region_ids(labels, (uint32_t*)ctx->rid.p, (uint32_t*)ctx->bcnt.p,
+                           (uint32_t*)ctx->dR.p, N, s, &nbo));
+  HIPCHK(hipMemcpyAsync(ctx->h_R, ctx->dR.p, 4, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  const i64 R = *ctx->h_R;
+  int rc = ensure_regions(ctx, R);
+  if (rc) return rc;
+  RegionAcc* acc = (RegionAcc*)ctx->acc.p;
+  HIPCHK(launch_region_assign(labels, (const uint32_t*)ctx->bcnt.p, (uint32_t*)ctx->rid.p, acc, R,
+                              N, s));
+  HIPCHK(launch_region_accum<TS>(labels, (const uint32_t*)ctx->rid.p, hess, norm, img, acc, R, W, H,
+                                 D, is_ave, s));
+  const i64 cap = ctx->out_cap;
+  RegionOut o;
+  o.label = (uint32_t*)ctx->o_label.p;
+  o.size = (uint16_t*)ctx->o_size.p;
+  o.voxels = (u64*)ctx->o_vox.p;
+  o.cx = (float*)ctx->o_cen.p;
+  o.cy = o.cx + cap;
+  o.cz = o.cx + 2 * cap;
+  o.st = (float*)ctx->o_eig.p;
+  o.bl = o.st + cap;
+  o.nrm = (float*)ctx->o_nrm.p;
+  o.hess = (float*)ctx->o_hess.p;
+  o.isum = (u64*)ctx->o_isum.p;
+  o.imax = (uint16_t*)ctx->o_imax.p;
+  HIPCHK(launch_region_finalize(acc, R, is_ave, o, cap, s));
+  ctx->R = R;
+  if (num_regions) *num_regions = R;
+  return HDOG_OK;
+}
+
+template <typename TS>
+int run_stages(hdog_ctx* ctx, hipStream_t s, const hdog_params* p, const uint16_t* img, TS* norm,
+               TS* dog, TS* hess, uint8_t* pd, uint32_t* labels, int64_t* num_regions) {
+  const int W = p->width, H = p->height, D = p->depth;
+  const i64 N = (i64)W * H * D;
+  HIPCHK(ensure(ctx->tmp1, (size_t)N * 8));
+  HIPCHK(ensure(ctx->tmp2, (size_t)N * 8));
+  struct ProfScope {  // points the launchers' marks at this context's pool for one call
+    explicit ProfScope(Prof* p) { tl_prof = p; }
+    ~ProfScope() { tl_prof = nullptr; }
+  } scope(ctx->prof ? &ctx->pool : nullptr);
+  ctx->ev_valid = false;
+  ctx->pool.n = 0;
+  mark(nullptr, s);
+  HIPCHK(launch_normalize<TS>(img, norm, ctx->tmp1.p, ctx->tmp2.p, p->min_intensity, W, H, D,
+                              p->radius_xy, p->radius_z, s));
+  const float nz = powf(ctx->sigma[0], p->gamma * 2) * powf(ctx->sigma[1], p->gamma);
+  HIPCHK(launch_dog<TS>(norm, dog, ctx->tmp1.p, ctx->tmp2.p, (const float*)ctx->weights.p, ctx->r,
+                        nz, W, H, D, s));
+  HIPCHK(launch_hessian<TS>(dog, hess, pd, W, H, D, s));
+  HIPCHK(launch_ccl(pd, labels, W, H, D, s));
+  int rc = regional_features_t<TS>(ctx, s, labels, hess, norm, img, W, H, D, p->is_ave_mode,
+                                   num_regions);
+  if (rc) return rc;
+  ctx->ev_valid = ctx->prof;
+  return HDOG_OK;
+}
+
+int check_params(hdog_ctx* ctx, const hdog_params* p) {
+  if (!p) return fail(ctx, HDOG_ERR_ARG, "params is NULL");
+  int rc = check_shape(ctx, p->width, p->height, p->depth);
+  if (rc) return rc;
+  if (p->radius_xy < 0 || p->radius_z < 0 || p->radius_xy > kMaxNormRadius ||
+      p->radius_z > kMaxNormRadius)
+    return fail(ctx, HDOG_ERR_ARG, "normalization radii (%d, %d) outside [0, %d]", p->radius_xy,
+                p->radius_z, kMaxNormRadius);
+  if (p->precision < HDOG_FP32 || p->precision > HDOG_BF16)
+    return fail(ctx, HDOG_ERR_ARG, "unknown precision %d", p->precision);
+  return HDOG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hdog_create(hdog_ctx** out, int device) {
+  if (!out) return HDOG_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return HDOG_ERR_HIP;
+  if (device < 0 || device >= n) return HDOG_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return HDOG_ERR_HIP;
+  hdog_ctx* c = new hdog_ctx();
+  c->device = device;
+  if (hipHostMalloc((void**)&c->h_R, 64, hipHostMallocDefault) != hipSuccess) {
+    delete c;
+    return HDOG_ERR_HIP;
+  }
+  *out = c;
+  return HDOG_OK;
+}
+
+int hdog_release_scratch(hdog_ctx* ctx) {
+  if (!ctx) return HDOG_ERR_ARG;
+  (void)hipSetDevice(ctx->device);
+  for (Buf* b : {&ctx->tmp1, &ctx->tmp2, &ctx->rid, &ctx->bcnt, &ctx->dR, &ctx->acc, &ctx->blobs,
+                 &ctx->o_label, &ctx->o_size, &ctx->o_vox, &ctx->o_cen, &ctx->o_eig, &ctx->o_nrm,
+                 &ctx->o_hess, &ctx->o_isum, &ctx->o_imax, &ctx->img, &ctx->norm, &ctx->dog,
+                 &ctx->hess, &ctx->pd, &ctx->labels})
+    release(*b);
+  ctx->out_cap = 0;
+  ctx->R = 0;
+  return HDOG_OK;
+}
+
+void hdog_destroy(hdog_ctx* ctx) {
+  if (!ctx) return;
+  hdog_release_scratch(ctx);
+  if (ctx->prof)
+    for (hipEvent_t& e : ctx->pool.ev) (void)hipEventDestroy(e);
+  release(ctx->weights);
+  if (ctx->h_R) (void)hipHostFree(ctx->h_R);
+  delete ctx;
+}
+
+const char* hdog_last_error(const hdog_ctx* ctx) { return ctx ? ctx->err.c_str() : "no context"; }
+
+int hdog_set_profiling(hdog_ctx* ctx, int on) {
+  if (!ctx) return HDOG_ERR_ARG;
+  HIPCHK(hipSetDevice(ctx->device));
+  if (on && !ctx->prof)
+    for (hipEvent_t& e : ctx->pool.ev) HIPCHK(hipEventCreate(&e));
+  if (!on && ctx->prof)
+    for (hipEvent_t& e : ctx->pool.ev) HIPCHK(hipEventDestroy(e));
+  ctx->prof = on != 0;
+  ctx->ev_valid = false;
+  return HDOG_OK;
+}
+
+int hdog_stage_times(hdog_ctx* ctx, double* ms, const char** names, int max_entries) {
+  if (!ctx || !ctx->prof || !ctx->ev_valid || ctx->pool.n < 2) return 0;
+  Prof& p = ctx->pool;
+  if (hipEventSynchronize(p.ev[p.n - 1]) != hipSuccess) return 0;
+  int n = 0;
+  for (int k = 1; k < p.n && n < max_entries; ++k) {
+    if (!p.name[k]) continue;  // interval that only restarts timing after a host wait
+    float t = 0.0f;
+    if (hipEventElapsedTime(&t, p.ev[k - 1], p.ev[k]) != hipSuccess) return n;
+    if (ms) ms[n] = t;
+    if (names) names[n] = p.name[k];
+    ++n;
+  }
+  return n;
+}
+
+int hdog_normalize3d(hdog_ctx* ctx, void* stream, const uint16_t* d_img, float* d_norm,
+                     float min_intensity, int width, int height, int depth, int radius_xy,
+                     int radius_z) {
+  if (!ctx) return HDOG_ERR_ARG;
+  int rc = check_shape(ctx, width, height, depth);
+  if (rc) return rc;
+  if (!d_img || !d_norm) return fail(ctx, HDOG_ERR_ARG, "NULL device pointer");
+  if (radius_xy < 0 || radius_z < 0 || radius_xy > kMaxNormRadius || radius_z > kMaxNormRadius)
+    return fail(ctx, HDOG_ERR_ARG, "normalization radii (%d, %d) outside [0, %d]", radius_xy,
+                radius_z, kMaxNormRadius);
+  HIPCHK(hipSetDevice(ctx->device));
+  const i64 N = (i64)width * height * depth;
+  HIPCHK(ensure(ctx->tmp1, (size_t)N * 8));
+  HIPCHK(ensure(ctx->tmp2, (size_t)N * 8));
+  HIPCHK(launch_normalize<float>(d_img, d_norm, ctx->tmp1.p, ctx->tmp2.p, min_intensity, width,
+                                 height, depth, radius_xy, radius_z, (hipStream_t)stream));
+  return HDOG_OK;
+}
+
+int hdog_init_gaussian3d(hdog_ctx* ctx, float sxy1, float sz1, float sxy2, float sz2,
+                         double truncate, int* r_z2) {
+  if (!ctx) return HDOG_ERR_ARG;
+  const float sg[4] = {sxy1, sz1, sxy2, sz2};
+  std::vector<float> w(4 * kMaxTaps, 0.0f);
+  int r[4];
+  for (int k = 0; k < 4; ++k) {
+    if (!(sg[k] > 0.0f) || !(truncate >= 0.0))
+      return fail(ctx, HDOG_ERR_ARG, "sigma %g / truncate %g must be positive", sg[k], truncate);
+    r[k] = gaussian_weights(sg[k], truncate, w.data() + k * kMaxTaps);
+    if (r[k] < 0)
+      return fail(ctx, HDOG_ERR_ARG, "Gaussian radius of sigma %g at truncate %g exceeds %d", sg[k],
+                  truncate, kMaxGaussRadius);
+  }
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(ensure(ctx->weights, w.size() * 4));
+  HIPCHK(hipMemcpy(ctx->weights.p, w.data(), w.size() * 4, hipMemcpyHostToDevice));
+  for (int k = 0; k < 4; ++k) {
+    ctx->sigma[k] = sg[k];
+    ctx->r[k] = r[k];
+  }
+  ctx->truncate = truncate;
+  ctx->gauss_ready = true;
+  if (r_z2) *r_z2 = r[3];
+  return HDOG_OK;
+}
+
+int hdog_dog3d(hdog_ctx* ctx, void* stream, const float* d_norm, float* d_dog, int width,
+               int height, int depth, float gamma) {
+  if (!ctx) return HDOG_ERR_ARG;
+  int rc = check_shape(ctx, width, height, depth);
+  if (rc) return rc;
+  if (!ctx->gauss_ready) return fail(ctx, HDOG_ERR_STATE, "hdog_init_gaussian3d not called");
+  if (!d_norm || !d_dog) return fail(ctx, HDOG_ERR_ARG, "NULL device pointer");
+  HIPCHK(hipSetDevice(ctx->device));
+  const i64 N = (i64)width * height * depth;
+  HIPCHK(ensure(ctx->tmp1, (size_t)N * 8));
+  HIPCHK(ensure(ctx->tmp2, (size_t)N * 8));
+  const float nz = powf(ctx->sigma[0], gamma * 2) * powf(ctx->sigma[1], gamma);
+  HIPCHK(launch_dog<float>(d_norm, d_dog, ctx->tmp1.p, ctx->tmp2.p, (const float*)ctx->weights.p,
+                           ctx->r, nz, width, height, depth, (hipStream_t)stream));
+  return HDOG_OK;
+}
+
+int hdog_hessian_pd(hdog_ctx* ctx, void* stream, const float* d_dog, float* d_hessian,
+                    uint8_t* d_pd, int width, int height, int depth) {
+  if (!ctx) return HDOG_ERR_ARG;
+  int rc = check_shape(ctx, width, height, depth);
+  if (rc) return rc;
+  if (!d_dog || !d_hessian || !d_pd) return fail(ctx, HDOG_ERR_ARG, "NULL device pointer");
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(launch_hessian<float>(d_dog, d_hessian, d_pd, width, height, depth, (hipStream_t)stream));
+  return HDOG_OK;
+}
+
+int hdog_ccl3d(hdog_ctx* ctx, void* stream, const uint8_t* d_pd, uint32_t* d_labels, int width,
+               int height, int depth) {
+  if (!ctx) return HDOG_ERR_ARG;
+  int rc = check_shape(ctx, width, height, depth);
+  if (rc) return rc;
+  if (!d_pd || !d_labels) return fail(ctx, HDOG_ERR_ARG, "NULL device pointer");
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(launch_ccl(d_pd, d_labels, width, height, depth, (hipStream_t)stream));
+  return HDOG_OK;
+}
+
+int hdog_regional_features(hdog_ctx* ctx, void* stream, const uint32_t* d_labels,
+                           const float* d_hessian, const float* d_norm, const uint16_t* d_img,
+                           int width, int height, int depth, int is_ave_mode,
+                           int64_t* num_regions) {
+  if (!ctx) return HDOG_ERR_ARG;
+  int rc = check_shape(ctx, width, height, depth);
+  if (rc) return rc;
+  if (!d_labels || !d_hessian || !d_norm) return fail(ctx, HDOG_ERR_ARG, "NULL device pointer");
+  HIPCHK(hipSetDevice(ctx->device));
+  return regional_features_t<float>(ctx, (hipStream_t)stream, d_labels, d_hessian, d_norm, d_img,
+                                    width, height, depth, is_ave_mode, num_regions);
+}
+
+int hdog_regions_download(hdog_ctx* ctx, void* stream, hdog_regions* out) {
+  if (!ctx || !out) return HDOG_ERR_ARG;
+  const i64 R = ctx->R;
+  out->count = R;
+  if (R > out->capacity)
+    return fail(ctx, HDOG_ERR_CAPACITY, "%lld regions exceed the caller capacity %lld",
+                (long long)R, (long long)out->capacity);
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t s = (hipStream_t)stream;
+  const i64 cap = ctx->out_cap;
+  auto cp = [&](void* dst, const void* src, size_t bytes) -> hipError_t {
+    if (!dst || bytes == 0) return hipSuccess;
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s);
+  };
+  const float* cen = (const float*)ctx->o_cen.p;
+  const float* eig = (const float*)ctx->o_eig.p;
+  HIPCHK(cp(out->label, ctx->o_label.p, R * 4));
+  HIPCHK(cp(out->size, ctx->o_size.p, R * 2));
+  HIPCHK(cp(out->voxels, ctx->o_vox.p, R * 8));
+  HIPCHK(cp(out->centroid_x, cen, R * 4));
+  HIPCHK(cp(out->centroid_y, cen + cap, R * 4));
+  HIPCHK(cp(out->centroid_z, cen + 2 * cap, R * 4));
+  HIPCHK(cp(out->structureness, eig, R * 4));
+  HIPCHK(cp(out->blobness, eig + cap, R * 4));
+  HIPCHK(cp(out->max_normalized, ctx->o_nrm.p, R * 4));
+  if (out->hessian)
+    for (int k = 0; k < 6; ++k)
+      HIPCHK(cp(out->hessian + k * out->capacity, (const float*)ctx->o_hess.p + k * cap, R * 4));
+  HIPCHK(cp(out->intensity_sum, ctx->o_isum.p, R * 8));
+  HIPCHK(cp(out->intensity_max, ctx->o_imax.p, R * 2));
+  HIPCHK(hipStreamSynchronize(s));
+  return HDOG_OK;
+}
+
+static int ensure_gauss(hdog_ctx* ctx, const hdog_params* p) {
+  if (ctx->gauss_ready && ctx->sigma[0] == p->sigma_xy1 && ctx->sigma[1] == p->sigma_z1 &&
+      ctx->sigma[2] == p->sigma_xy2 && ctx->sigma[3] == p->sigma_z2 && ctx->truncate == p->truncate)
+    return HDOG_OK;
+  return hdog_init_gaussian3d(ctx, p->sigma_xy1, p->sigma_z1, p->sigma_xy2, p->sigma_z2,
+                              p->truncate, nullptr);
+}
+
+int hdog_run_device(hdog_ctx* ctx, void* stream, const hdog_params* p, const uint16_t* d_img,
+                    void* d_norm, void* d_dog, void* d_hessian, uint8_t* d_pd, uint32_t* d_labels,
+                    int64_t* num_regions) {
+  if (!ctx) return HDOG_ERR_ARG;
+  int rc = check_params(ctx, p);
+  if (rc) return rc;
+  if (!d_img) return fail(ctx, HDOG_ERR_ARG, "NULL image pointer");
+  HIPCHK(hipSetDevice(ctx->device));
+  if ((rc = ensure_gauss(ctx, p))) return rc;
+  const i64 N = (i64)p->width * p->height * p->depth;
+  const size_t es = p->precision == HDOG_FP32 ? 4 : 2;
+  if (!d_norm) { HIPCHK(ensure(ctx->norm, N * es)); d_norm = ctx->norm.p; }
+  if (!d_dog) { HIPCHK(ensure(ctx->dog, N * es)); d_dog = ctx->dog.p; }
+  if (!d_hessian) { HIPCHK(ensure(ctx->hess, 6 * N * es)); d_hessian = ctx->hess.p; }
+  if (!d_pd) { HIPCHK(ensure(ctx->pd, N)); d_pd = (uint8_t*)ctx->pd.p; }
+  if (!d_labels) { HIPCHK(ensure(ctx->labels, N * 4)); d_labels = (uint32_t*)ctx->labels.p; }
+  hipStream_t s = (hipStream_t)stream;
+  switch (p->precision) {
+    case HDOG_FP16:
+      return run_stages<__half>(ctx, s, p, d_img, (__half*)d_norm, (__half*)d_dog,
+                                (__half*)d_hessian, d_pd, d_labels, num_regions);
+    case HDOG_BF16:
+      return run_stages<__hip_bfloat16>(ctx, s, p, d_img, (__hip_bfloat16*)d_norm,
+                                        (__hip_bfloat16*)d_dog, (__hip_bfloat16*)d_hessian, d_pd,
+                                        d_labels, num_regions);
+    default:
+      return run_stages<float>(ctx, s, p, d_img, (float*)d_norm, (float*)d_dog, (float*)d_hessian,
+                               d_pd, d_labels, num_regions);
+  }
+}
+
+int hdog_process_substack(hdog_ctx* ctx, void* stream, const hdog_params* p,
+                          const uint16_t* h_img, int depth_valid, hdog_regions* out) {
+  if (!ctx) return HDOG_ERR_ARG;
+  int rc = check_params(ctx, p);
+  if (rc) return rc;
+  if (!h_img || depth_valid < 1 || depth_valid > p->depth)
+    return fail(ctx, HDOG_ERR_ARG, "depth_valid %d outside [1, %d]", depth_valid, p->depth);
+  HIPCHK(hipSetDevice(ctx->device));
+  const i64 plane = (i64)p->width * p->height;
+  const i64 N = plane * p->depth;
+  HIPCHK(ensure(ctx->img, N * 2));
+  hipStream_t s = (hipStream_t)stream;
+  HIPCHK(hipMemcpyAsync(ctx->img.p, h_img, plane * depth_valid * 2, hipMemcpyHostToDevice, s));
+  if (depth_valid < p->depth)  // zero-fill, src/3D/HDoG3D_main.cpp:166-169
+    HIPCHK(hipMemsetAsync((uint16_t*)ctx->img.p + plane * depth_valid, 0,
+                          (N - plane * depth_valid) * 2, s));
+  int64_t R = 0;
+  rc = hdog_run_device(ctx, stream, p, (const uint16_t*)ctx->img.p, nullptr, nullptr, nullptr,
+                       nullptr, nullptr, &R);
+  if (rc) return rc;
+  if (!out) return HDOG_OK;
+  return hdog_regions_download(ctx, stream, out);
+}
+
+int hdog_synth_volume(hdog_ctx* ctx, void* stream, uint16_t* d_img, float* d_scratch, int width,
+                      int height, int depth, uint64_t seed, double mean0, double mean1, double sd,
+                      const double* blobs, int64_t n_blobs) {
+  if (!ctx) return HDOG_ERR_ARG;
+  int rc = check_shape(ctx, width, height, depth);
+  if (rc) return rc;
+  if (!d_img || !d_scratch || (n_blobs > 0 && !blobs) || n_blobs < 0)
+    return fail(ctx, HDOG_ERR_ARG, "NULL pointer or negative blob count");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t s = (hipStream_t)stream;
+  if (n_blobs > 0) {
+    HIPCHK(ensure(ctx->blobs, (size_t)n_blobs * 7 * 8));
+    HIPCHK(hipMemcpyAsync(ctx->blobs.p, blobs, (size_t)n_blobs * 7 * 8, hipMemcpyHostToDevice, s));
+  }
+  HIPCHK(launch_synth(d_img, d_scratch, width, height, depth, seed, mean0, mean1, sd,
+                      (const double*)ctx->blobs.p, n_blobs, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return HDOG_OK;
+}
+
+}  // extern "C"

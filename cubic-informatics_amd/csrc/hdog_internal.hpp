@@ -1,0 +1,101 @@
+// hdog_internal.hpp -- shared declarations of the HIP kernels' launchers and the
+// context object behind include/hdog.h.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+#include <cstdint>
+#include <string>
+
+typedef int64_t i64;
+typedef uint64_t u64;
+
+namespace hdog {
+
+constexpr int kMaxGaussRadius = 40;            // 2r+1 <= 81 taps per axis
+constexpr int kMaxTaps = 2 * kMaxGaussRadius + 1;
+constexpr int kMaxNormRadius = 200;
+
+// ---- storage types of float intermediates --------------------------------------
+template <typename T> __device__ __forceinline__ float ld(const T* p, i64 i);
+template <> __device__ __forceinline__ float ld<float>(const float* p, i64 i) { return p[i]; }
+template <> __device__ __forceinline__ float ld<__half>(const __half* p, i64 i) {
+  return __half2float(p[i]);
+}
+template <> __device__ __forceinline__ float ld<__hip_bfloat16>(const __hip_bfloat16* p, i64 i) {
+  return __bfloat162float(p[i]);
+}
+template <typename T> __device__ __forceinline__ T cvt(float v);
+template <> __device__ __forceinline__ float cvt<float>(float v) { return v; }
+template <> __device__ __forceinline__ __half cvt<__half>(float v) { return __float2half_rn(v); }
+template <> __device__ __forceinline__ __hip_bfloat16 cvt<__hip_bfloat16>(float v) {
+  return __float2bfloat16(v);
+}
+
+__host__ __device__ __forceinline__ int clampi(int v, int lo, int hi) {
+  return v < lo ? lo : (v > hi ? hi : v);
+}
+
+// ---- per-region accumulator (one 128-byte line per region) -----------------------
+struct alignas(128) RegionAcc {
+  unsigned long long sx, sy, sz, isum;
+  double h[6];
+  double nsum;
+  unsigned int cnt, nmax_key, imax, root;
+  unsigned int pad[2];
+};
+
+// ---- optional per-kernel event marks (profiling mode of the C-ABI) ----------------
+// A launcher calls mark(name, s) after each kernel it enqueues; with profiling on the
+// context points tl_prof at its event pool for the duration of the call.  A mark with
+// name nullptr only starts a new interval (after a host synchronization).
+struct Prof {
+  static constexpr int kMax = 64;
+  hipEvent_t ev[kMax];
+  const char* name[kMax];
+  int n = 0;
+};
+extern thread_local Prof* tl_prof;
+inline void mark(const char* name, hipStream_t s) {
+  Prof* p = tl_prof;
+  if (p && p->n < Prof::kMax) {
+    (void)hipEventRecord(p->ev[p->n], s);
+    p->name[p->n++] = name;
+  }
+}
+
+// ---- launchers (return hipSuccess or the launch error) ----------------------------
+// normalize stage; tmp1/tmp2 each hold >= 8*N bytes
+template <typename TS>
+hipError_t launch_normalize(const uint16_t* img, TS* norm, void* tmp1, void* tmp2,
+                            float min_intensity, int W, int H, int D, int rxy, int rz,
+                            hipStream_t s);
+// DoG stage; weights: device array [4][kMaxTaps] (xy1, z1, xy2, z2); tmp1/tmp2 >= 8*N bytes
+template <typename TS>
+hipError_t launch_dog(const TS* norm, TS* dog, void* tmp1, void* tmp2, const float* weights,
+                      const int r[4], float normalizer, int W, int H, int D, hipStream_t s);
+template <typename TS>
+hipError_t launch_hessian(const TS* dog, TS* hess, uint8_t* pd, int W, int H, int D,
+                          hipStream_t s);
+// CCL: labels out; union-find in place
+hipError_t launch_ccl(const uint8_t* pd, uint32_t* labels, int W, int H, int D, hipStream_t s);
+// region numbering: fills rid[root] for every root, block counts, and *d_R = R
+hipError_t launch_region_ids(const uint32_t* labels, uint32_t* rid, uint32_t* block_cnt,
+                             uint32_t* d_R, i64 N, hipStream_t s, i64* nblocks_out);
+hipError_t launch_region_assign(const uint32_t* labels, const uint32_t* block_off, uint32_t* rid,
+                                RegionAcc* acc, i64 R, i64 N, hipStream_t s);
+template <typename TS>
+hipError_t launch_region_accum(const uint32_t* labels, const uint32_t* rid, const TS* hess,
+                               const TS* norm, const uint16_t* img, RegionAcc* acc, i64 R,
+                               int W, int H, int D, int is_ave, hipStream_t s);
+struct RegionOut {
+  uint32_t* label; uint16_t* size; u64* voxels; float* cx; float* cy; float* cz;
+  float* st; float* bl; float* nrm; float* hess; u64* isum; uint16_t* imax;
+};
+hipError_t launch_region_finalize(const RegionAcc* acc, i64 R, int is_ave, RegionOut o,
+                                  i64 pitch, hipStream_t s);
+hipError_t launch_synth(uint16_t* img, float* scratch, int W, int H, int D, u64 seed,
+                        double mean0, double mean1, double sd, const double* d_blobs,
+                        i64 n_blobs, hipStream_t s);
+
+}  // namespace hdog

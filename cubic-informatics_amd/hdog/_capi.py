@@ -1,0 +1,83 @@
+"""ctypes binding of libhdog.so (include/hdog.h).
+
+torch is imported first so that libhdog.so resolves its HIP runtime dependency to the
+runtime torch already loaded: device pointers and streams are then shared.  There is
+no fallback: if the library is missing this module raises at import.
+"""
+import ctypes
+import os
+
+import torch  # noqa: F401  (loads the HIP runtime libhdog.so binds to)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhdog.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"{LIB_PATH} is missing: build it with `make -C cubic-informatics_amd` "
+        "(or __graft_entry__.build()); the HDoG path has no CPU fallback")
+
+lib = ctypes.CDLL(LIB_PATH)
+
+HDOG_OK, HDOG_ERR_ARG, HDOG_ERR_HIP, HDOG_ERR_STATE, HDOG_ERR_CAPACITY = range(5)
+HDOG_FP32, HDOG_FP16, HDOG_BF16 = range(3)
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_f = ctypes.c_float
+_d = ctypes.c_double
+_i64 = ctypes.c_int64
+
+
+class Regions(ctypes.Structure):
+    _fields_ = [("capacity", _i64), ("count", _i64), ("label", _vp), ("size", _vp),
+                ("voxels", _vp), ("centroid_x", _vp), ("centroid_y", _vp), ("centroid_z", _vp),
+                ("structureness", _vp), ("blobness", _vp), ("max_normalized", _vp),
+                ("hessian", _vp), ("intensity_sum", _vp), ("intensity_max", _vp)]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("width", _i), ("height", _i), ("depth", _i), ("radius_xy", _i),
+                ("radius_z", _i), ("min_intensity", _f), ("sigma_xy1", _f), ("sigma_z1", _f),
+                ("sigma_xy2", _f), ("sigma_z2", _f), ("truncate", _d), ("gamma", _f),
+                ("is_ave_mode", _i), ("precision", _i)]
+
+
+def _sig(name, args):
+    fn = getattr(lib, name)
+    fn.argtypes = args
+    fn.restype = _i
+    return fn
+
+
+hdog_create = _sig("hdog_create", [ctypes.POINTER(_vp), _i])
+lib.hdog_destroy.argtypes = [_vp]
+lib.hdog_destroy.restype = None
+hdog_destroy = lib.hdog_destroy
+lib.hdog_last_error.argtypes = [_vp]
+lib.hdog_last_error.restype = ctypes.c_char_p
+hdog_last_error = lib.hdog_last_error
+hdog_release_scratch = _sig("hdog_release_scratch", [_vp])
+hdog_normalize3d = _sig("hdog_normalize3d", [_vp, _vp, _vp, _vp, _f, _i, _i, _i, _i, _i])
+hdog_init_gaussian3d = _sig("hdog_init_gaussian3d", [_vp, _f, _f, _f, _f, _d, ctypes.POINTER(_i)])
+hdog_dog3d = _sig("hdog_dog3d", [_vp, _vp, _vp, _vp, _i, _i, _i, _f])
+hdog_hessian_pd = _sig("hdog_hessian_pd", [_vp, _vp, _vp, _vp, _vp, _i, _i, _i])
+hdog_ccl3d = _sig("hdog_ccl3d", [_vp, _vp, _vp, _vp, _i, _i, _i])
+hdog_regional_features = _sig("hdog_regional_features", [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i,
+                                                         _i, ctypes.POINTER(_i64)])
+hdog_regions_download = _sig("hdog_regions_download", [_vp, _vp, ctypes.POINTER(Regions)])
+hdog_process_substack = _sig("hdog_process_substack", [_vp, _vp, ctypes.POINTER(Params), _vp, _i,
+                                                       ctypes.POINTER(Regions)])
+hdog_run_device = _sig("hdog_run_device", [_vp, _vp, ctypes.POINTER(Params), _vp, _vp, _vp, _vp,
+                                           _vp, _vp, ctypes.POINTER(_i64)])
+hdog_synth_volume = _sig("hdog_synth_volume", [_vp, _vp, _vp, _vp, _i, _i, _i, ctypes.c_uint64,
+                                               _d, _d, _d, _vp, _i64])
+
+hdog_set_profiling = _sig("hdog_set_profiling", [_vp, _i])
+hdog_stage_times = _sig("hdog_stage_times", [_vp, ctypes.POINTER(_d), ctypes.POINTER(ctypes.c_char_p), _i])
+
+EXPORTED = ["hdog_create", "hdog_destroy", "hdog_last_error", "hdog_release_scratch",
+            "hdog_normalize3d", "hdog_init_gaussian3d", "hdog_dog3d", "hdog_hessian_pd",
+            "hdog_ccl3d", "hdog_regional_features", "hdog_regions_download",
+            "hdog_process_substack", "hdog_run_device", "hdog_synth_volume",
+            "hdog_set_profiling", "hdog_stage_times"]

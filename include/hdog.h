@@ -1,0 +1,175 @@
+/* hdog.h -- C-ABI of the MI355X HDoG cell-candidate detection library (libhdog.so).
+ *
+ * Plain C types only: device pointers are caller-owned HIP allocations, the stream
+ * is a hipStream_t passed as void* (NULL = default stream).  Every entry point
+ * returns an hdog_status; hdog_last_error() gives the message of the last failure
+ * on that context.  One context per host thread; contexts share nothing.
+ *
+ * Each entry point names the reference interface it replaces (paths relative to
+ * the lsb-riken/CUBIC-informatics tree).  The reference functions are C/C++ linkage
+ * launch wrappers on the legacy default stream with process-global state; these take
+ * an explicit context and stream instead and hold no globals.
+ *
+ * Volumes are x-fastest: index i = (z*height + y)*width + x, N = width*height*depth,
+ * N < 2^32 (labels are uint32; the reference's int32 labels overflow at 2^31).
+ */
+#ifndef HDOG_H
+#define HDOG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hdog_ctx hdog_ctx;
+
+enum hdog_status {
+  HDOG_OK = 0,
+  HDOG_ERR_ARG = 1,      /* invalid argument (shape, radius, NULL pointer) */
+  HDOG_ERR_HIP = 2,      /* HIP runtime error (allocation, launch) */
+  HDOG_ERR_STATE = 3,    /* call order violated (e.g. DoG before init_gaussian3d) */
+  HDOG_ERR_CAPACITY = 4  /* caller's region arrays are too small */
+};
+
+/* Storage type of the float intermediates (normalized, DoG, Hessian).  Every
+ * separable pass accumulates in FP32 whatever the storage type. */
+enum hdog_precision { HDOG_FP32 = 0, HDOG_FP16 = 1, HDOG_BF16 = 2 };
+
+/* Per-region results, structure of arrays indexed by region id r = 0..count-1 in
+ * ascending label order (region 0 = background, never filled).  The caller supplies
+ * `capacity` entries for every non-NULL array; NULL arrays are skipped.
+ * Mirrors the arrays src/3D/HDoG3D_main.cpp:216-227 downloads. */
+typedef struct hdog_regions {
+  int64_t capacity;         /* in: entries available per array */
+  int64_t count;            /* out: number of regions R including region 0 */
+  uint32_t* label;          /* minimum linear index of the component */
+  uint16_t* size;           /* voxel count modulo 65536 (u16 RLE of the reference) */
+  uint64_t* voxels;         /* exact voxel count */
+  float* centroid_x;        /* sum(x) / (float)size */
+  float* centroid_y;
+  float* centroid_z;
+  float* structureness;     /* sum of squared eigenvalues of the summed Hessian */
+  float* blobness;          /* e2^2 / (e0*e1) */
+  float* max_normalized;    /* max of normalized intensity (sum in ave mode) */
+  float* hessian;           /* 6*capacity: [xx,xy,xz,yy,yz,zz] sums, pitch capacity */
+  uint64_t* intensity_sum;  /* exact sum of raw uint16 intensity */
+  uint16_t* intensity_max;  /* max raw intensity (MaxIntensity, RegionalFeatures.cu:39-63) */
+} hdog_regions;
+
+/* Parameters of one substack run (HDoG_param of the JSON, src/3D/utils.h:48-93). */
+typedef struct hdog_params {
+  int width, height, depth;  /* substack shape; depth = HDoG_param.depth */
+  int radius_xy, radius_z;   /* radius_norm.{xy_large,z_large} */
+  float min_intensity;       /* HDoG_param.min_intensity */
+  float sigma_xy1, sigma_z1, sigma_xy2, sigma_z2;  /* sigma_dog */
+  double truncate;           /* Gaussian radius = (int)(truncate*sigma + 0.5); reference 2.0 */
+  float gamma;               /* gamma_n */
+  int is_ave_mode;           /* top-level is_ave_mode: sum instead of max of normalized */
+  int precision;             /* enum hdog_precision */
+} hdog_params;
+
+/* ---- context --------------------------------------------------------------- */
+/* Selects `device` (hipSetDevice) and creates an empty context.  Replaces the device
+ * set-up of initializeGPU, src/3D/HDoG3D_main.cpp:58-109. */
+int hdog_create(hdog_ctx** ctx, int device);
+void hdog_destroy(hdog_ctx* ctx);
+const char* hdog_last_error(const hdog_ctx* ctx);
+/* Frees scratch buffers (they grow on demand and are kept between calls). */
+int hdog_release_scratch(hdog_ctx* ctx);
+
+/* ---- stages (device pointers, stream ordered) ---------------------------- */
+
+/* Separable min and max filter (clamp), box means of both (clamp), then
+ * norm = I >= min_intensity ? (I - E)/(D - E) : 0; a thresholded copy when either
+ * radius is 0.  Replaces Normalize3DFilter, src/3D/NormalizeFilter.cuh:5-12
+ * (src/3D/NormalizeFilter.cu:603-654). */
+int hdog_normalize3d(hdog_ctx* ctx, void* stream, const uint16_t* d_img, float* d_norm,
+                     float min_intensity, int width, int height, int depth,
+                     int radius_xy, int radius_z);
+
+/* Gaussian weights for sigma1 (xy, z) and sigma2 (xy, z); *r_z2 receives the z radius
+ * of sigma2, which sets the substack depth margin.  Replaces initGaussian3DKernel,
+ * src/3D/DoG3DFilter.cuh:5-9 (src/3D/DoG3DFilter.cu:372-415) with the truncation
+ * (reference constant 2.0, :15) as a parameter. */
+int hdog_init_gaussian3d(hdog_ctx* ctx, float sigma_xy1, float sigma_z1, float sigma_xy2,
+                         float sigma_z2, double truncate, int* r_z2);
+
+/* dog = (G_sigma1 - G_sigma2) * powf(sigma_xy1, 2 gamma) * powf(sigma_z1, gamma), zero
+ * padding.  Replaces DoG3DFilter, src/3D/DoG3DFilter.cuh:17-24. */
+int hdog_dog3d(hdog_ctx* ctx, void* stream, const float* d_norm, float* d_dog,
+               int width, int height, int depth, float gamma);
+
+/* Hessian elements by repeated central differences (zero padding), SoA
+ * [xx,xy,xz,yy,yz,zz] at pitch N, and the negative-definite candidate mask.
+ * Replaces HessianPositiveDefiniteWithElement, src/3D/Hessian3DFilter_element.cuh:4-8. */
+int hdog_hessian_pd(hdog_ctx* ctx, void* stream, const float* d_dog, float* d_hessian,
+                    uint8_t* d_pd, int width, int height, int depth);
+
+/* 26-neighbour connected components over linear-index offsets (x and y borders wrap
+ * as in the reference); foreground voxels receive the minimum linear index of their
+ * component, background 0, voxel 0 is always background.  Replaces CCL,
+ * src/3D/CCL3D.cuh:5-10. */
+int hdog_ccl3d(hdog_ctx* ctx, void* stream, const uint8_t* d_pd, uint32_t* d_labels,
+               int width, int height, int depth);
+
+/* Per-region features of a labelled substack: Hessian sums, eigen-features, max (or
+ * sum) of normalized intensity, size, centroid, plus exact intensity sum / max.
+ * d_img may be NULL (intensity outputs then read 0).  Results stay on the device in
+ * the context; read them with hdog_regions_download.  Synchronizes `stream` once to
+ * learn the region count.  Replaces HessianFeatures + Eigenvalues + MaxNormalized /
+ * SumNormalized + RegionalSizeAndCentroid (src/3D/RegionalFeatures.cuh:16-102,
+ * src/3D/Eigenvalues.cuh:21-32) as sequenced at src/3D/HDoG3D_main.cpp:184-214. */
+int hdog_regional_features(hdog_ctx* ctx, void* stream, const uint32_t* d_labels,
+                           const float* d_hessian, const float* d_norm,
+                           const uint16_t* d_img, int width, int height, int depth,
+                           int is_ave_mode, int64_t* num_regions);
+
+/* Copies the last hdog_regional_features result into caller host arrays (regions
+ * 0..count-1; entry 0 zero) and synchronizes `stream`.  Replaces the D2H copies of
+ * src/3D/HDoG3D_main.cpp:216-227. */
+int hdog_regions_download(hdog_ctx* ctx, void* stream, hdog_regions* out);
+
+/* ---- fused substack ------------------------------------------------------- */
+/* One substack end to end: copies depth_valid host slices (h_img, x-fastest) to the
+ * device, zero-fills up to p->depth, runs every stage on p->depth slices and
+ * downloads the region arrays into *out.  Replaces processSubstack,
+ * src/3D/HDoG3D_main.cpp:134-257 (minus file I/O, which stays in the host program).
+ * Gaussian weights are (re)initialised from p when they differ from the last call. */
+int hdog_process_substack(hdog_ctx* ctx, void* stream, const hdog_params* p,
+                          const uint16_t* h_img, int depth_valid, hdog_regions* out);
+
+/* Same pipeline on a volume already resident on the device.  The intermediates are
+ * written to caller buffers when the pointers are non-NULL (else context scratch is
+ * used); storage type follows p->precision for norm / dog / hessian (2 bytes per
+ * element for FP16 and BF16).  Region results stay on the device as for
+ * hdog_regional_features. */
+int hdog_run_device(hdog_ctx* ctx, void* stream, const hdog_params* p, const uint16_t* d_img,
+                    void* d_norm, void* d_dog, void* d_hessian, uint8_t* d_pd,
+                    uint32_t* d_labels, int64_t* num_regions);
+
+/* ---- kernel timing ----------------------------------------------------------- */
+/* With profiling on, hdog_run_device / hdog_process_substack record a HIP event on the
+ * caller's stream after every kernel they enqueue; hdog_stage_times waits for the last
+ * run and writes up to max_entries (device milliseconds, "stage/kernel" name) pairs in
+ * launch order; returns how many were written (0 when none are available).  The
+ * reference only prints host wall-clock per substack (src/3D/HDoG3D_main.cpp:162-230). */
+int hdog_set_profiling(hdog_ctx* ctx, int on);
+int hdog_stage_times(hdog_ctx* ctx, double* ms, const char** names, int max_entries);
+
+/* ---- synthetic input (benchmark / tests) ---------------------------------- */
+/* Renders noise + Gaussian blobs into d_img (depth x height x width uint16):
+ * background mean0 + (mean1 - mean0) * z / (depth-1), sd `sd` (counter-based normal
+ * variates keyed by seed), clipped at 0, plus n_blobs blobs given as rows
+ * (cz, cy, cx, sz, sy, sx, A) of the host table `blobs` (double), each evaluated within
+ * 4 sigma of its centre; then rint and clip to [0, 65535].  d_scratch must hold N
+ * floats. */
+int hdog_synth_volume(hdog_ctx* ctx, void* stream, uint16_t* d_img, float* d_scratch,
+                      int width, int height, int depth, uint64_t seed, double mean0,
+                      double mean1, double sd, const double* blobs, int64_t n_blobs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HDOG_H */

@@ -1,0 +1,236 @@
+"""GPU parity: the hand-written gfx950 kernels (through the C-ABI) against the CPU oracle.
+
+Bar: bit-exact for the normalized volume, DoG, Hessian elements, candidate mask, labels,
+region ids, voxel counts, u16 sizes, coordinate and intensity sums, max-normalized
+intensity and centroids; Hessian sums within FP64-accumulation-order noise; structureness
+and blobness within 1e-5 relative (north-star FP32 tolerance).  NaN equals NaN.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from hdog import synth
+
+pytestmark = pytest.mark.gpu
+
+RNG = np.random.default_rng(99)
+
+
+@pytest.fixture(scope="module")
+def hd():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from hdog import HDoG
+    h = HDoG(0)
+    yield h
+    h.close()
+
+
+def to_np(t):
+    import torch
+    if t.dtype in (torch.uint16, torch.uint32):
+        return t.cpu().numpy()
+    return t.cpu().numpy()
+
+
+def assert_regions_match(g, o, hess):
+    """g: GPU regions dict, o: oracle regions dict."""
+    R = len(o["label"])
+    assert len(g["label"]) == R, (len(g["label"]), R)
+    for k in ("label", "count", "size16", "isum", "imax"):
+        np.testing.assert_array_equal(g[k][1:], o[k][1:], err_msg=k)
+    np.testing.assert_array_equal(g["centroid"][1:], o["centroid"][1:])
+    np.testing.assert_array_equal(g["norm"][1:], o["norm"][1:])
+    # Hessian sums: both round one FP64 sum to FP32; the FP64 sums differ only in
+    # accumulation order (GPU atomics vs linear order)
+    mag = np.stack([np.abs(hess[k]).max() for k in range(6)])[None, :] * o["count"][1:, None]
+    diff = np.abs(g["hsum"][1:].astype(np.float64) - o["hsum"][1:])
+    tol = 1.2e-7 * np.abs(o["hsum"][1:]) + 1e-12 * mag + 1e-30
+    assert np.all(diff <= tol), float((diff / tol).max())
+    for k, col in (("structureness", 0), ("blobness", 1)):
+        a, b = g[k][1:].astype(np.float64), o["eig"][1:, col].astype(np.float64)
+        np.testing.assert_allclose(a, b, rtol=1e-5, atol=1e-30, err_msg=k)
+
+
+# ------------------------------------------------------------------ stages
+@pytest.mark.parametrize("shape,rxy,rz,m", [((24, 48, 40), 4, 3, 0), ((17, 33, 70), 10, 10, 300),
+                                            ((5, 7, 300), 10, 2, 0), ((9, 20, 19), 0, 3, 250),
+                                            ((6, 40, 40), 100, 5, 0), ((40, 3, 2), 2, 25, 0),
+                                            ((1, 1, 1), 3, 3, 0)])
+def test_normalize_bitexact(hd, shape, rxy, rz, m):
+    img = RNG.integers(0, 4000, size=shape, dtype=np.uint16)
+    got = to_np(hd.Normalize3DFilter(img, m, rxy, rz))
+    np.testing.assert_array_equal(got, oracle.normalize3d(img, m, rxy, rz))
+
+
+@pytest.mark.parametrize("shape,s,trunc", [((20, 36, 44), (2.0, 2.0, 3.2, 3.2), 4.0),
+                                           ((33, 17, 65), (2.0, 1.6, 3.2, 2.56), 4.0),
+                                           ((16, 64, 64), (4.0, 1.8, 6.0, 2.7), 2.0),
+                                           ((3, 5, 7), (1.0, 1.0, 1.5, 1.5), 2.0)])
+def test_dog_bitexact(hd, shape, s, trunc):
+    norm = (RNG.standard_normal(shape) * 0.3).astype(np.float32)
+    rz2 = hd.initGaussian3DKernel(*s, truncate=trunc)
+    got = to_np(hd.DoG3DFilter(norm, 1.0))
+    ref, ref_rz2 = oracle.dog3d(norm, *s, truncate=trunc, gamma=1.0)
+    assert rz2 == ref_rz2
+    np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("shape", [(12, 20, 70), (5, 9, 130), (1, 1, 5), (7, 1, 3), (4, 6, 64)])
+def test_hessian_bitexact(hd, shape):
+    dog = (RNG.standard_normal(shape) * 0.01).astype(np.float32)
+    hess, pd = hd.HessianPositiveDefiniteWithElement(dog)
+    rh, rpd = oracle.hessian3d(dog)
+    np.testing.assert_array_equal(to_np(hess), rh)
+    np.testing.assert_array_equal(to_np(pd), rpd)
+
+
+@pytest.mark.parametrize("shape,p", [((10, 20, 30), 0.3), ((4, 5, 6), 0.5), ((1, 1, 64), 0.6),
+                                     ((33, 1, 7), 0.5), ((9, 64, 65), 0.45), ((2, 3, 1), 0.9)])
+def test_ccl_exact(hd, shape, p):
+    for seed in range(3):
+        mask = (np.random.default_rng(seed).random(shape) < p).astype(np.uint8)
+        got = to_np(hd.CCL(mask))
+        np.testing.assert_array_equal(got, oracle.ccl3d(mask))
+
+
+def test_ccl_large_component_and_wrap(hd):
+    mask = np.ones((8, 40, 50), np.uint8)
+    mask[:, :, 0] = 0                  # a plane of background: rows still join through wrap
+    got = to_np(hd.CCL(mask))
+    np.testing.assert_array_equal(got, oracle.ccl3d(mask))
+
+
+@pytest.mark.parametrize("ave", [False, True])
+def test_regions_from_oracle_labels(hd, ave):
+    vol = synth.make_volume("c0", shape=(32, 64, 72), n_blobs=60, seed=3)
+    o = oracle.substack(vol, dict(min_intensity=0, rxy=4, rz=3, sigma_xy1=1.5, sigma_z1=1.5,
+                                  sigma_xy2=2.4, sigma_z2=2.4, truncate=2.0, gamma=1.0,
+                                  is_ave_mode=ave))
+    g = hd.RegionalFeatures(o["labels"], o["hess"], o["norm"], vol, is_ave_mode=ave)
+    if ave:
+        # the sum of normalized intensity: FP64 sums in different order, one rounding each
+        np.testing.assert_allclose(g["norm"][1:], o["regions"]["norm"][1:], rtol=2e-7, atol=1e-6)
+        g = dict(g, norm=o["regions"]["norm"])
+    assert_regions_match(g, o["regions"], o["hess"])
+
+
+def test_region_size_wraps_modulo_65536(hd):
+    D, H, W = 4, 160, 120
+    labels = np.zeros((D, H, W), np.uint32)
+    blk = np.zeros((D, H, W), bool)
+    blk[:, 2:147, 1:121] = True
+    labels[blk] = np.flatnonzero(blk.ravel())[0]
+    hess = RNG.standard_normal((6, D, H, W)).astype(np.float32)
+    norm = RNG.random((D, H, W)).astype(np.float32)
+    img = RNG.integers(0, 60000, (D, H, W), dtype=np.uint16)
+    g = hd.RegionalFeatures(labels, hess, norm, img)
+    o = oracle.regions(labels, hess, norm, img)
+    assert g["count"][1] == 69600 and g["size16"][1] == 69600 % 65536
+    assert_regions_match(g, o, hess)
+
+
+# ------------------------------------------------------------------ whole path
+C0P = dict(min_intensity=0, rxy=10, rz=10, sigma_xy1=2.0, sigma_z1=2.0, sigma_xy2=3.2,
+           sigma_z2=3.2, gamma=1.0)
+
+
+def run_gpu(hd, vol, p, precision="fp32"):
+    params = hd.make_params(vol.shape, p["rxy"], p["rz"], p["min_intensity"], p["sigma_xy1"],
+                            p["sigma_z1"], p["sigma_xy2"], p["sigma_z2"], p.get("truncate", 2.0),
+                            p.get("gamma", 1.0), p.get("is_ave_mode", False), precision)
+    bufs = hd.alloc_intermediates(vol.shape, precision)
+    import torch
+    img = torch.from_numpy(vol).to(hd.device)
+    R = hd.run_device(img, params, bufs)
+    return R, bufs, hd.download_regions()
+
+
+@pytest.mark.parametrize("trunc", [4.0, 2.0])
+def test_pipeline_config0_bitexact(hd, trunc):
+    vol = synth.make_volume("c0")
+    p = dict(C0P, truncate=trunc)
+    o = oracle.substack(vol, p)
+    R, b, g = run_gpu(hd, vol, p)
+    np.testing.assert_array_equal(to_np(b["norm"]), o["norm"])
+    np.testing.assert_array_equal(to_np(b["dog"]), o["dog"])
+    np.testing.assert_array_equal(to_np(b["hess"]), o["hess"])
+    np.testing.assert_array_equal(to_np(b["pd"]), o["pd"])
+    np.testing.assert_array_equal(to_np(b["labels"]), o["labels"])
+    assert R == len(o["regions"]["label"])
+    assert_regions_match(g, o["regions"], o["hess"])
+
+
+def test_pipeline_anisotropic_config1_crop(hd):
+    # the config-1 model and parameters on a 48x160x200 crop
+    vol = synth.make_volume("c1", shape=(48, 160, 200), n_blobs=77, seed=11)
+    p = dict(min_intensity=0, rxy=10, rz=10, sigma_xy1=2.0, sigma_z1=1.6, sigma_xy2=3.2,
+             sigma_z2=2.56, truncate=4.0, gamma=1.0)
+    o = oracle.substack(vol, p)
+    R, b, g = run_gpu(hd, vol, p)
+    np.testing.assert_array_equal(to_np(b["pd"]), o["pd"])
+    np.testing.assert_array_equal(to_np(b["labels"]), o["labels"])
+    assert_regions_match(g, o["regions"], o["hess"])
+
+
+def test_pipeline_is_deterministic(hd):
+    vol = synth.make_volume("c0", shape=(40, 96, 96), n_blobs=150, seed=8)
+    p = dict(C0P, truncate=4.0)
+    _, b1, g1 = run_gpu(hd, vol, p)
+    _, b2, g2 = run_gpu(hd, vol, p)
+    np.testing.assert_array_equal(to_np(b1["labels"]), to_np(b2["labels"]))
+    for k in ("label", "count", "centroid", "norm", "isum"):
+        np.testing.assert_array_equal(g1[k], g2[k])
+
+
+@pytest.mark.parametrize("precision", ["fp16", "bf16"])
+def test_precision_forms_candidate_agreement(hd, precision):
+    vol = synth.make_volume("c0")
+    p = dict(C0P, truncate=4.0)
+    _, b32, _ = run_gpu(hd, vol, p, "fp32")
+    pd32 = to_np(b32["pd"]).astype(bool)
+    _, bl, _ = run_gpu(hd, vol, p, precision)
+    pdl = to_np(bl["pd"]).astype(bool)
+    jac = (pd32 & pdl).sum() / max(1, (pd32 | pdl).sum())
+    print(f"{precision} candidate Jaccard vs fp32: {jac:.6f}")
+    assert jac >= 0.99
+
+
+def test_process_substack_zero_fills_short_tail(hd):
+    import ctypes
+    from hdog import _capi as C
+    vol = synth.make_volume("c0", shape=(32, 64, 64), n_blobs=50, seed=21)
+    valid = 27
+    p = dict(C0P, truncate=2.0)
+    padded = vol.copy()
+    padded[valid:] = 0
+    o = oracle.substack(padded, p)
+    params = hd.make_params(vol.shape, p["rxy"], p["rz"], p["min_intensity"], p["sigma_xy1"],
+                            p["sigma_z1"], p["sigma_xy2"], p["sigma_z2"], 2.0)
+    R = len(o["regions"]["label"])
+    arr = dict(label=np.zeros(R, np.uint32), size=np.zeros(R, np.uint16),
+               voxels=np.zeros(R, np.uint64))
+    reg = C.Regions()
+    reg.capacity = R
+    for k, v in arr.items():
+        setattr(reg, k, v.ctypes.data)
+    src = np.ascontiguousarray(vol[:valid])
+    rc = C.hdog_process_substack(hd._h, hd._stream(), ctypes.byref(params),
+                                 ctypes.c_void_p(src.ctypes.data), valid, ctypes.byref(reg))
+    assert rc == 0, C.hdog_last_error(hd._h)
+    assert reg.count == R
+    np.testing.assert_array_equal(arr["label"][1:], o["regions"]["label"][1:])
+    np.testing.assert_array_equal(arr["voxels"][1:], o["regions"]["count"][1:])
+
+
+def test_errors_raise(hd):
+    from hdog import HDoG, HDoGError
+    with pytest.raises(HDoGError):
+        hd.Normalize3DFilter(np.zeros((4, 4, 4), np.uint16), 0, 500, 1)
+    fresh = HDoG(0)
+    with pytest.raises(HDoGError, match="init_gaussian3d"):
+        fresh.DoG3DFilter(np.zeros((4, 4, 4), np.float32))
+    with pytest.raises(HDoGError):
+        fresh.initGaussian3DKernel(30.0, 1.0, 40.0, 1.0, truncate=4.0)
+    fresh.close()
