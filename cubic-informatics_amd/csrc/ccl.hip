@@ -23,28 +23,43 @@ struct Offsets {
   int n;
 };
 
-__device__ __forceinline__ uint32_t ld_agent(uint32_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// parents only ever decrease, so every chain ends at a root even under concurrent links
+// Parents only ever decrease, so every chain ends at a root.  Plain (cacheable) loads are
+// enough: a stale parent is still an ancestor, two voxels that ever shared an ancestor are
+// in the same set, and every link goes through an atomicMin whose returned value is
+// current, so each retry of unite strictly lowers `a` and the loop ends.  Path halving
+// (x's parent replaced by its grandparent) keeps the trees shallow; the plain store only
+// ever replaces a parent by one of its ancestors, so no union is lost.
+// Only the union kernel may halve paths: in the flatten kernel a halving store could
+// overwrite another voxel's final root with an intermediate ancestor.
 __device__ __forceinline__ uint32_t find_root(uint32_t* L, uint32_t x) {
-  uint32_t p = ld_agent(L + x);
+  uint32_t p = L[x];
   while (p != x) {
+    const uint32_t gp = L[p];
+    if (gp != p) L[x] = gp;
     x = p;
-    p = ld_agent(L + x);
+    p = gp;
   }
   return x;
 }
 
-__device__ void unite(uint32_t* L, uint32_t a, uint32_t b) {
+__device__ __forceinline__ uint32_t find_root_readonly(const uint32_t* L, uint32_t x) {
+  uint32_t p = L[x];
+  while (p != x) {
+    x = p;
+    p = L[x];
+  }
+  return x;
+}
+
+// links the sets of a and b; returns the root of the union at the time of the link
+__device__ uint32_t unite(uint32_t* L, uint32_t a, uint32_t b) {
   while (true) {
     a = find_root(L, a);
     b = find_root(L, b);
-    if (a == b) return;
+    if (a == b) return a;
     if (a < b) { const uint32_t t = a; a = b; b = t; }
     const uint32_t old = atomicMin(L + a, b);
-    if (old == a) return;
+    if (old == a) return b;
     a = old;  // a had been linked meanwhile: continue with its new parent
   }
 }
@@ -59,11 +74,15 @@ __global__ __launch_bounds__(NT) void k_ccl_union(const uint8_t* __restrict__ pd
                                                   i64 N, Offsets offs) {
   for (i64 i = (i64)blockIdx.x * NT + threadIdx.x; i < N; i += (i64)gridDim.x * NT) {
     if (i == 0 || !pd[i]) continue;
+    uint32_t root = (uint32_t)i;  // a representative of i's set, refreshed by each link
 #pragma unroll
     for (int k = 0; k < 13; ++k) {
       if (k < offs.n) {
         const i64 j = i + offs.o[k];
-        if (j < N && pd[j]) unite(L, (uint32_t)i, (uint32_t)j);
+        if (j < N && pd[j]) {
+          // a neighbour whose parent already is our representative is in our set
+          if (L[j] != root) root = unite(L, root, (uint32_t)j);
+        }
       }
     }
   }
@@ -72,7 +91,135 @@ __global__ __launch_bounds__(NT) void k_ccl_union(const uint8_t* __restrict__ pd
 __global__ __launch_bounds__(NT) void k_ccl_flatten(uint32_t* L, i64 N) {
   for (i64 i = (i64)blockIdx.x * NT + threadIdx.x; i < N; i += (i64)gridDim.x * NT) {
     const uint32_t l = L[i];
-    if (l && l != (uint32_t)i) L[i] = find_root(L, l);
+    if (l && l != (uint32_t)i) L[i] = find_root_readonly(L, l);
+  }
+}
+
+// ---- tiled form (width >= 2 and height >= 2) ----------------------------------------
+// Kernel 1 labels each 32 x 8 x 8 tile on its own with a union-find in LDS (links of
+// voxel pairs whose neighbour lies inside the tile and inside the volume, i.e. without
+// wrap); every foreground voxel then points at the global index of its tile-local root,
+// which is the tile-local minimum because local order follows global order.  Kernel 2
+// links, with the global union-find, only the pairs that leave the tile or wrap.  For
+// width, height >= 2 the 13 forward neighbours (dz = 1; dz = 0, dy = 1; dz = dy = 0,
+// dx = 1) are exactly the positive linear offsets of the reference's adjacency.
+constexpr int CX = 32, CY = 8, CZ = 8, CV = CX * CY * CZ, CPT = CV / NT;
+
+__device__ __forceinline__ uint32_t lds_find(const uint32_t* lab, uint32_t x) {
+  uint32_t p = lab[x];
+  while (p != x) {
+    x = p;
+    p = lab[x];
+  }
+  return x;
+}
+
+__device__ __forceinline__ void lds_unite(uint32_t* lab, uint32_t a, uint32_t b) {
+  while (true) {
+    a = lds_find(lab, a);
+    b = lds_find(lab, b);
+    if (a == b) return;
+    if (a < b) { const uint32_t t = a; a = b; b = t; }
+    const uint32_t old = atomicMin(lab + a, b);
+    if (old == a) return;
+    a = old;
+  }
+}
+
+__device__ __forceinline__ void fwd_offset(int k, int& dx, int& dy, int& dz) {
+  if (k < 9) { dz = 1; dy = k / 3 - 1; dx = k % 3 - 1; }
+  else if (k < 12) { dz = 0; dy = 1; dx = k - 10; }
+  else { dz = 0; dy = 0; dx = 1; }
+}
+
+__global__ __launch_bounds__(NT) void k_ccl_local(const uint8_t* __restrict__ pd,
+                                                  uint32_t* __restrict__ L, int W, int H, int D,
+                                                  int nbx, int nby) {
+  constexpr uint32_t NONE = 0xFFFFFFFFu;
+  __shared__ uint32_t lab[CV];
+  unsigned b = blockIdx.x;
+  const int bx = b % nbx; b /= nbx;
+  const int by = b % nby; b /= nby;
+  const int x0 = bx * CX, y0 = by * CY, z0 = (int)b * CZ;
+  const i64 WH = (i64)W * H;
+  uint8_t fg[CPT];
+#pragma unroll
+  for (int k = 0; k < CPT; ++k) {
+    const int v = threadIdx.x + k * NT;
+    const int gx = x0 + (v & (CX - 1)), gy = y0 + ((v / CX) & (CY - 1)), gz = z0 + v / (CX * CY);
+    const i64 g = (i64)gz * WH + (i64)gy * W + gx;
+    fg[k] = (gx < W && gy < H && gz < D && g > 0) ? pd[g] : 0;
+  }
+#pragma unroll
+  for (int k = 0; k < CPT; ++k) {
+    const int v = threadIdx.x + k * NT;
+    lab[v] = fg[k] ? (uint32_t)v : NONE;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < CPT; ++k) {
+    if (!fg[k]) continue;
+    const int v = threadIdx.x + k * NT;
+    const int lx = v & (CX - 1), ly = (v / CX) & (CY - 1), lz = v / (CX * CY);
+#pragma unroll
+    for (int q = 0; q < 13; ++q) {
+      int dx, dy, dz;
+      fwd_offset(q, dx, dy, dz);
+      const int nx = lx + dx, ny = ly + dy, nz = lz + dz;
+      if (nx < 0 || nx >= CX || ny < 0 || ny >= CY || nz >= CZ) continue;
+      const int u = nx + CX * (ny + CY * nz);
+      if (lab[u] != NONE) lds_unite(lab, (uint32_t)v, (uint32_t)u);  // outside-volume = NONE
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < CPT; ++k) {
+    const int v = threadIdx.x + k * NT;
+    const int gx = x0 + (v & (CX - 1)), gy = y0 + ((v / CX) & (CY - 1)), gz = z0 + v / (CX * CY);
+    if (gx >= W || gy >= H || gz >= D) continue;
+    const i64 g = (i64)gz * WH + (i64)gy * W + gx;
+    uint32_t out = 0u;
+    if (fg[k]) {
+      const uint32_t r = lds_find(lab, (uint32_t)v);
+      const int rx = x0 + (int)(r & (CX - 1)), ry = y0 + (int)((r / CX) & (CY - 1)),
+                rz = z0 + (int)(r / (CX * CY));
+      out = (uint32_t)((i64)rz * WH + (i64)ry * W + rx);
+    }
+    L[g] = out;
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_ccl_merge(const uint8_t* __restrict__ pd, uint32_t* L,
+                                                  int W, int H, int D, int nbx, int nby) {
+  unsigned b = blockIdx.x;
+  const int bx = b % nbx; b /= nbx;
+  const int by = b % nby; b /= nby;
+  const int x0 = bx * CX, y0 = by * CY, z0 = (int)b * CZ;
+  const i64 WH = (i64)W * H, N = WH * D;
+#pragma unroll
+  for (int k = 0; k < CPT; ++k) {
+    const int v = threadIdx.x + k * NT;
+    const int lx = v & (CX - 1), ly = (v / CX) & (CY - 1), lz = v / (CX * CY);
+    const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
+    if (gx >= W || gy >= H || gz >= D) continue;
+    // interior: every forward neighbour is inside the tile and the volume (linked locally)
+    if (lx >= 1 && lx <= CX - 2 && ly >= 1 && ly <= CY - 2 && lz <= CZ - 2 && gx >= 1 &&
+        gx <= W - 2 && gy >= 1 && gy <= H - 2 && gz <= D - 2)
+      continue;
+    const i64 g = (i64)gz * WH + (i64)gy * W + gx;
+    if (g == 0 || !pd[g]) continue;
+    uint32_t root = L[g];
+#pragma unroll
+    for (int q = 0; q < 13; ++q) {
+      int dx, dy, dz;
+      fwd_offset(q, dx, dy, dz);
+      const int nx = lx + dx, ny = ly + dy, nz = lz + dz;
+      const bool inside_volume = gx + dx >= 0 && gx + dx < W && gy + dy >= 0 && gy + dy < H &&
+                                 gz + dz < D;
+      if (inside_volume && nx >= 0 && nx < CX && ny >= 0 && ny < CY && nz < CZ) continue;
+      const i64 j = g + (i64)dz * WH + (i64)dy * W + dx;  // linear offset, wraps like the reference
+      if (j < N && pd[j] && L[j] != root) root = unite(L, root, (uint32_t)j);
+    }
   }
 }
 
@@ -98,13 +245,26 @@ hipError_t launch_ccl(const uint8_t* pd, uint32_t* labels, int W, int H, int D, 
     offs.n = n;
   }
   const i64 nb = std::min<i64>((N + NT - 1) / NT, 1 << 20);
-  hipLaunchKernelGGL(k_ccl_init, dim3((unsigned)nb), dim3(NT), 0, s, pd, labels, N);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  mark("ccl/init", s);
-  hipLaunchKernelGGL(k_ccl_union, dim3((unsigned)nb), dim3(NT), 0, s, pd, labels, N, offs);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  mark("ccl/union", s);
+  hipError_t e;
+  if (W >= 2 && H >= 2) {
+    const int nbx = (W + CX - 1) / CX, nby = (H + CY - 1) / CY, nbz = (D + CZ - 1) / CZ;
+    const i64 nt = (i64)nbx * nby * nbz;
+    hipLaunchKernelGGL(k_ccl_local, dim3((unsigned)nt), dim3(NT), 0, s, pd, labels, W, H, D, nbx,
+                       nby);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    mark("ccl/local", s);
+    hipLaunchKernelGGL(k_ccl_merge, dim3((unsigned)nt), dim3(NT), 0, s, pd, labels, W, H, D, nbx,
+                       nby);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    mark("ccl/merge", s);
+  } else {
+    hipLaunchKernelGGL(k_ccl_init, dim3((unsigned)nb), dim3(NT), 0, s, pd, labels, N);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    mark("ccl/init", s);
+    hipLaunchKernelGGL(k_ccl_union, dim3((unsigned)nb), dim3(NT), 0, s, pd, labels, N, offs);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    mark("ccl/union", s);
+  }
   hipLaunchKernelGGL(k_ccl_flatten, dim3((unsigned)nb), dim3(NT), 0, s, labels, N);
   mark("ccl/flatten", s);
   return hipGetLastError();

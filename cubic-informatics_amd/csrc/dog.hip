@@ -9,6 +9,11 @@
 // the z pass forms the difference directly, so `norm` is read once and the two
 // intermediate volumes of the reference become one interleaved stream.  Intermediates
 // stay FP32 whatever the storage type of norm / dog.
+//
+// Radius pairs (r1, r2) that occur in practice have compile-time kernels: weights live in
+// scalar registers, and a thread produces RUN consecutive outputs from a register window
+// of RUN + 2*r2 values (x pass: 4 outputs from 128-bit LDS reads; y / z passes: 16
+// outputs).  Other radii use the generic loops.
 #include "hdog_internal.hpp"
 
 namespace hdog {
@@ -16,6 +21,7 @@ namespace {
 
 constexpr int NT = 256;
 
+// ------------------------------------------------------------------- generic radii
 template <int ROWS, typename TS>
 __global__ __launch_bounds__(NT) void k_gauss_x(const TS* __restrict__ src, float2* __restrict__ dst,
                                                 const float* __restrict__ w1,
@@ -27,14 +33,15 @@ __global__ __launch_bounds__(NT) void k_gauss_x(const TS* __restrict__ src, floa
   const int ct = blockIdx.x % nct;
   const i64 row0 = (i64)(blockIdx.x / nct) * ROWS;
   const int c0 = ct * C;
-  for (int e = threadIdx.x; e < ROWS * span; e += NT) {
-    const int row = e / span, p = e % span;
-    const i64 grow = row0 + row;
-    const int x = c0 - R + p;
-    float v = 0.0f;
-    if (grow < nrows && x >= 0 && x < W) v = ld(src, grow * W + x);
-    smf[e] = v;
-  }
+  stage_tile<8, NT, float>(
+      ROWS * span,
+      [&](int e) -> float {
+        const int row = e / span, p = e % span;
+        const i64 grow = row0 + row;
+        const int x = c0 - R + p;
+        return (grow < nrows && x >= 0 && x < W) ? ld(src, grow * W + x) : 0.0f;
+      },
+      [&](int e, float v) { smf[e] = v; });
   __syncthreads();
   for (int e = threadIdx.x; e < ROWS * C; e += NT) {
     const int row = e / C, i = e % C, x = c0 + i;
@@ -48,7 +55,6 @@ __global__ __launch_bounds__(NT) void k_gauss_x(const TS* __restrict__ src, floa
   }
 }
 
-// y or z pass on the (G1, G2) pair with zero padding; LAST forms the scaled difference.
 template <int XW, bool LAST, typename TS>
 __global__ __launch_bounds__(NT) void k_gauss_line(const float2* __restrict__ src,
                                                    float2* __restrict__ dst, TS* __restrict__ dog,
@@ -64,12 +70,13 @@ __global__ __launch_bounds__(NT) void k_gauss_line(const float2* __restrict__ sr
   const int ct = b % nct; b /= nct;
   const i64 base = (i64)b * so;
   const int x0 = xt * XW, c0 = ct * C;
-  for (int e = threadIdx.x; e < span * XW; e += NT) {
-    const int p = e / XW, xl = e % XW, x = x0 + xl, c = c0 - R + p;
-    float2 v = make_float2(0.0f, 0.0f);
-    if (x < W && c >= 0 && c < L) v = src[base + (i64)c * sa + x];
-    smf2[e] = v;
-  }
+  stage_tile<8, NT, float2>(
+      span * XW,
+      [&](int e) -> float2 {
+        const int p = e / XW, x = x0 + e % XW, c = c0 - R + p;
+        return (x < W && c >= 0 && c < L) ? src[base + (i64)c * sa + x] : make_float2(0.0f, 0.0f);
+      },
+      [&](int e, float2 v) { smf2[e] = v; });
   __syncthreads();
   for (int e = threadIdx.x; e < C * XW; e += NT) {
     const int i = e / XW, xl = e % XW, c = c0 + i, x = x0 + xl;
@@ -86,10 +93,150 @@ __global__ __launch_bounds__(NT) void k_gauss_line(const float2* __restrict__ sr
   }
 }
 
+// ------------------------------------------------------------- compile-time radii
+// x pass: tile ROWS rows x (64*4) outputs; thread (row, seg) makes 4 consecutive outputs.
+// LDS row position p <-> x = c0 - RP + p with RP = R rounded up to 4 (16-byte aligned
+// window reads).
+template <int R1, int R2, typename TS>
+__global__ __launch_bounds__(NT) void k_gauss_x_r(const TS* __restrict__ src,
+                                                  float2* __restrict__ dst,
+                                                  const float* __restrict__ w1,
+                                                  const float* __restrict__ w2, int W, i64 nrows,
+                                                  int nct) {
+  constexpr int ROWS = 4, RUN = 4, C = NT / ROWS * RUN;
+  constexpr int R = R1 > R2 ? R1 : R2;
+  constexpr int RP = (R + 3) / 4 * 4;
+  constexpr int SPAN = C + 2 * RP;
+  constexpr int NV = (RUN + 2 * RP) / 4;  // float4 reads per window
+  __shared__ __align__(16) float T[ROWS * SPAN];
+  const int ct = blockIdx.x % nct;
+  const i64 row0 = (i64)(blockIdx.x / nct) * ROWS;
+  const int c0 = ct * C;
+  stage_tile<8, NT, float>(
+      ROWS * SPAN,
+      [&](int e) -> float {
+        const int row = e / SPAN, p = e % SPAN;
+        const i64 grow = row0 + row;
+        const int x = c0 - RP + p;
+        return (grow < nrows && x >= 0 && x < W) ? ld(src, grow * W + x) : 0.0f;
+      },
+      [&](int e, float v) { T[e] = v; });
+  __syncthreads();
+  const int row = threadIdx.x / (NT / ROWS), seg = threadIdx.x % (NT / ROWS);
+  const i64 grow = row0 + row;
+  const int xs = c0 + seg * RUN;
+  if (grow >= nrows || xs >= W) return;
+  float win[4 * NV];
+  const float4* T4 = reinterpret_cast<const float4*>(T + row * SPAN + seg * RUN);
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    const float4 q = T4[v];
+    win[4 * v] = q.x; win[4 * v + 1] = q.y; win[4 * v + 2] = q.z; win[4 * v + 3] = q.w;
+  }
+#pragma unroll
+  for (int o = 0; o < RUN; ++o) {
+    if (xs + o >= W) break;
+    // window index of x = xs + o + j is o + j + RP
+    float s1 = 0.0f, s2 = 0.0f;
+#pragma unroll
+    for (int j = -R1; j <= R1; ++j) s1 = fmaf(w1[R1 - j], win[o + j + RP], s1);
+#pragma unroll
+    for (int j = -R2; j <= R2; ++j) s2 = fmaf(w2[R2 - j], win[o + j + RP], s2);
+    dst[grow * W + xs + o] = make_float2(s1, s2);
+  }
+}
+
+template <int R1, int R2, bool LAST, typename TS>
+__global__ __launch_bounds__(NT) void k_gauss_line_r(const float2* __restrict__ src,
+                                                     float2* __restrict__ dst,
+                                                     TS* __restrict__ dog,
+                                                     const float* __restrict__ w1,
+                                                     const float* __restrict__ w2, float nz, int W,
+                                                     int L, i64 sa, i64 so, int nxt, int nct) {
+  constexpr int XW = 32, RUN = 16, NSEG = NT / XW, C = NSEG * RUN;
+  constexpr int R = R1 > R2 ? R1 : R2;
+  constexpr int SPAN = C + 2 * R;
+  __shared__ float2 T[SPAN * XW];
+  unsigned b = blockIdx.x;
+  const int xt = b % nxt; b /= nxt;
+  const int ct = b % nct; b /= nct;
+  const i64 base = (i64)b * so;
+  const int x0 = xt * XW, c0 = ct * C;
+  stage_tile<8, NT, float2>(
+      SPAN * XW,
+      [&](int e) -> float2 {
+        const int p = e / XW, x = x0 + e % XW, c = c0 - R + p;
+        return (x < W && c >= 0 && c < L) ? src[base + (i64)c * sa + x] : make_float2(0.0f, 0.0f);
+      },
+      [&](int e, float2 v) { T[e] = v; });
+  __syncthreads();
+  const int xl = threadIdx.x % XW, seg = threadIdx.x / XW;
+  const int x = x0 + xl;
+  if (x >= W) return;
+  float2 win[RUN + 2 * R];
+#pragma unroll
+  for (int q = 0; q < RUN + 2 * R; ++q) win[q] = T[(seg * RUN + q) * XW + xl];
+#pragma unroll
+  for (int o = 0; o < RUN; ++o) {
+    const int c = c0 + seg * RUN + o;
+    if (c >= L) break;
+    // window index of position c + j is o + j + R
+    float s1 = 0.0f, s2 = 0.0f;
+#pragma unroll
+    for (int j = -R1; j <= R1; ++j) s1 = fmaf(w1[R1 - j], win[o + j + R].x, s1);
+#pragma unroll
+    for (int j = -R2; j <= R2; ++j) s2 = fmaf(w2[R2 - j], win[o + j + R].y, s2);
+    const i64 idx = base + (i64)c * sa + x;
+    if (LAST)
+      dog[idx] = cvt<TS>((s1 - s2) * nz);
+    else
+      dst[idx] = make_float2(s1, s2);
+  }
+}
+
+// (r1, r2) pairs with compile-time kernels: truncation 4 and 2 of the benchmark sigmas
+// (2.0 / 3.2, 1.6 / 2.56) and of the reference's example sigmas (4.0 / 6.0, 1.8 / 2.7).
+#define HDOG_GAUSS_PAIRS(X) X(8, 13) X(6, 10) X(4, 6) X(3, 5) X(8, 12) X(4, 5) X(7, 11)
+
+template <typename TS>
+hipError_t gauss_x(const TS* src, float2* dst, const float* w1, const float* w2, int r1, int r2,
+                   int W, i64 nrows, hipStream_t s) {
+#define X(A, B)                                                                              \
+  if (r1 == A && r2 == B) {                                                                  \
+    constexpr int C = NT / 4 * 4;                                                            \
+    const int nct = (W + C - 1) / C;                                                         \
+    const i64 nblk = ((nrows + 3) / 4) * nct;                                                \
+    hipLaunchKernelGGL((k_gauss_x_r<A, B, TS>), dim3((unsigned)nblk), dim3(NT), 0, s, src,   \
+                       dst, w1, w2, W, nrows, nct);                                          \
+    return hipGetLastError();                                                                \
+  }
+  HDOG_GAUSS_PAIRS(X)
+#undef X
+  constexpr int ROWS = 4;
+  const int C = 256;
+  const int nct = (W + C - 1) / C;
+  const i64 nblk = ((nrows + ROWS - 1) / ROWS) * nct;
+  const size_t lds = (size_t)ROWS * (C + 2 * std::max(r1, r2)) * 4;
+  hipLaunchKernelGGL((k_gauss_x<ROWS, TS>), dim3((unsigned)nblk), dim3(NT), lds, s, src, dst, w1,
+                     w2, r1, r2, W, nrows, C, nct);
+  return hipGetLastError();
+}
+
 template <bool LAST, typename TS>
 hipError_t gauss_line(const float2* src, float2* dst, TS* dog, const float* w1, const float* w2,
                       int r1, int r2, float nz, int W, int L, i64 sa, i64 so, i64 n_other,
                       hipStream_t s) {
+#define X(A, B)                                                                               \
+  if (r1 == A && r2 == B) {                                                                   \
+    constexpr int C = NT / 32 * 16;                                                           \
+    const int nxt = (W + 31) / 32, nct = (L + C - 1) / C;                                     \
+    const i64 nblk = (i64)nxt * nct * n_other;                                                \
+    hipLaunchKernelGGL((k_gauss_line_r<A, B, LAST, TS>), dim3((unsigned)nblk), dim3(NT), 0, s, \
+                       src, dst, dog, w1, w2, nz, W, L, sa, so, nxt, nct);                    \
+    return hipGetLastError();                                                                 \
+  }
+  HDOG_GAUSS_PAIRS(X)
+#undef X
   const int R = std::max(r1, r2);
   int XW = R > 24 ? 32 : 64;
   int C = 64;
@@ -117,18 +264,10 @@ hipError_t launch_dog(const TS* norm, TS* dog, void* tmp1, void* tmp2, const flo
   const float* wxy2 = weights + 2 * kMaxTaps;
   const float* wz2 = weights + 3 * kMaxTaps;
   hipError_t e;
-  {
-    constexpr int ROWS = 4;
-    const int C = 256;
-    const int nct = (W + C - 1) / C;
-    const i64 nrows = (i64)H * D;
-    const i64 nblk = ((nrows + ROWS - 1) / ROWS) * nct;
-    const size_t lds = (size_t)ROWS * (C + 2 * std::max(r[0], r[2])) * 4;
-    hipLaunchKernelGGL((k_gauss_x<ROWS, TS>), dim3((unsigned)nblk), dim3(NT), lds, s, norm,
-                       (float2*)tmp1, wxy1, wxy2, r[0], r[2], W, nrows, C, nct);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    mark("dog/gauss_x", s);
-  }
+  if ((e = gauss_x<TS>(norm, (float2*)tmp1, wxy1, wxy2, r[0], r[2], W, (i64)H * D, s)) !=
+      hipSuccess)
+    return e;
+  mark("dog/gauss_x", s);
   if ((e = gauss_line<false, TS>((const float2*)tmp1, (float2*)tmp2, nullptr, wxy1, wxy2, r[0],
                                  r[2], 0.0f, W, H, W, WH, D, s)) != hipSuccess)
     return e;

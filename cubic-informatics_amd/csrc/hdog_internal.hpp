@@ -36,6 +36,26 @@ __host__ __device__ __forceinline__ int clampi(int v, int lo, int hi) {
   return v < lo ? lo : (v > hi ? hi : v);
 }
 
+// ---- tile staging -----------------------------------------------------------------
+// Copies n elements global -> LDS with B independent loads in flight per lane before the
+// first LDS store (a plain strided loop waits for each load in turn).
+template <int B, int NTH, typename T, typename LD, typename ST>
+__device__ __forceinline__ void stage_tile(int n, LD&& ldf, ST&& stf) {
+  for (int base = (int)threadIdx.x; base < n; base += B * NTH) {
+    T v[B];
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      const int e = base + k * NTH;
+      if (e < n) v[k] = ldf(e);
+    }
+#pragma unroll
+    for (int k = 0; k < B; ++k) {
+      const int e = base + k * NTH;
+      if (e < n) stf(e, v[k]);
+    }
+  }
+}
+
 // ---- per-region accumulator (one 128-byte line per region) -----------------------
 struct alignas(128) RegionAcc {
   unsigned long long sx, sy, sz, isum;

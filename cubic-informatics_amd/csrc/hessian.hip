@@ -7,11 +7,11 @@
 // determinant < 0, evaluated in C parse order without contraction (the library is
 // built with -ffp-contract=off).
 //
-// MI355X design: one pass over `dog` instead of nine.  A work-group stages a
-// (64+4) x (4+4) x (4+4) tile of dog in LDS, forms the first derivatives Hx, Hy, Hz on
-// the halo positions the second pass needs (zero outside the volume, as the reference's
-// second pass pads its own input), then writes the six elements (SoA, pitch N) and the
-// mask with coalesced stores.
+// MI355X design: one pass over `dog` instead of nine.  A work-group streams a 64 x 8
+// column of slices through LDS rings (see k_hessian), forms the first derivatives Hx, Hy,
+// Hz on the halo positions the second pass needs (zero outside the volume, as the
+// reference's second pass pads its own input), then writes the six elements (SoA,
+// pitch N) and the mask with coalesced stores.
 //
 // Reduced-precision storage (FP16 / BF16): the elements are computed in FP32 from the
 // stored DoG and rounded only on the store; the candidate test uses the FP32 values, and
@@ -22,7 +22,7 @@
 namespace hdog {
 namespace {
 
-constexpr int TX = 64, TY = 4, TZ = 4;
+constexpr int TX = 64, TY = 8, ZC = 16;
 constexpr int NT = 256;
 
 __device__ __forceinline__ float g2(float hi, float lo) {
@@ -56,81 +56,124 @@ __device__ __forceinline__ uint8_t negdef_band(float xx, float xy, float xz, flo
   return (xx < 0 && m2 > 0 && m3 < 0) ? 1 : 0;
 }
 
+// z-streaming form.  A work-group owns a TX x TY column of outputs over ZC consecutive
+// slices and walks s = z0-2 .. z1+1 with three-slot LDS rings:
+//   step s: load dog(s) (xy halo 2)  ->  Hx(s-1), Hy(s-1) from dog(s-1) (xy halo 1),
+//           Hz(s-1) from dog(s) - dog(s-2)  ->  outputs of slice z = s-2, which need
+//           Hx / Hy at z-1..z+1 and Hz at z-1, z+1.
+// Every dog value is read once per column (plus the xy halo, served by L2) instead of
+// (TZ+4)/TZ times; first derivatives outside the volume are zero as in the reference's
+// second pass.
 template <typename TS, bool BAND>
 __global__ __launch_bounds__(NT) void k_hessian(const TS* __restrict__ dog, TS* __restrict__ hess,
                                                 uint8_t* __restrict__ pd, int W, int H, int D,
                                                 int nbx, int nby) {
-  __shared__ float sD[TZ + 4][TY + 4][TX + 4];
-  __shared__ float sHx[TZ + 2][TY + 2][TX + 2];
-  __shared__ float sHy[TZ + 2][TY + 2][TX];
-  __shared__ float sHz[TZ + 2][TY][TX];
+  constexpr int DX = TX + 4, DY = TY + 4;        // dog slice with halo 2
+  constexpr int XX = TX + 2, XY = TY + 2;        // Hx slice with halo 1 in x and y
+  constexpr int YX = TX, YY = TY + 2;            // Hy slice with halo 1 in y
+  __shared__ float sD[3][DY * DX];
+  __shared__ float sHx[3][XY * XX];
+  __shared__ float sHy[3][YY * YX];
+  __shared__ float sHz[3][TY * TX];
   unsigned b = blockIdx.x;
   const int bx = b % nbx; b /= nbx;
   const int by = b % nby; b /= nby;
-  const int x0 = bx * TX, y0 = by * TY, z0 = (int)b * TZ;
+  const int x0 = bx * TX, y0 = by * TY;
+  const int z0 = (int)b * ZC, z1 = min(z0 + ZC, D);
   const i64 WH = (i64)W * H;
   const i64 N = WH * D;
-
-  constexpr int ND = (TZ + 4) * (TY + 4) * (TX + 4);
-  for (int e = threadIdx.x; e < ND; e += NT) {
-    const int lx = e % (TX + 4), t = e / (TX + 4), ly = t % (TY + 4), lz = t / (TY + 4);
-    const int gx = x0 - 2 + lx, gy = y0 - 2 + ly, gz = z0 - 2 + lz;
-    float v = 0.0f;
-    if (gx >= 0 && gx < W && gy >= 0 && gy < H && gz >= 0 && gz < D)
-      v = ld(dog, gz * WH + (i64)gy * W + gx);
-    sD[lz][ly][lx] = v;
-  }
-  __syncthreads();
-  // first derivatives on the halo positions; zero outside the volume
-  constexpr int NHX = (TZ + 2) * (TY + 2) * (TX + 2);
-  for (int e = threadIdx.x; e < NHX; e += NT) {
-    const int lx = e % (TX + 2), t = e / (TX + 2), ly = t % (TY + 2), lz = t / (TY + 2);
-    const int gx = x0 - 1 + lx, gy = y0 - 1 + ly, gz = z0 - 1 + lz;
-    float v = 0.0f;
-    if (gx >= 0 && gx < W && gy >= 0 && gy < H && gz >= 0 && gz < D)
-      v = g2(sD[lz + 1][ly + 1][lx + 2], sD[lz + 1][ly + 1][lx]);
-    sHx[lz][ly][lx] = v;
-  }
-  constexpr int NHY = (TZ + 2) * (TY + 2) * TX;
-  for (int e = threadIdx.x; e < NHY; e += NT) {
-    const int lx = e % TX, t = e / TX, ly = t % (TY + 2), lz = t / (TY + 2);
-    const int gx = x0 + lx, gy = y0 - 1 + ly, gz = z0 - 1 + lz;
-    float v = 0.0f;
-    if (gx < W && gy >= 0 && gy < H && gz >= 0 && gz < D)
-      v = g2(sD[lz + 1][ly + 2][lx + 2], sD[lz + 1][ly][lx + 2]);
-    sHy[lz][ly][lx] = v;
-  }
-  constexpr int NHZ = (TZ + 2) * TY * TX;
-  for (int e = threadIdx.x; e < NHZ; e += NT) {
-    const int lx = e % TX, t = e / TX, ly = t % TY, lz = t / TY;
-    const int gx = x0 + lx, gy = y0 + ly, gz = z0 - 1 + lz;
-    float v = 0.0f;
-    if (gx < W && gy < H && gz >= 0 && gz < D)
-      v = g2(sD[lz + 2][ly + 2][lx + 2], sD[lz][ly + 2][lx + 2]);
-    sHz[lz][ly][lx] = v;
-  }
-  __syncthreads();
-  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
-  const int gx = x0 + tx, gy = y0 + ty;
-  if (gx >= W || gy >= H) return;
+  const int tx = threadIdx.x % TX, ty0 = threadIdx.x / TX;  // outputs (tx, ty0 + k*(NT/TX))
+  for (int s = z0 - 2; s <= z1 + 1; ++s) {
+    // 1. dog(s) -> ring slot
+    {
+      float* d = sD[(s + 3) % 3];
+      const bool zin = s >= 0 && s < D;
+      stage_tile<4, NT, float>(
+          DY * DX,
+          [&](int e) -> float {
+            const int lx = e % DX, ly = e / DX;
+            const int gx = x0 - 2 + lx, gy = y0 - 2 + ly;
+            return (zin && gx >= 0 && gx < W && gy >= 0 && gy < H)
+                       ? ld(dog, (i64)s * WH + (i64)gy * W + gx)
+                       : 0.0f;
+          },
+          [&](int e, float v) { d[e] = v; });
+    }
+    __syncthreads();
+    // 2. first derivatives of slice s-1
+    {
+      const int zc = s - 1;
+      const bool zin = zc >= 0 && zc < D;
+      const float* dc = sD[(zc + 3) % 3];  // dog(s-1)
+      const float* dn = sD[(s + 3) % 3];   // dog(s)
+      const float* dp = sD[(s + 1) % 3];   // dog(s-2)
+      float* hx = sHx[(zc + 3) % 3];
+      float* hy = sHy[(zc + 3) % 3];
+      float* hz = sHz[(zc + 3) % 3];
+      for (int e = threadIdx.x; e < XY * XX; e += NT) {
+        const int lx = e % XX, ly = e / XX;
+        const int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
+        float v = 0.0f;
+        if (zin && gx >= 0 && gx < W && gy >= 0 && gy < H)
+          v = g2(dc[(ly + 1) * DX + lx + 2], dc[(ly + 1) * DX + lx]);
+        hx[e] = v;
+      }
+      for (int e = threadIdx.x; e < YY * YX; e += NT) {
+        const int lx = e % YX, ly = e / YX;
+        const int gx = x0 + lx, gy = y0 - 1 + ly;
+        float v = 0.0f;
+        if (zin && gx < W && gy >= 0 && gy < H)
+          v = g2(dc[(ly + 2) * DX + lx + 2], dc[ly * DX + lx + 2]);
+        hy[e] = v;
+      }
+      for (int e = threadIdx.x; e < TY * TX; e += NT) {
+        const int lx = e % TX, ly = e / TX;
+        const int gx = x0 + lx, gy = y0 + ly;
+        float v = 0.0f;
+        if (zin && gx < W && gy < H)
+          v = g2(dn[(ly + 2) * DX + lx + 2], dp[(ly + 2) * DX + lx + 2]);
+        hz[e] = v;
+      }
+    }
+    __syncthreads();
+    // 3. outputs of slice z = s - 2
+    const int z = s - 2;
+    if (z >= z0 && z < z1) {
+      const float* hxm = sHx[(z + 2) % 3];  // Hx(z-1)
+      const float* hx0 = sHx[(z + 3) % 3];  // Hx(z)
+      const float* hxp = sHx[(z + 4) % 3];  // Hx(z+1)
+      const float* hym = sHy[(z + 2) % 3];
+      const float* hy0 = sHy[(z + 3) % 3];
+      const float* hyp = sHy[(z + 4) % 3];
+      const float* hzm = sHz[(z + 2) % 3];
+      const float* hzp = sHz[(z + 4) % 3];
+      const int gx = x0 + tx;
 #pragma unroll
-  for (int tz = 0; tz < TZ; ++tz) {
-    const int gz = z0 + tz;
-    if (gz >= D) break;
-    const float xx = g2(sHx[tz + 1][ty + 1][tx + 2], sHx[tz + 1][ty + 1][tx]);
-    const float xy = g2(sHx[tz + 1][ty + 2][tx + 1], sHx[tz + 1][ty][tx + 1]);
-    const float xz = g2(sHx[tz + 2][ty + 1][tx + 1], sHx[tz][ty + 1][tx + 1]);
-    const float yy = g2(sHy[tz + 1][ty + 2][tx], sHy[tz + 1][ty][tx]);
-    const float yz = g2(sHy[tz + 2][ty + 1][tx], sHy[tz][ty + 1][tx]);
-    const float zz = g2(sHz[tz + 2][ty][tx], sHz[tz][ty][tx]);
-    const i64 idx = gz * WH + (i64)gy * W + gx;
-    hess[idx] = cvt<TS>(xx);
-    hess[N + idx] = cvt<TS>(xy);
-    hess[2 * N + idx] = cvt<TS>(xz);
-    hess[3 * N + idx] = cvt<TS>(yy);
-    hess[4 * N + idx] = cvt<TS>(yz);
-    hess[5 * N + idx] = cvt<TS>(zz);
-    pd[idx] = BAND ? negdef_band(xx, xy, xz, yy, yz, zz) : negdef32(xx, xy, xz, yy, yz, zz);
+      for (int k = 0; k < TY / (NT / TX); ++k) {
+        const int ty = ty0 + k * (NT / TX);
+        const int gy = y0 + ty;
+        if (gx < W && gy < H) {
+          const int cx = (ty + 1) * XX + tx + 1;  // Hx centre
+          const int cy = (ty + 1) * YX + tx;      // Hy centre
+          const int cz = ty * TX + tx;            // Hz centre
+          const float xx = g2(hx0[cx + 1], hx0[cx - 1]);
+          const float xy = g2(hx0[cx + XX], hx0[cx - XX]);
+          const float xz = g2(hxp[cx], hxm[cx]);
+          const float yy = g2(hy0[cy + YX], hy0[cy - YX]);
+          const float yz = g2(hyp[cy], hym[cy]);
+          const float zz = g2(hzp[cz], hzm[cz]);
+          const i64 idx = (i64)z * WH + (i64)gy * W + gx;
+          hess[idx] = cvt<TS>(xx);
+          hess[N + idx] = cvt<TS>(xy);
+          hess[2 * N + idx] = cvt<TS>(xz);
+          hess[3 * N + idx] = cvt<TS>(yy);
+          hess[4 * N + idx] = cvt<TS>(yz);
+          hess[5 * N + idx] = cvt<TS>(zz);
+          pd[idx] = BAND ? negdef_band(xx, xy, xz, yy, yz, zz) : negdef32(xx, xy, xz, yy, yz, zz);
+        }
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -139,7 +182,7 @@ __global__ __launch_bounds__(NT) void k_hessian(const TS* __restrict__ dog, TS* 
 template <typename TS>
 hipError_t launch_hessian(const TS* dog, TS* hess, uint8_t* pd, int W, int H, int D,
                           hipStream_t s) {
-  const int nbx = (W + TX - 1) / TX, nby = (H + TY - 1) / TY, nbz = (D + TZ - 1) / TZ;
+  const int nbx = (W + TX - 1) / TX, nby = (H + TY - 1) / TY, nbz = (D + ZC - 1) / ZC;
   const i64 nblk = (i64)nbx * nby * nbz;
   constexpr bool band = !std::is_same<TS, float>::value;
   hipLaunchKernelGGL((k_hessian<TS, band>), dim3((unsigned)nblk), dim3(NT), 0, s, dog, hess, pd,

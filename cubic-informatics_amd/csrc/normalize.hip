@@ -12,10 +12,13 @@
 //   * Each 1-D min/max pass is van Herk / Gil-Werman (3 packed mins per output for any
 //     radius) on an LDS tile; tiles are x-contiguous so every HBM access coalesces.
 //   * Min/max over a clamped box is separable and exact in any order, so the passes run
-//     z, y, x and the x pass is fused with the x pass of the box filter (integer-valued
-//     sums, exact in any order).  The y and z box passes sum FP32 values and keep the
-//     reference's sequential order (c-r .. c+r) for bit-exactness; the z pass is fused
-//     with the normalization.
+//     z, y, x and the x pass is fused with the x pass of the box filter (sums of
+//     integers, exact in any order, so a sliding sum is used).  The y and z box passes
+//     sum FP32 values and keep the reference's sequential order (c-r .. c+r); the z pass
+//     is fused with the normalization.
+//   * The y / z box passes have compile-time-radius variants: a thread produces RUN
+//     consecutive outputs from a register window of RUN + 2R LDS values (2.25 LDS reads
+//     per output at R = 10 instead of 21); other radii use a generic loop.
 #include "hdog_internal.hpp"
 
 namespace hdog {
@@ -30,8 +33,26 @@ __device__ __forceinline__ uint32_t pkmin(uint32_t a, uint32_t b) {
 }
 __device__ __forceinline__ uint32_t pack(uint32_t v) { return v | ((65535u - v) << 16); }
 
+// van Herk prefix (P) / suffix (in place over A) minima over blocks of w along one LDS
+// line whose consecutive elements are `stride` words apart.
+__device__ __forceinline__ void vh_block(uint32_t* A, uint32_t* P, int p0, int p1, int stride) {
+  uint32_t m = A[p0 * stride];
+  P[p0 * stride] = m;
+#pragma unroll 4
+  for (int p = p0 + 1; p < p1; ++p) {
+    m = pkmin(m, A[p * stride]);
+    P[p * stride] = m;
+  }
+  m = A[(p1 - 1) * stride];
+#pragma unroll 4
+  for (int p = p1 - 2; p >= p0; --p) {
+    m = pkmin(m, A[p * stride]);
+    A[p * stride] = m;
+  }
+}
+
 // 1-D packed min/max along y (sa = W) or z (sa = W*H).  Work-group tile: XW consecutive
-// x positions times C consecutive outputs along the axis, at one coordinate `o` of the
+// x positions times C consecutive outputs along the axis, at one coordinate of the
 // remaining axis (stride so).  LDS holds the clamped input span (C + 2r) x XW twice:
 // A (input, then suffix minima in place) and P (prefix minima).
 template <int XW, bool FROM_U16>
@@ -48,26 +69,22 @@ __global__ __launch_bounds__(NT) void k_minmax_line(const void* __restrict__ src
   const int ct = b % nct; b /= nct;
   const i64 base = (i64)b * so;
   const int x0 = xt * XW, c0 = ct * C;
-  for (int e = threadIdx.x; e < span * XW; e += NT) {
-    const int p = e / XW, xl = e % XW, x = x0 + xl;
-    uint32_t v = 0;
-    if (x < W) {
-      const i64 idx = base + (i64)clampi(c0 - r + p, 0, L - 1) * sa + x;
-      v = FROM_U16 ? pack(((const uint16_t*)src)[idx]) : ((const uint32_t*)src)[idx];
-    }
-    A[e] = v;
-  }
+  stage_tile<8, NT, uint32_t>(
+      span * XW,
+      [&](int e) -> uint32_t {
+        const int p = e / XW, x = x0 + e % XW;
+        if (x >= W) return 0u;
+        const i64 idx = base + (i64)clampi(c0 - r + p, 0, L - 1) * sa + x;
+        return FROM_U16 ? pack(((const uint16_t*)src)[idx]) : ((const uint32_t*)src)[idx];
+      },
+      [&](int e, uint32_t v) { A[e] = v; });
   __syncthreads();
   const int w = 2 * r + 1;
   const int nb = (span + w - 1) / w;
   for (int t = threadIdx.x; t < nb * XW; t += NT) {
     const int blk = t / XW, xl = t % XW;
-    const int p0 = blk * w, p1 = min(p0 + w, span);
-    uint32_t m = A[p0 * XW + xl];
-    P[p0 * XW + xl] = m;
-    for (int p = p0 + 1; p < p1; ++p) { m = pkmin(m, A[p * XW + xl]); P[p * XW + xl] = m; }
-    m = A[(p1 - 1) * XW + xl];
-    for (int p = p1 - 2; p >= p0; --p) { m = pkmin(m, A[p * XW + xl]); A[p * XW + xl] = m; }
+    const int p0 = blk * w;
+    vh_block(A + xl, P + xl, p0, min(p0 + w, span), XW);
   }
   __syncthreads();
   for (int e = threadIdx.x; e < C * XW; e += NT) {
@@ -79,12 +96,14 @@ __global__ __launch_bounds__(NT) void k_minmax_line(const void* __restrict__ src
 
 // x pass of min/max (packed) fused with the x pass of the box filter.  Tile: ROWS rows
 // times C outputs.  The min/max result is needed on [c0-r, c0+C+r) clamped to [0, W)
-// because the box pass clamps to the edge of the *filtered* row.
-// Output: float2 (E_x, D_x) = (sum_x(min) * k, sum_x(max) * k); sums of integers.
-template <int ROWS>
+// because the box pass clamps to the edge of the *filtered* row.  Each thread then
+// sums RUN consecutive box windows by sliding (integers: exact in any order).
+// Output: float2 (E_x, D_x) = (sum(min) * k, sum(max) * k).
+template <int ROWS, int RUN>
 __global__ __launch_bounds__(NT) void k_minmax_x_box(const uint32_t* __restrict__ src,
                                                      float2* __restrict__ dst, int W, i64 nrows,
-                                                     int r, int C, int nct, float k) {
+                                                     int r, int nct, float k) {
+  constexpr int C = NT / ROWS * RUN;
   extern __shared__ uint32_t smem[];
   const int spanmax = C + 4 * r;
   uint32_t* A = smem;
@@ -95,51 +114,56 @@ __global__ __launch_bounds__(NT) void k_minmax_x_box(const uint32_t* __restrict_
   const int qa = max(0, c0 - r), qb = min(W, c0 + C + r);
   const int nq = qb - qa;
   const int span = nq + 2 * r;
-  for (int e = threadIdx.x; e < ROWS * span; e += NT) {
-    const int row = e / span, p = e % span;
-    const i64 grow = row0 + row;
-    uint32_t v = 0;
-    if (grow < nrows) v = src[grow * W + clampi(qa - r + p, 0, W - 1)];
-    A[row * spanmax + p] = v;
-  }
+  stage_tile<8, NT, uint32_t>(
+      ROWS * span,
+      [&](int e) -> uint32_t {
+        const int row = e / span, p = e % span;
+        const i64 grow = row0 + row;
+        return grow < nrows ? src[grow * W + clampi(qa - r + p, 0, W - 1)] : 0u;
+      },
+      [&](int e, uint32_t v) { A[(e / span) * spanmax + e % span] = v; });
   __syncthreads();
   const int w = 2 * r + 1;
   const int nb = (span + w - 1) / w;
   for (int t = threadIdx.x; t < ROWS * nb; t += NT) {
     const int row = t / nb, blk = t % nb;
-    uint32_t* a = A + row * spanmax;
-    uint32_t* pp = P + row * spanmax;
-    const int p0 = blk * w, p1 = min(p0 + w, span);
-    uint32_t m = a[p0];
-    pp[p0] = m;
-    for (int p = p0 + 1; p < p1; ++p) { m = pkmin(m, a[p]); pp[p] = m; }
-    m = a[p1 - 1];
-    for (int p = p1 - 2; p >= p0; --p) { m = pkmin(m, a[p]); a[p] = m; }
+    const int p0 = blk * w;
+    vh_block(A + row * spanmax, P + row * spanmax, p0, min(p0 + w, span), 1);
   }
   __syncthreads();
-  // filtered value at position qa + j, stored in place over the suffix array
   for (int e = threadIdx.x; e < ROWS * nq; e += NT) {
     const int row = e / nq, j = e % nq;
     A[row * spanmax + j] = pkmin(A[row * spanmax + j], P[row * spanmax + j + 2 * r]);
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < ROWS * C; e += NT) {
-    const int row = e / C, i = e % C, x = c0 + i;
-    const i64 grow = row0 + row;
-    if (x >= W || grow >= nrows) continue;
-    const uint32_t* m = A + row * spanmax - qa;
-    float se = 0.0f, sd = 0.0f;
-    for (int j = -r; j <= r; ++j) {
-      const uint32_t v = m[clampi(x + j, 0, W - 1)];
-      se += (float)(v & 0xFFFFu);
-      sd += (float)(65535u - (v >> 16));
+  const int row = threadIdx.x / (NT / ROWS), seg = threadIdx.x % (NT / ROWS);
+  const i64 grow = row0 + row;
+  if (grow >= nrows) return;
+  const uint32_t* m = A + row * spanmax - qa;
+  const int xs = c0 + seg * RUN;
+  if (xs >= W) return;
+  uint32_t se = 0, sdn = 0;  // sum of min, sum of (65535 - max)
+  for (int j = -r; j <= r; ++j) {
+    const uint32_t v = m[clampi(xs + j, 0, W - 1)];
+    se += v & 0xFFFFu;
+    sdn += v >> 16;
+  }
+  const uint32_t full = (uint32_t)(2 * r + 1) * 65535u;
+#pragma unroll
+  for (int q = 0; q < RUN; ++q) {
+    const int x = xs + q;
+    if (x >= W) break;
+    if (q > 0) {
+      const uint32_t vin = m[clampi(x + r, 0, W - 1)], vout = m[clampi(x - r - 1, 0, W - 1)];
+      se += (vin & 0xFFFFu) - (vout & 0xFFFFu);
+      sdn += (vin >> 16) - (vout >> 16);
     }
-    dst[grow * W + x] = make_float2(se * k, sd * k);
+    dst[grow * W + x] = make_float2((float)se * k, (float)(full - sdn) * k);
   }
 }
 
-// Box pass on float2 (E, D) along y or z with clamp-to-edge and the sequential FP32 sum
-// order of the reference.  With NORM the z pass also normalizes against img.
+// Generic box pass on float2 (E, D) along y or z: clamp-to-edge, sequential FP32 sums.
+// With NORM the z pass also normalizes against img.
 template <int XW, bool NORM, typename TS>
 __global__ __launch_bounds__(NT) void k_box_line(const float2* __restrict__ src,
                                                  float2* __restrict__ dst, TS* __restrict__ norm,
@@ -153,21 +177,78 @@ __global__ __launch_bounds__(NT) void k_box_line(const float2* __restrict__ src,
   const int ct = b % nct; b /= nct;
   const i64 base = (i64)b * so;
   const int x0 = xt * XW, c0 = ct * C;
-  for (int e = threadIdx.x; e < span * XW; e += NT) {
-    const int p = e / XW, xl = e % XW, x = x0 + xl;
-    float2 v = make_float2(0.0f, 0.0f);
-    if (x < W) v = src[base + (i64)clampi(c0 - r + p, 0, L - 1) * sa + x];
-    smem2[e] = v;
-  }
+  stage_tile<8, NT, float2>(
+      span * XW,
+      [&](int e) -> float2 {
+        const int p = e / XW, x = x0 + e % XW;
+        if (x >= W) return make_float2(0.0f, 0.0f);
+        return src[base + (i64)clampi(c0 - r + p, 0, L - 1) * sa + x];
+      },
+      [&](int e, float2 v) { smem2[e] = v; });
   __syncthreads();
   for (int e = threadIdx.x; e < C * XW; e += NT) {
     const int i = e / XW, xl = e % XW, c = c0 + i, x = x0 + xl;
     if (c >= L || x >= W) continue;
     float ve = 0.0f, vd = 0.0f;
+#pragma unroll 4
     for (int j = 0; j <= 2 * r; ++j) {
       const float2 t = smem2[(i + j) * XW + xl];
       ve += t.x;
       vd += t.y;
+    }
+    ve *= k;
+    vd *= k;
+    const i64 idx = base + (i64)c * sa + x;
+    if (NORM) {
+      const float I = (float)img[idx];
+      norm[idx] = cvt<TS>((I >= min_int) ? (I - ve) / (vd - ve) : 0.0f);
+    } else {
+      dst[idx] = make_float2(ve, vd);
+    }
+  }
+}
+
+// Compile-time radius: tile XW x (NSEG*RUN) outputs; thread (xl, seg) keeps a register
+// window of RUN + 2R pairs and produces RUN consecutive outputs along the axis.
+template <int R, int XW, int RUN, bool NORM, typename TS>
+__global__ __launch_bounds__(NT) void k_box_line_r(const float2* __restrict__ src,
+                                                   float2* __restrict__ dst, TS* __restrict__ norm,
+                                                   const uint16_t* __restrict__ img,
+                                                   float min_int, int W, int L, i64 sa, i64 so,
+                                                   int nxt, int nct, float k) {
+  constexpr int NSEG = NT / XW;
+  constexpr int C = NSEG * RUN;
+  constexpr int SPAN = C + 2 * R;
+  __shared__ float2 T[SPAN * XW];
+  unsigned b = blockIdx.x;
+  const int xt = b % nxt; b /= nxt;
+  const int ct = b % nct; b /= nct;
+  const i64 base = (i64)b * so;
+  const int x0 = xt * XW, c0 = ct * C;
+  stage_tile<8, NT, float2>(
+      SPAN * XW,
+      [&](int e) -> float2 {
+        const int p = e / XW, x = x0 + e % XW;
+        if (x >= W) return make_float2(0.0f, 0.0f);
+        return src[base + (i64)clampi(c0 - R + p, 0, L - 1) * sa + x];
+      },
+      [&](int e, float2 v) { T[e] = v; });
+  __syncthreads();
+  const int xl = threadIdx.x % XW, seg = threadIdx.x / XW;
+  const int x = x0 + xl;
+  if (x >= W) return;
+  float2 win[RUN + 2 * R];
+#pragma unroll
+  for (int q = 0; q < RUN + 2 * R; ++q) win[q] = T[(seg * RUN + q) * XW + xl];
+#pragma unroll
+  for (int o = 0; o < RUN; ++o) {
+    const int c = c0 + seg * RUN + o;
+    if (c >= L) break;
+    float ve = 0.0f, vd = 0.0f;
+#pragma unroll
+    for (int j = 0; j <= 2 * R; ++j) {
+      ve += win[o + j].x;
+      vd += win[o + j].y;
     }
     ve *= k;
     vd *= k;
@@ -224,14 +305,34 @@ hipError_t minmax_line(const void* src, bool from_u16, uint32_t* dst, int W, int
   });
 }
 
+template <int R, bool NORM, typename TS>
+hipError_t box_line_fast(const float2* src, float2* dst, TS* norm, const uint16_t* img,
+                         float min_int, int W, int L, i64 sa, i64 so, i64 n_other, float k,
+                         hipStream_t s) {
+  constexpr int XW = 32, RUN = 16, C = NT / XW * RUN;
+  const int nxt = (W + XW - 1) / XW, nct = (L + C - 1) / C;
+  const i64 nblk = (i64)nxt * nct * n_other;
+  hipLaunchKernelGGL((k_box_line_r<R, XW, RUN, NORM, TS>), dim3((unsigned)nblk), dim3(NT), 0, s,
+                     src, dst, norm, img, min_int, W, L, sa, so, nxt, nct, k);
+  return hipGetLastError();
+}
+
 template <bool NORM, typename TS>
 hipError_t box_line(const float2* src, float2* dst, TS* norm, const uint16_t* img, float min_int,
                     int W, int L, i64 sa, i64 so, i64 n_other, int r, hipStream_t s) {
+  const float k = 1.0f / (float)(2 * r + 1);
+  switch (r) {  // radii with a register-window kernel
+    case 1: return box_line_fast<1, NORM, TS>(src, dst, norm, img, min_int, W, L, sa, so, n_other, k, s);
+    case 2: return box_line_fast<2, NORM, TS>(src, dst, norm, img, min_int, W, L, sa, so, n_other, k, s);
+    case 3: return box_line_fast<3, NORM, TS>(src, dst, norm, img, min_int, W, L, sa, so, n_other, k, s);
+    case 5: return box_line_fast<5, NORM, TS>(src, dst, norm, img, min_int, W, L, sa, so, n_other, k, s);
+    case 10: return box_line_fast<10, NORM, TS>(src, dst, norm, img, min_int, W, L, sa, so, n_other, k, s);
+    default: break;
+  }
   const LineCfg c = line_cfg(r, L, 8, 1);
   const int nxt = (W + c.XW - 1) / c.XW, nct = (L + c.C - 1) / c.C;
   const i64 nblk = (i64)nxt * nct * n_other;
   const size_t lds = (size_t)(c.C + 2 * r) * c.XW * 8;
-  const float k = 1.0f / (float)(2 * r + 1);
   return dispatch_xw(c.XW, [&](auto xw) {
     constexpr int XW = decltype(xw)::value;
     hipLaunchKernelGGL((k_box_line<XW, NORM, TS>), dim3((unsigned)nblk), dim3(NT), lds, s, src,
@@ -264,15 +365,15 @@ hipError_t launch_normalize(const uint16_t* img, TS* norm, void* tmp1, void* tmp
   mark("normalize/minmax_y", s);
   // min/max along x fused with the box x pass -> float2 (E, D) in tmp1
   {
-    constexpr int ROWS = 4;
-    const int C = 256;
+    constexpr int ROWS = 4, RUN = 4;
+    constexpr int C = NT / ROWS * RUN;
     const int nct = (W + C - 1) / C;
     const i64 nrows = (i64)H * D;
     const i64 nblk = ((nrows + ROWS - 1) / ROWS) * nct;
     const size_t lds = 2 * (size_t)ROWS * (C + 4 * rxy) * 4;
     const float k = 1.0f / (float)(2 * rxy + 1);
-    hipLaunchKernelGGL((k_minmax_x_box<ROWS>), dim3((unsigned)nblk), dim3(NT), lds, s, p2,
-                       (float2*)tmp1, W, nrows, rxy, C, nct, k);
+    hipLaunchKernelGGL((k_minmax_x_box<ROWS, RUN>), dim3((unsigned)nblk), dim3(NT), lds, s, p2,
+                       (float2*)tmp1, W, nrows, rxy, nct, k);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     mark("normalize/minmax_x_box_x", s);
   }

@@ -116,37 +116,74 @@ __device__ __forceinline__ float fdecode(uint32_t k) {
   return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
 }
 
-// one work-group per (row, 256-wide x segment)
+// Each wave takes 64 consecutive x of one row.  Runs of equal region id along x are
+// reduced inside the wave first (segmented shuffle reduction, every lane of a run ends on
+// the same last lane), then only the first lane of each run issues the atomics: the
+// atomics per candidate voxel drop by the mean run length.
 template <typename TS>
 __global__ __launch_bounds__(NT) void k_accum(const uint32_t* __restrict__ L,
                                               const uint32_t* __restrict__ rid,
                                               const TS* __restrict__ hess,
                                               const TS* __restrict__ norm,
                                               const uint16_t* __restrict__ img, RegionAcc* acc,
-                                              int W, int H, i64 N, int nxs, int is_ave) {
-  const unsigned row = blockIdx.x / nxs;
-  const int x = (blockIdx.x % nxs) * NT + threadIdx.x;
-  if (x >= W) return;
-  const unsigned y = row % (unsigned)H, z = row / (unsigned)H;
-  const i64 i = (i64)row * W + x;
-  const uint32_t l = L[i];
-  if (!l) return;
-  RegionAcc* a = acc + rid[l];
-  atomicAdd(&a->cnt, 1u);
-  atomicAdd(&a->sx, (unsigned long long)x);
-  atomicAdd(&a->sy, (unsigned long long)y);
-  atomicAdd(&a->sz, (unsigned long long)z);
+                                              int W, int H, i64 nrows, i64 N, int is_ave) {
+  const int lane = threadIdx.x & 63;
+  const int nxw = (W + 63) >> 6;
+  const i64 wid = (i64)blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+  const i64 row = wid / nxw;
+  if (row >= nrows) return;  // whole wave
+  const int x = (int)(wid - row * nxw) * 64 + lane;
+  const i64 i = row * W + x;
+  const uint32_t l = x < W ? L[i] : 0u;
+  if (__ballot(l != 0) == 0ull) return;  // whole wave: background only
+  const uint32_t r = l ? rid[l] : 0u;
+  const uint32_t rprev = __shfl_up(r, 1, 64);
+  const bool head = lane == 0 || r != rprev;
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long later = lane == 63 ? 0ull : (heads & (~0ull << (lane + 1)));
+  const int run_end = later ? (__ffsll((long long)later) - 2) : 63;
+
+  uint32_t cnt = l ? 1u : 0u, sx = l ? (uint32_t)x : 0u, isum = 0u, imax = 0u, nkey = 0u;
+  double h[6], ns = 0.0;
 #pragma unroll
-  for (int k = 0; k < 6; ++k) unsafeAtomicAdd(&a->h[k], (double)ld(hess, (i64)k * N + i));
-  const float v = ld(norm, i);
+  for (int k = 0; k < 6; ++k) h[k] = l ? (double)ld(hess, (i64)k * N + i) : 0.0;
+  if (l) {
+    const float v = ld(norm, i);
+    if (is_ave) ns = (double)v; else nkey = fkey(v);
+    if (img) { isum = img[i]; imax = isum; }
+  }
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const bool take = lane + d <= run_end;
+    const uint32_t c2 = __shfl_down(cnt, d, 64), x2 = __shfl_down(sx, d, 64);
+    const uint32_t i2 = __shfl_down(isum, d, 64), m2 = __shfl_down(imax, d, 64);
+    const uint32_t k2 = __shfl_down(nkey, d, 64);
+    const double n2 = __shfl_down(ns, d, 64);
+    double h2[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) h2[k] = __shfl_down(h[k], d, 64);
+    if (take) {
+      cnt += c2; sx += x2; isum += i2; imax = max(imax, m2); nkey = max(nkey, k2); ns += n2;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) h[k] += h2[k];
+    }
+  }
+  if (!head || !l) return;
+  const unsigned y = (unsigned)(row % H), z = (unsigned)(row / H);
+  RegionAcc* a = acc + r;
+  atomicAdd(&a->cnt, cnt);
+  atomicAdd(&a->sx, (unsigned long long)sx);
+  atomicAdd(&a->sy, (unsigned long long)cnt * y);
+  atomicAdd(&a->sz, (unsigned long long)cnt * z);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) unsafeAtomicAdd(&a->h[k], h[k]);
   if (is_ave)
-    unsafeAtomicAdd(&a->nsum, (double)v);
+    unsafeAtomicAdd(&a->nsum, ns);
   else
-    atomicMax(&a->nmax_key, fkey(v));
+    atomicMax(&a->nmax_key, nkey);
   if (img) {
-    const uint32_t I = img[i];
-    atomicAdd(&a->isum, (unsigned long long)I);
-    atomicMax(&a->imax, I);
+    atomicAdd(&a->isum, (unsigned long long)isum);
+    atomicMax(&a->imax, imax);
   }
 }
 
@@ -252,11 +289,12 @@ hipError_t launch_region_accum(const uint32_t* labels, const uint32_t* rid, cons
                                const TS* norm, const uint16_t* img, RegionAcc* acc, i64 R,
                                int W, int H, int D, int is_ave, hipStream_t s) {
   (void)R;
-  const int nxs = (W + NT - 1) / NT;
-  const i64 nblk = (i64)H * D * nxs;
+  const i64 nrows = (i64)H * D;
+  const i64 nwaves = nrows * ((W + 63) / 64);
+  const i64 nblk = (nwaves + NT / 64 - 1) / (NT / 64);
   const i64 N = (i64)W * H * D;
   hipLaunchKernelGGL((k_accum<TS>), dim3((unsigned)nblk), dim3(NT), 0, s, labels, rid, hess, norm,
-                     img, acc, W, H, N, nxs, is_ave);
+                     img, acc, W, H, nrows, N, is_ave);
   mark("regions/accum", s);
   return hipGetLastError();
 }
