@@ -85,10 +85,11 @@ def run_form(hd, img, shape, precision, steps, warmup, keep_pd=False):
             kms[name].append(ms)
     hd.set_profiling(False)
     pd = bufs["pd"].clone() if keep_pd else None
+    cand = int(bufs["pd"].sum(dtype=torch.int64).item())
     del bufs
     torch.cuda.empty_cache()
     return dict(R=R, wall=wall, dev_ms=dev_ms, kernels={k: sum(v) / len(v) for k, v in kms.items()},
-                pd=pd)
+                pd=pd, cand=cand)
 
 
 def cpu_baseline(img, depth):
@@ -117,6 +118,9 @@ def main():
     ap.add_argument("--forms", default="fp32,fp16", help="storage forms to time; first is `value`")
     ap.add_argument("--depth", type=int, default=512, help="stack depth (z) of the workload")
     ap.add_argument("--cpu-depth", type=int, default=32, help="substack depth of the CPU sample; 0 skips")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run a plain device-to-device copy of MIB MiB (three launches), "
+                         "the reference kernel for scaling HBM counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); the multi-GPU design is "
@@ -126,6 +130,17 @@ def main():
     import numpy as np
     import torch
     from hdog import HDoG, synth
+
+    if args.copy_calibration > 0:
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: 3 x {args.copy_calibration} MiB device-to-device")
+        del src, dst
+        torch.cuda.empty_cache()
 
     hd = HDoG(0)
     cfg = synth.CONFIGS["c1"]
@@ -186,6 +201,7 @@ def main():
         "pipeline_roofline": {"bytes_per_voxel": PIPELINE_BYTES[main_f], "achieved": pipe_gbs,
                               "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": pipe_gbs / HBM_PEAK_GBS},
         "num_regions": r["R"],
+        "candidate_fraction": r["cand"] / N,
         "kernels_ms": {k: round(v, 4) for k, v in kernels.items()},
         "precision_forms": forms_out,
         "wall_s_timed": r["wall"],

@@ -114,14 +114,14 @@ __device__ __forceinline__ uint32_t lds_find(const uint32_t* lab, uint32_t x) {
   return x;
 }
 
-__device__ __forceinline__ void lds_unite(uint32_t* lab, uint32_t a, uint32_t b) {
+__device__ __forceinline__ uint32_t lds_unite(uint32_t* lab, uint32_t a, uint32_t b) {
   while (true) {
     a = lds_find(lab, a);
     b = lds_find(lab, b);
-    if (a == b) return;
+    if (a == b) return a;
     if (a < b) { const uint32_t t = a; a = b; b = t; }
     const uint32_t old = atomicMin(lab + a, b);
-    if (old == a) return;
+    if (old == a) return b;
     a = old;
   }
 }
@@ -156,11 +156,12 @@ __global__ __launch_bounds__(NT) void k_ccl_local(const uint8_t* __restrict__ pd
     lab[v] = fg[k] ? (uint32_t)v : NONE;
   }
   __syncthreads();
-#pragma unroll
+#pragma unroll 1
   for (int k = 0; k < CPT; ++k) {
     if (!fg[k]) continue;
     const int v = threadIdx.x + k * NT;
     const int lx = v & (CX - 1), ly = (v / CX) & (CY - 1), lz = v / (CX * CY);
+    uint32_t root = (uint32_t)v;  // a representative of v's set, refreshed by each link
 #pragma unroll
     for (int q = 0; q < 13; ++q) {
       int dx, dy, dz;
@@ -168,7 +169,10 @@ __global__ __launch_bounds__(NT) void k_ccl_local(const uint8_t* __restrict__ pd
       const int nx = lx + dx, ny = ly + dy, nz = lz + dz;
       if (nx < 0 || nx >= CX || ny < 0 || ny >= CY || nz >= CZ) continue;
       const int u = nx + CX * (ny + CY * nz);
-      if (lab[u] != NONE) lds_unite(lab, (uint32_t)v, (uint32_t)u);  // outside-volume = NONE
+      const uint32_t lu = lab[u];
+      // outside-volume and background positions hold NONE; a parent equal to our
+      // representative means u is already in our set
+      if (lu != NONE && lu != root) root = lds_unite(lab, root, (uint32_t)u);
     }
   }
   __syncthreads();

@@ -10,10 +10,10 @@
 // intermediate volumes of the reference become one interleaved stream.  Intermediates
 // stay FP32 whatever the storage type of norm / dog.
 //
-// Radius pairs (r1, r2) that occur in practice have compile-time kernels: weights live in
-// scalar registers, and a thread produces RUN consecutive outputs from a register window
-// of RUN + 2*r2 values (x pass: 4 outputs from 128-bit LDS reads; y / z passes: 16
-// outputs).  Other radii use the generic loops.
+// Radius pairs (r1, r2) that occur in practice have compile-time kernels with the weights
+// in scalar registers: the x pass makes 4 outputs per thread from 128-bit LDS reads, the
+// y / z passes are column walkers (one thread per line, a register ring, every input
+// read once, no LDS).  Other radii use the generic LDS-tile loops.
 #include "hdog_internal.hpp"
 
 namespace hdog {
@@ -146,51 +146,44 @@ __global__ __launch_bounds__(NT) void k_gauss_x_r(const TS* __restrict__ src,
   }
 }
 
+// y / z pass as a column walker: one thread per line, a register ring of the last 2R+1
+// (G1, G2) pairs (zero outside the line), unrolled by the ring period so every index is
+// static; every input is read once and no LDS is used.
 template <int R1, int R2, bool LAST, typename TS>
-__global__ __launch_bounds__(NT) void k_gauss_line_r(const float2* __restrict__ src,
-                                                     float2* __restrict__ dst,
-                                                     TS* __restrict__ dog,
-                                                     const float* __restrict__ w1,
-                                                     const float* __restrict__ w2, float nz, int W,
-                                                     int L, i64 sa, i64 so, int nxt, int nct) {
-  constexpr int XW = 32, RUN = 16, NSEG = NT / XW, C = NSEG * RUN;
+__global__ __launch_bounds__(NT) void k_gauss_walk(const float2* __restrict__ src,
+                                                   float2* __restrict__ dst, TS* __restrict__ dog,
+                                                   const float* __restrict__ w1,
+                                                   const float* __restrict__ w2, float nz, int W,
+                                                   int L, i64 sa, i64 so, int nxb) {
   constexpr int R = R1 > R2 ? R1 : R2;
-  constexpr int SPAN = C + 2 * R;
-  __shared__ float2 T[SPAN * XW];
-  unsigned b = blockIdx.x;
-  const int xt = b % nxt; b /= nxt;
-  const int ct = b % nct; b /= nct;
-  const i64 base = (i64)b * so;
-  const int x0 = xt * XW, c0 = ct * C;
-  stage_tile<8, NT, float2>(
-      SPAN * XW,
-      [&](int e) -> float2 {
-        const int p = e / XW, x = x0 + e % XW, c = c0 - R + p;
-        return (x < W && c >= 0 && c < L) ? src[base + (i64)c * sa + x] : make_float2(0.0f, 0.0f);
-      },
-      [&](int e, float2 v) { T[e] = v; });
-  __syncthreads();
-  const int xl = threadIdx.x % XW, seg = threadIdx.x / XW;
-  const int x = x0 + xl;
+  constexpr int WB = 2 * R + 1;
+  const int x = (blockIdx.x % nxb) * NT + threadIdx.x;
   if (x >= W) return;
-  float2 win[RUN + 2 * R];
+  const i64 base = (i64)(blockIdx.x / nxb) * so + x;
+  auto load = [&](int p) -> float2 {
+    return (p >= 0 && p < L) ? src[base + (i64)p * sa] : make_float2(0.0f, 0.0f);
+  };
+  float2 ring[WB];  // position p lives in ring[(p + R) % WB]
 #pragma unroll
-  for (int q = 0; q < RUN + 2 * R; ++q) win[q] = T[(seg * RUN + q) * XW + xl];
+  for (int q = 0; q < WB; ++q) ring[q] = load(q - R);
+  for (int c0 = 0; c0 < L; c0 += WB) {
 #pragma unroll
-  for (int o = 0; o < RUN; ++o) {
-    const int c = c0 + seg * RUN + o;
-    if (c >= L) break;
-    // window index of position c + j is o + j + R
-    float s1 = 0.0f, s2 = 0.0f;
+    for (int q = 0; q < WB; ++q) {
+      const int c = c0 + q;
+      if (c < L) {
+        float s1 = 0.0f, s2 = 0.0f;
 #pragma unroll
-    for (int j = -R1; j <= R1; ++j) s1 = fmaf(w1[R1 - j], win[o + j + R].x, s1);
+        for (int j = -R1; j <= R1; ++j) s1 = fmaf(w1[R1 - j], ring[(q + j + R) % WB].x, s1);
 #pragma unroll
-    for (int j = -R2; j <= R2; ++j) s2 = fmaf(w2[R2 - j], win[o + j + R].y, s2);
-    const i64 idx = base + (i64)c * sa + x;
-    if (LAST)
-      dog[idx] = cvt<TS>((s1 - s2) * nz);
-    else
-      dst[idx] = make_float2(s1, s2);
+        for (int j = -R2; j <= R2; ++j) s2 = fmaf(w2[R2 - j], ring[(q + j + R) % WB].y, s2);
+        const i64 idx = base + (i64)c * sa;
+        if (LAST)
+          dog[idx] = cvt<TS>((s1 - s2) * nz);
+        else
+          dst[idx] = make_float2(s1, s2);
+      }
+      ring[q] = load(c + R + 1);  // replaces position c - R
+    }
   }
 }
 
@@ -226,14 +219,13 @@ template <bool LAST, typename TS>
 hipError_t gauss_line(const float2* src, float2* dst, TS* dog, const float* w1, const float* w2,
                       int r1, int r2, float nz, int W, int L, i64 sa, i64 so, i64 n_other,
                       hipStream_t s) {
-#define X(A, B)                                                                               \
-  if (r1 == A && r2 == B) {                                                                   \
-    constexpr int C = NT / 32 * 16;                                                           \
-    const int nxt = (W + 31) / 32, nct = (L + C - 1) / C;                                     \
-    const i64 nblk = (i64)nxt * nct * n_other;                                                \
-    hipLaunchKernelGGL((k_gauss_line_r<A, B, LAST, TS>), dim3((unsigned)nblk), dim3(NT), 0, s, \
-                       src, dst, dog, w1, w2, nz, W, L, sa, so, nxt, nct);                    \
-    return hipGetLastError();                                                                 \
+#define X(A, B)                                                                              \
+  if (r1 == A && r2 == B) {                                                                  \
+    const int nxb = (W + NT - 1) / NT;                                                       \
+    const i64 nblk = (i64)nxb * n_other;                                                     \
+    hipLaunchKernelGGL((k_gauss_walk<A, B, LAST, TS>), dim3((unsigned)nblk), dim3(NT), 0, s, \
+                       src, dst, dog, w1, w2, nz, W, L, sa, so, nxb);                        \
+    return hipGetLastError();                                                                \
   }
   HDOG_GAUSS_PAIRS(X)
 #undef X

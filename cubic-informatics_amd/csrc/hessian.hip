@@ -58,9 +58,9 @@ __device__ __forceinline__ uint8_t negdef_band(float xx, float xy, float xz, flo
 
 // z-streaming form.  A work-group owns a TX x TY column of outputs over ZC consecutive
 // slices and walks s = z0-2 .. z1+1 with three-slot LDS rings:
-//   step s: load dog(s) (xy halo 2)  ->  Hx(s-1), Hy(s-1) from dog(s-1) (xy halo 1),
-//           Hz(s-1) from dog(s) - dog(s-2)  ->  outputs of slice z = s-2, which need
-//           Hx / Hy at z-1..z+1 and Hz at z-1, z+1.
+//   step s: Hx(s-1), Hy(s-1) from dog(s-1) (xy halo 1), Hz(s-1) from dog(s) - dog(s-2)
+//           ->  outputs of slice z = s-2, which need Hx / Hy at z-1..z+1 and Hz at
+//           z-1, z+1  ->  dog(s+1) (xy halo 2) into the ring, fetched one step earlier.
 // Every dog value is read once per column (plus the xy halo, served by L2) instead of
 // (TZ+4)/TZ times; first derivatives outside the volume are zero as in the reference's
 // second pass.
@@ -83,24 +83,37 @@ __global__ __launch_bounds__(NT) void k_hessian(const TS* __restrict__ dog, TS* 
   const i64 WH = (i64)W * H;
   const i64 N = WH * D;
   const int tx = threadIdx.x % TX, ty0 = threadIdx.x / TX;  // outputs (tx, ty0 + k*(NT/TX))
-  for (int s = z0 - 2; s <= z1 + 1; ++s) {
-    // 1. dog(s) -> ring slot
-    {
-      float* d = sD[(s + 3) % 3];
-      const bool zin = s >= 0 && s < D;
-      stage_tile<4, NT, float>(
-          DY * DX,
-          [&](int e) -> float {
-            const int lx = e % DX, ly = e / DX;
-            const int gx = x0 - 2 + lx, gy = y0 - 2 + ly;
-            return (zin && gx >= 0 && gx < W && gy >= 0 && gy < H)
-                       ? ld(dog, (i64)s * WH + (i64)gy * W + gx)
-                       : 0.0f;
-          },
-          [&](int e, float v) { d[e] = v; });
+  // dog slices go global -> registers one step ahead -> LDS ring, so the loads of slice
+  // s+2 are in flight while slice s is processed
+  constexpr int NPF = (DY * DX + NT - 1) / NT;
+  float pf[NPF];
+  auto fetch = [&](int sl) {
+    const bool zin = sl >= 0 && sl < D;
+#pragma unroll
+    for (int k = 0; k < NPF; ++k) {
+      const int e = threadIdx.x + k * NT;
+      const int lx = e % DX, ly = e / DX;
+      const int gx = x0 - 2 + lx, gy = y0 - 2 + ly;
+      pf[k] = (e < DY * DX && zin && gx >= 0 && gx < W && gy >= 0 && gy < H)
+                  ? ld(dog, (i64)sl * WH + (i64)gy * W + gx)
+                  : 0.0f;
     }
-    __syncthreads();
-    // 2. first derivatives of slice s-1
+  };
+  auto put = [&](int sl) {
+    float* d = sD[(sl + 3) % 3];
+#pragma unroll
+    for (int k = 0; k < NPF; ++k) {
+      const int e = threadIdx.x + k * NT;
+      if (e < DY * DX) d[e] = pf[k];
+    }
+  };
+  fetch(z0 - 2);
+  put(z0 - 2);
+  fetch(z0 - 1);
+  __syncthreads();
+  for (int s = z0 - 2; s <= z1 + 1; ++s) {
+    // invariant: dog(s-2..s) are in LDS, pf holds dog(s+1)
+    // 1. first derivatives of slice s-1 (Hz from dog(s) - dog(s-2))
     {
       const int zc = s - 1;
       const bool zin = zc >= 0 && zc < D;
@@ -136,7 +149,8 @@ __global__ __launch_bounds__(NT) void k_hessian(const TS* __restrict__ dog, TS* 
       }
     }
     __syncthreads();
-    // 3. outputs of slice z = s - 2
+    // 2. outputs of slice z = s - 2; then dog(s+1) into the slot of dog(s-2) (last read
+    //    above) and start fetching dog(s+2)
     const int z = s - 2;
     if (z >= z0 && z < z1) {
       const float* hxm = sHx[(z + 2) % 3];  // Hx(z-1)
@@ -173,6 +187,8 @@ __global__ __launch_bounds__(NT) void k_hessian(const TS* __restrict__ dog, TS* 
         }
       }
     }
+    put(s + 1);
+    fetch(s + 2);
     __syncthreads();
   }
 }

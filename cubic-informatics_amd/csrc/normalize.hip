@@ -16,9 +16,9 @@
 //     integers, exact in any order, so a sliding sum is used).  The y and z box passes
 //     sum FP32 values and keep the reference's sequential order (c-r .. c+r); the z pass
 //     is fused with the normalization.
-//   * The y / z box passes have compile-time-radius variants: a thread produces RUN
-//     consecutive outputs from a register window of RUN + 2R LDS values (2.25 LDS reads
-//     per output at R = 10 instead of 21); other radii use a generic loop.
+//   * For the radii in use, the y / z passes (min/max and box) are column walkers: one
+//     thread per line keeps a compile-time-sized register ring, reads every input once
+//     and needs no LDS.  Other radii use the LDS-tile kernels.
 #include "hdog_internal.hpp"
 
 namespace hdog {
@@ -208,56 +208,89 @@ __global__ __launch_bounds__(NT) void k_box_line(const float2* __restrict__ src,
   }
 }
 
-// Compile-time radius: tile XW x (NSEG*RUN) outputs; thread (xl, seg) keeps a register
-// window of RUN + 2R pairs and produces RUN consecutive outputs along the axis.
-template <int R, int XW, int RUN, bool NORM, typename TS>
-__global__ __launch_bounds__(NT) void k_box_line_r(const float2* __restrict__ src,
-                                                   float2* __restrict__ dst, TS* __restrict__ norm,
-                                                   const uint16_t* __restrict__ img,
-                                                   float min_int, int W, int L, i64 sa, i64 so,
-                                                   int nxt, int nct, float k) {
-  constexpr int NSEG = NT / XW;
-  constexpr int C = NSEG * RUN;
-  constexpr int SPAN = C + 2 * R;
-  __shared__ float2 T[SPAN * XW];
-  unsigned b = blockIdx.x;
-  const int xt = b % nxt; b /= nxt;
-  const int ct = b % nct; b /= nct;
-  const i64 base = (i64)b * so;
-  const int x0 = xt * XW, c0 = ct * C;
-  stage_tile<8, NT, float2>(
-      SPAN * XW,
-      [&](int e) -> float2 {
-        const int p = e / XW, x = x0 + e % XW;
-        if (x >= W) return make_float2(0.0f, 0.0f);
-        return src[base + (i64)clampi(c0 - R + p, 0, L - 1) * sa + x];
-      },
-      [&](int e, float2 v) { T[e] = v; });
-  __syncthreads();
-  const int xl = threadIdx.x % XW, seg = threadIdx.x / XW;
-  const int x = x0 + xl;
+// ---- column walkers (compile-time radius) ------------------------------------------
+// One thread owns one line along the axis (x-consecutive threads -> coalesced rows) and
+// walks it front to back, so every input is read once and no LDS is needed.
+//
+// Min / max: van Herk blocks of w = 2R+1 over the clamp-extended line g[k] = f[c(k-R)].
+// With S = suffix minima of the previous block and P = prefix minima of the current
+// one, output c = (b-1)w + j is S[0] for j = 0 and min(S[j], P[j-1]) otherwise.
+template <int R, bool FROM_U16>
+__global__ __launch_bounds__(NT) void k_minmax_walk(const void* __restrict__ src,
+                                                    uint32_t* __restrict__ dst, int W, int L,
+                                                    i64 sa, i64 so, int nxb) {
+  constexpr int WB = 2 * R + 1;
+  const int x = (blockIdx.x % nxb) * NT + threadIdx.x;
   if (x >= W) return;
-  float2 win[RUN + 2 * R];
+  const i64 base = (i64)(blockIdx.x / nxb) * so + x;
+  auto load = [&](int k) -> uint32_t {
+    const i64 idx = base + (i64)clampi(k - R, 0, L - 1) * sa;
+    return FROM_U16 ? pack(((const uint16_t*)src)[idx]) : ((const uint32_t*)src)[idx];
+  };
+  uint32_t S[WB], cur[WB];
 #pragma unroll
-  for (int q = 0; q < RUN + 2 * R; ++q) win[q] = T[(seg * RUN + q) * XW + xl];
+  for (int j = 0; j < WB; ++j) cur[j] = load(j);
+  S[WB - 1] = cur[WB - 1];
 #pragma unroll
-  for (int o = 0; o < RUN; ++o) {
-    const int c = c0 + seg * RUN + o;
-    if (c >= L) break;
-    float ve = 0.0f, vd = 0.0f;
+  for (int j = WB - 2; j >= 0; --j) S[j] = pkmin(cur[j], S[j + 1]);
+  const int nb = (L + WB - 1) / WB;
+  for (int b = 1; b <= nb; ++b) {
 #pragma unroll
-    for (int j = 0; j <= 2 * R; ++j) {
-      ve += win[o + j].x;
-      vd += win[o + j].y;
+    for (int j = 0; j < WB; ++j) cur[j] = load(b * WB + j);
+    const int c0 = (b - 1) * WB;
+    uint32_t p = cur[0];
+#pragma unroll
+    for (int j = 0; j < WB; ++j) {
+      if (j > 0) {
+        if (c0 + j < L) dst[base + (i64)(c0 + j) * sa] = pkmin(S[j], p);
+        p = pkmin(p, cur[j]);
+      } else if (c0 < L) {
+        dst[base + (i64)c0 * sa] = S[0];
+      }
     }
-    ve *= k;
-    vd *= k;
-    const i64 idx = base + (i64)c * sa + x;
-    if (NORM) {
-      const float I = (float)img[idx];
-      norm[idx] = cvt<TS>((I >= min_int) ? (I - ve) / (vd - ve) : 0.0f);
-    } else {
-      dst[idx] = make_float2(ve, vd);
+    S[WB - 1] = cur[WB - 1];
+#pragma unroll
+    for (int j = WB - 2; j >= 0; --j) S[j] = pkmin(cur[j], S[j + 1]);
+  }
+}
+
+// Box mean: a register ring of the last 2R+1 inputs (clamp), unrolled by the ring period
+// so every index is static; each output sums its window in the reference order.
+template <int R, bool NORM, typename TS>
+__global__ __launch_bounds__(NT) void k_box_walk(const float2* __restrict__ src,
+                                                 float2* __restrict__ dst, TS* __restrict__ norm,
+                                                 const uint16_t* __restrict__ img, float min_int,
+                                                 int W, int L, i64 sa, i64 so, int nxb, float k) {
+  constexpr int WB = 2 * R + 1;
+  const int x = (blockIdx.x % nxb) * NT + threadIdx.x;
+  if (x >= W) return;
+  const i64 base = (i64)(blockIdx.x / nxb) * so + x;
+  float2 ring[WB];  // position p lives in ring[(p + R) % WB]
+#pragma unroll
+  for (int q = 0; q < WB; ++q) ring[q] = src[base + (i64)clampi(q - R, 0, L - 1) * sa];
+  for (int c0 = 0; c0 < L; c0 += WB) {
+#pragma unroll
+    for (int q = 0; q < WB; ++q) {
+      const int c = c0 + q;
+      if (c < L) {
+        float ve = 0.0f, vd = 0.0f;
+#pragma unroll
+        for (int j = 0; j <= 2 * R; ++j) {  // position c - R + j -> ring[(q + j) % WB]
+          ve += ring[(q + j) % WB].x;
+          vd += ring[(q + j) % WB].y;
+        }
+        ve *= k;
+        vd *= k;
+        const i64 idx = base + (i64)c * sa;
+        if (NORM) {
+          const float I = (float)img[idx];
+          norm[idx] = cvt<TS>((I >= min_int) ? (I - ve) / (vd - ve) : 0.0f);
+        } else {
+          dst[idx] = make_float2(ve, vd);
+        }
+      }
+      // slot of position c - R is reused for position c + R + 1
+      ring[q] = src[base + (i64)clampi(c + R + 1, 0, L - 1) * sa];
     }
   }
 }
@@ -287,8 +320,30 @@ hipError_t dispatch_xw(int XW, F&& f) {
   return f(std::integral_constant<int, 16>{});
 }
 
+template <int R>
+hipError_t minmax_walk(const void* src, bool from_u16, uint32_t* dst, int W, int L, i64 sa, i64 so,
+                       i64 n_other, hipStream_t s) {
+  const int nxb = (W + NT - 1) / NT;
+  const i64 nblk = (i64)nxb * n_other;
+  if (from_u16)
+    hipLaunchKernelGGL((k_minmax_walk<R, true>), dim3((unsigned)nblk), dim3(NT), 0, s, src, dst, W,
+                       L, sa, so, nxb);
+  else
+    hipLaunchKernelGGL((k_minmax_walk<R, false>), dim3((unsigned)nblk), dim3(NT), 0, s, src, dst,
+                       W, L, sa, so, nxb);
+  return hipGetLastError();
+}
+
 hipError_t minmax_line(const void* src, bool from_u16, uint32_t* dst, int W, int L, i64 sa,
                        i64 so, i64 n_other, int r, hipStream_t s) {
+  switch (r) {  // radii with a column-walker kernel
+    case 1: return minmax_walk<1>(src, from_u16, dst, W, L, sa, so, n_other, s);
+    case 2: return minmax_walk<2>(src, from_u16, dst, W, L, sa, so, n_other, s);
+    case 3: return minmax_walk<3>(src, from_u16, dst, W, L, sa, so, n_other, s);
+    case 5: return minmax_walk<5>(src, from_u16, dst, W, L, sa, so, n_other, s);
+    case 10: return minmax_walk<10>(src, from_u16, dst, W, L, sa, so, n_other, s);
+    default: break;
+  }
   const LineCfg c = line_cfg(r, L, 4, 2);
   const int nxt = (W + c.XW - 1) / c.XW, nct = (L + c.C - 1) / c.C;
   const i64 nblk = (i64)nxt * nct * n_other;
@@ -309,11 +364,10 @@ template <int R, bool NORM, typename TS>
 hipError_t box_line_fast(const float2* src, float2* dst, TS* norm, const uint16_t* img,
                          float min_int, int W, int L, i64 sa, i64 so, i64 n_other, float k,
                          hipStream_t s) {
-  constexpr int XW = 32, RUN = 16, C = NT / XW * RUN;
-  const int nxt = (W + XW - 1) / XW, nct = (L + C - 1) / C;
-  const i64 nblk = (i64)nxt * nct * n_other;
-  hipLaunchKernelGGL((k_box_line_r<R, XW, RUN, NORM, TS>), dim3((unsigned)nblk), dim3(NT), 0, s,
-                     src, dst, norm, img, min_int, W, L, sa, so, nxt, nct, k);
+  const int nxb = (W + NT - 1) / NT;
+  const i64 nblk = (i64)nxb * n_other;
+  hipLaunchKernelGGL((k_box_walk<R, NORM, TS>), dim3((unsigned)nblk), dim3(NT), 0, s, src, dst,
+                     norm, img, min_int, W, L, sa, so, nxb, k);
   return hipGetLastError();
 }
 

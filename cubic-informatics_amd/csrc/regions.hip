@@ -116,74 +116,164 @@ __device__ __forceinline__ float fdecode(uint32_t k) {
   return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
 }
 
-// Each wave takes 64 consecutive x of one row.  Runs of equal region id along x are
-// reduced inside the wave first (segmented shuffle reduction, every lane of a run ends on
-// the same last lane), then only the first lane of each run issues the atomics: the
-// atomics per candidate voxel drop by the mean run length.
+// A work-group takes a brick of 64 x BY x BZ voxels; each of its four waves walks the
+// rows (y, z) of its y.  Inside a row, runs of equal region id along x are reduced with a
+// segmented shuffle reduction (all lanes of a run finish on its last lane's value); the
+// first lane of each run then folds the run into a per-work-group hash table in LDS
+// keyed by region id.  Finally each occupied slot issues one set of global atomics, so
+// a region pays one set per brick it touches instead of one per voxel.  A run that
+// finds the table full falls back to global atomics directly (128 slots: at most 128
+// separate 26-connected regions fit in a 64 x 4 x 4 brick).
+constexpr int BY = 4, BZ = 4, HSB = 7, HS = 1 << HSB;
+constexpr uint32_t EMPTY = 0xFFFFFFFFu;
+
+struct RunSums {
+  uint32_t cnt, sx, sy, sz, isum, imax, nkey;
+  double h[6], ns;
+};
+
+__device__ __forceinline__ void flush_global(RegionAcc* a, const RunSums& v, int is_ave,
+                                             bool has_img) {
+  atomicAdd(&a->cnt, v.cnt);
+  atomicAdd(&a->sx, (unsigned long long)v.sx);
+  atomicAdd(&a->sy, (unsigned long long)v.sy);
+  atomicAdd(&a->sz, (unsigned long long)v.sz);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) unsafeAtomicAdd(&a->h[k], v.h[k]);
+  if (is_ave)
+    unsafeAtomicAdd(&a->nsum, v.ns);
+  else
+    atomicMax(&a->nmax_key, v.nkey);
+  if (has_img) {
+    atomicAdd(&a->isum, (unsigned long long)v.isum);
+    atomicMax(&a->imax, v.imax);
+  }
+}
+
 template <typename TS>
 __global__ __launch_bounds__(NT) void k_accum(const uint32_t* __restrict__ L,
                                               const uint32_t* __restrict__ rid,
                                               const TS* __restrict__ hess,
                                               const TS* __restrict__ norm,
                                               const uint16_t* __restrict__ img, RegionAcc* acc,
-                                              int W, int H, i64 nrows, i64 N, int is_ave) {
-  const int lane = threadIdx.x & 63;
-  const int nxw = (W + 63) >> 6;
-  const i64 wid = (i64)blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
-  const i64 row = wid / nxw;
-  if (row >= nrows) return;  // whole wave
-  const int x = (int)(wid - row * nxw) * 64 + lane;
-  const i64 i = row * W + x;
-  const uint32_t l = x < W ? L[i] : 0u;
-  if (__ballot(l != 0) == 0ull) return;  // whole wave: background only
-  const uint32_t r = l ? rid[l] : 0u;
-  const uint32_t rprev = __shfl_up(r, 1, 64);
-  const bool head = lane == 0 || r != rprev;
-  const unsigned long long heads = __ballot(head);
-  const unsigned long long later = lane == 63 ? 0ull : (heads & (~0ull << (lane + 1)));
-  const int run_end = later ? (__ffsll((long long)later) - 2) : 63;
-
-  uint32_t cnt = l ? 1u : 0u, sx = l ? (uint32_t)x : 0u, isum = 0u, imax = 0u, nkey = 0u;
-  double h[6], ns = 0.0;
+                                              int W, int H, int D, i64 N, int nbx, int nby,
+                                              int is_ave) {
+  __shared__ uint32_t key[HS];
+  __shared__ uint32_t tcnt[HS], tsx[HS], tsy[HS], tsz[HS], tis[HS], tim[HS], tnk[HS];
+  __shared__ double th[6][HS], tns[HS];
+  for (int t = threadIdx.x; t < HS; t += NT) {
+    key[t] = EMPTY;
+    tcnt[t] = tsx[t] = tsy[t] = tsz[t] = tis[t] = tim[t] = tnk[t] = 0u;
 #pragma unroll
-  for (int k = 0; k < 6; ++k) h[k] = l ? (double)ld(hess, (i64)k * N + i) : 0.0;
-  if (l) {
-    const float v = ld(norm, i);
-    if (is_ave) ns = (double)v; else nkey = fkey(v);
-    if (img) { isum = img[i]; imax = isum; }
+    for (int k = 0; k < 6; ++k) th[k][t] = 0.0;
+    tns[t] = 0.0;
+  }
+  unsigned b = blockIdx.x;
+  const int bx = b % nbx; b /= nbx;
+  const int by = b % nby; b /= nby;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int x = bx * 64 + lane;
+  const int y = by * BY + wave;
+  const int zb = (int)b * BZ;
+  const i64 WH = (i64)W * H;
+  const bool has_img = img != nullptr;
+  // issue the loads of all BZ rows before any use: three dependent rounds in total
+  uint32_t l[BZ], r[BZ];
+#pragma unroll
+  for (int zz = 0; zz < BZ; ++zz) {
+    const int z = zb + zz;
+    l[zz] = (x < W && y < H && z < D) ? L[(i64)z * WH + (i64)y * W + x] : 0u;
   }
 #pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const bool take = lane + d <= run_end;
-    const uint32_t c2 = __shfl_down(cnt, d, 64), x2 = __shfl_down(sx, d, 64);
-    const uint32_t i2 = __shfl_down(isum, d, 64), m2 = __shfl_down(imax, d, 64);
-    const uint32_t k2 = __shfl_down(nkey, d, 64);
-    const double n2 = __shfl_down(ns, d, 64);
-    double h2[6];
+  for (int zz = 0; zz < BZ; ++zz) r[zz] = l[zz] ? rid[l[zz]] : 0u;
+  float hv[BZ][6], nv[BZ];
+  uint32_t iv[BZ];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) h2[k] = __shfl_down(h[k], d, 64);
-    if (take) {
-      cnt += c2; sx += x2; isum += i2; imax = max(imax, m2); nkey = max(nkey, k2); ns += n2;
+  for (int zz = 0; zz < BZ; ++zz) {
+    const i64 i = (i64)(zb + zz) * WH + (i64)y * W + x;
+    const bool on = l[zz] != 0u;
 #pragma unroll
-      for (int k = 0; k < 6; ++k) h[k] += h2[k];
+    for (int k = 0; k < 6; ++k) hv[zz][k] = on ? ld(hess, (i64)k * N + i) : 0.0f;
+    nv[zz] = on ? ld(norm, i) : 0.0f;
+    iv[zz] = (on && has_img) ? (uint32_t)img[i] : 0u;
+  }
+  __syncthreads();  // table initialised
+#pragma unroll
+  for (int zz = 0; zz < BZ; ++zz) {
+    const uint32_t lz = l[zz], rz = r[zz];
+    if (__ballot(lz != 0) == 0ull) continue;  // wave-uniform
+    const uint32_t rprev = __shfl_up(rz, 1, 64);
+    const bool head = lane == 0 || rz != rprev;
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long later = lane == 63 ? 0ull : (heads & (~0ull << (lane + 1)));
+    const int run_end = later ? (__ffsll((long long)later) - 2) : 63;
+    RunSums v;
+    v.cnt = lz ? 1u : 0u;
+    v.sx = lz ? (uint32_t)x : 0u;
+    v.isum = iv[zz];
+    v.imax = iv[zz];
+    v.nkey = (lz && !is_ave) ? fkey(nv[zz]) : 0u;
+    v.ns = (lz && is_ave) ? (double)nv[zz] : 0.0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) v.h[k] = (double)hv[zz][k];
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const bool take = lane + d <= run_end;
+      const uint32_t c2 = __shfl_down(v.cnt, d, 64), x2 = __shfl_down(v.sx, d, 64);
+      const uint32_t i2 = __shfl_down(v.isum, d, 64), m2 = __shfl_down(v.imax, d, 64);
+      const uint32_t k2 = __shfl_down(v.nkey, d, 64);
+      const double n2 = __shfl_down(v.ns, d, 64);
+      double h2[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) h2[k] = __shfl_down(v.h[k], d, 64);
+      if (take) {
+        v.cnt += c2; v.sx += x2; v.isum += i2; v.imax = max(v.imax, m2);
+        v.nkey = max(v.nkey, k2); v.ns += n2;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) v.h[k] += h2[k];
+      }
+    }
+    if (!head || !lz) continue;
+    v.sy = v.cnt * (uint32_t)y;
+    v.sz = v.cnt * (uint32_t)(zb + zz);
+    uint32_t slot = (rz * 2654435761u) >> (32 - HSB);
+    int probes = 0;
+    bool placed = false;
+    while (probes < HS) {
+      const uint32_t k0 = atomicCAS(&key[slot], EMPTY, rz);
+      if (k0 == EMPTY || k0 == rz) { placed = true; break; }
+      slot = (slot + 1) & (HS - 1);
+      ++probes;
+    }
+    if (!placed) {
+      flush_global(acc + rz, v, is_ave, has_img);
+      continue;
+    }
+    atomicAdd(&tcnt[slot], v.cnt);
+    atomicAdd(&tsx[slot], v.sx);
+    atomicAdd(&tsy[slot], v.sy);
+    atomicAdd(&tsz[slot], v.sz);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) unsafeAtomicAdd(&th[k][slot], v.h[k]);
+    if (is_ave)
+      unsafeAtomicAdd(&tns[slot], v.ns);
+    else
+      atomicMax(&tnk[slot], v.nkey);
+    if (has_img) {
+      atomicAdd(&tis[slot], v.isum);
+      atomicMax(&tim[slot], v.imax);
     }
   }
-  if (!head || !l) return;
-  const unsigned y = (unsigned)(row % H), z = (unsigned)(row / H);
-  RegionAcc* a = acc + r;
-  atomicAdd(&a->cnt, cnt);
-  atomicAdd(&a->sx, (unsigned long long)sx);
-  atomicAdd(&a->sy, (unsigned long long)cnt * y);
-  atomicAdd(&a->sz, (unsigned long long)cnt * z);
+  __syncthreads();
+  for (int t = threadIdx.x; t < HS; t += NT) {
+    const uint32_t rr = key[t];
+    if (rr == EMPTY) continue;
+    RunSums v;
+    v.cnt = tcnt[t]; v.sx = tsx[t]; v.sy = tsy[t]; v.sz = tsz[t];
+    v.isum = tis[t]; v.imax = tim[t]; v.nkey = tnk[t]; v.ns = tns[t];
 #pragma unroll
-  for (int k = 0; k < 6; ++k) unsafeAtomicAdd(&a->h[k], h[k]);
-  if (is_ave)
-    unsafeAtomicAdd(&a->nsum, ns);
-  else
-    atomicMax(&a->nmax_key, nkey);
-  if (img) {
-    atomicAdd(&a->isum, (unsigned long long)isum);
-    atomicMax(&a->imax, imax);
+    for (int k = 0; k < 6; ++k) v.h[k] = th[k][t];
+    flush_global(acc + rr, v, is_ave, has_img);
   }
 }
 
@@ -289,12 +379,11 @@ hipError_t launch_region_accum(const uint32_t* labels, const uint32_t* rid, cons
                                const TS* norm, const uint16_t* img, RegionAcc* acc, i64 R,
                                int W, int H, int D, int is_ave, hipStream_t s) {
   (void)R;
-  const i64 nrows = (i64)H * D;
-  const i64 nwaves = nrows * ((W + 63) / 64);
-  const i64 nblk = (nwaves + NT / 64 - 1) / (NT / 64);
+  const int nbx = (W + 63) / 64, nby = (H + BY - 1) / BY, nbz = (D + BZ - 1) / BZ;
+  const i64 nblk = (i64)nbx * nby * nbz;
   const i64 N = (i64)W * H * D;
   hipLaunchKernelGGL((k_accum<TS>), dim3((unsigned)nblk), dim3(NT), 0, s, labels, rid, hess, norm,
-                     img, acc, W, H, nrows, N, is_ave);
+                     img, acc, W, H, D, N, nbx, nby, is_ave);
   mark("regions/accum", s);
   return hipGetLastError();
 }
