@@ -13,11 +13,12 @@ One step = the whole path on the resident stack: normalize -> DoG -> Hessian/can
 region count is inside, as in the reference).  Input generation and disk / PCIe I/O are
 outside the timed region.
 
-Prints one JSON line.  `value` is Gvoxel/s of the FP32 path; the FP16 form is measured
-in the same run and reported under "precision_forms".  `roofline` is the dominant
+Prints one JSON line.  `value` is Gvoxel/s of the FP32 path; the reduced-precision
+storage forms are measured in the same run and reported under "precision_forms" with
+their candidate-set agreement (Jaccard) against the FP32 mask.  `roofline` is the dominant
 kernel (event-timed in a profiled pass after the timed one) against the 8 TB/s HBM
 peak; `pipeline_roofline` is the whole step against the 80 B/voxel (46 B/voxel for
-FP16/BF16 storage) algorithmic traffic model.  `cpu_baseline` times the CPU oracle
+FP16/BF16 storage, 50 with an FP32 DoG) algorithmic traffic model.  `cpu_baseline` times the CPU oracle
 (OpenMP, all host threads) on one 32-slice substack of the same stack.
 """
 import argparse
@@ -31,19 +32,39 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "cubic-informatics_amd"))
 
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes per voxel of each kernel, run as its own pass: every tensor it
-# must read or write, once (FP32 storage / 2-byte storage of norm, DoG and Hessian).
+# Algorithmic bytes per voxel of each kernel, run as its own pass: every tensor it must
+# read or write, once.  sn, sd, sh: element size of the stored normalized volume, DoG and
+# Hessian elements for the storage form being measured.
 KERNEL_BYTES = {
-    "normalize/minmax_z": (6, 6), "normalize/minmax_y": (8, 8),
-    "normalize/minmax_x_box_x": (12, 12), "normalize/box_y": (16, 16),
-    "normalize/box_z_norm": (14, 12), "normalize/copy": (6, 4),
-    "dog/gauss_x": (12, 10), "dog/gauss_y": (16, 16), "dog/gauss_z_dog": (12, 10),
-    "hessian/hessian_pd": (29, 15),
-    "ccl/init": (5, 5), "ccl/union": (1, 1), "ccl/flatten": (8, 8),
-    "regions/count_roots": (4, 4), "regions/scan": (0, 0), "regions/zero": (0, 0),
-    "regions/assign_ids": (4, 4), "regions/accum": (32, 18), "regions/finalize": (0, 0),
+    "normalize/minmax_z": lambda sn, sd, sh: 2 + 4,
+    "normalize/minmax_y": lambda sn, sd, sh: 4 + 4,
+    "normalize/minmax_x_box_x": lambda sn, sd, sh: 4 + 8,
+    "normalize/box_y": lambda sn, sd, sh: 8 + 8,
+    "normalize/box_z_norm": lambda sn, sd, sh: 8 + 2 + sn,
+    "normalize/copy": lambda sn, sd, sh: 2 + sn,
+    "dog/gauss_x": lambda sn, sd, sh: sn + 8,
+    "dog/gauss_y": lambda sn, sd, sh: 8 + 8,
+    "dog/gauss_z_dog": lambda sn, sd, sh: 8 + sd,
+    "hessian/hessian_pd": lambda sn, sd, sh: sd + 6 * sh + 1,
+    "ccl/local": lambda sn, sd, sh: 1 + 4, "ccl/merge": lambda sn, sd, sh: 1,
+    "ccl/init": lambda sn, sd, sh: 1 + 4, "ccl/union": lambda sn, sd, sh: 1,
+    "ccl/flatten": lambda sn, sd, sh: 4 + 4,
+    "regions/count_roots": lambda sn, sd, sh: 4, "regions/scan": lambda sn, sd, sh: 0,
+    "regions/zero": lambda sn, sd, sh: 0, "regions/assign_ids": lambda sn, sd, sh: 4,
+    "regions/accum": lambda sn, sd, sh: 4 + 6 * sh + sn,
+    "regions/finalize": lambda sn, sd, sh: 0,
 }
-PIPELINE_BYTES = {"fp32": 80, "fp16": 46, "bf16": 46}
+# (norm, dog, hessian) element sizes of each storage form
+FORM_SIZES = {"fp32": (4, 4, 4), "fp16": (2, 2, 2), "bf16": (2, 2, 2),
+              "fp16_dog32": (2, 4, 2), "bf16_dog32": (2, 4, 2)}
+
+
+def pipeline_bytes(form):
+    """Stage-boundary model: img 2 + norm (write, read by DoG, read by regions) + dog (write,
+    read by Hessian) + 6 Hessian elements (write, read by regions) + mask 2 + labels 8."""
+    sn, sd, sh = FORM_SIZES[form]
+    return 2 + 3 * sn + 2 * sd + 12 * sh + 2 + 8
+
 
 C1 = dict(name="c1", rxy=10, rz=10, min_intensity=0, sigma_xy1=2.0, sigma_z1=1.6,
           sigma_xy2=3.2, sigma_z2=2.56, truncate=4.0, gamma=1.0)
@@ -115,7 +136,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--forms", default="fp32,fp16", help="storage forms to time; first is `value`")
+    ap.add_argument("--forms", default="fp32,fp16,fp16_dog32",
+                    help="storage forms to time; the first one gives `value`")
     ap.add_argument("--depth", type=int, default=512, help="stack depth (z) of the workload")
     ap.add_argument("--cpu-depth", type=int, default=32, help="substack depth of the CPU sample; 0 skips")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -165,10 +187,9 @@ def main():
     value = N / (ms_step * 1e-3) / 1e9
     kernels = r["kernels"]
     dom = max(kernels, key=kernels.get)
-    bi = 0 if main_f == "fp32" else 1
-    dom_bytes = KERNEL_BYTES.get(dom, (0, 0))[bi] * N
-    achieved = dom_bytes / (kernels[dom] * 1e-3) / 1e9
-    pipe_gbs = PIPELINE_BYTES[main_f] * N / (ms_step * 1e-3) / 1e9
+    kb = KERNEL_BYTES.get(dom, lambda sn, sd, sh: 0)(*FORM_SIZES[main_f])
+    achieved = kb * N / (kernels[dom] * 1e-3) / 1e9
+    pipe_gbs = pipeline_bytes(main_f) * N / (ms_step * 1e-3) / 1e9
 
     forms_out = {}
     for f in forms[1:]:
@@ -181,7 +202,7 @@ def main():
                             speedup_vs_fp32=ms_step / msf if main_f == "fp32" else None,
                             candidate_jaccard_vs_fp32=inter / max(union, 1),
                             num_regions=rf["R"],
-                            pipeline_roofline_frac=PIPELINE_BYTES[f] * N / (msf * 1e-3) / 1e9 / HBM_PEAK_GBS)
+                            pipeline_roofline_frac=pipeline_bytes(f) * N / (msf * 1e-3) / 1e9 / HBM_PEAK_GBS)
 
     out = {
         "metric": "Gvoxel/s HDoG pipeline, 2160x2560x512 u16; FP16 vs FP32 at fixed accuracy",
@@ -196,9 +217,9 @@ def main():
                    "truncate": C1["truncate"], "storage": main_f},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "alg_bytes_per_voxel": KERNEL_BYTES.get(dom, (0, 0))[bi],
+                     "alg_bytes_per_voxel": kb,
                      "avg_ms": kernels[dom]},
-        "pipeline_roofline": {"bytes_per_voxel": PIPELINE_BYTES[main_f], "achieved": pipe_gbs,
+        "pipeline_roofline": {"bytes_per_voxel": pipeline_bytes(main_f), "achieved": pipe_gbs,
                               "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": pipe_gbs / HBM_PEAK_GBS},
         "num_regions": r["R"],
         "candidate_fraction": r["cand"] / N,

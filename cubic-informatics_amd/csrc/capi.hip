@@ -50,11 +50,11 @@ int fail(hdog_ctx* c, int code, const char* fmt, ...) {
   return code;
 }
 
-#define HIPCHK(expr)                                                                  \
-  do {                                                                                \
-    hipError_t e_ = (expr);                                                           \
-    if (e_ != hipSuccess)                                                             \
-      return fail(ctx, HDOG_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));  \
+#define HIPCHK(...)                                                                      \
+  do {                                                                                   \
+    hipError_t e_ = (__VA_ARGS__);                                                       \
+    if (e_ != hipSuccess)                                                                \
+      return fail(ctx, HDOG_ERR_HIP, "%s failed: %s", #__VA_ARGS__, hipGetErrorString(e_)); \
   } while (0)
 
 hipError_t ensure(Buf& b, size_t bytes) {
@@ -121,9 +121,9 @@ int ensure_regions(hdog_ctx* ctx, i64 R) {
   return HDOG_OK;
 }
 
-template <typename TS>
-int regional_features_t(hdog_ctx* ctx, hipStream_t s, const uint32_t* labels, const TS* hess,
-                        const TS* norm, const uint16_t* img, int W, int H, int D, int is_ave,
+template <typename TH, typename TN>
+int regional_features_t(hdog_ctx* ctx, hipStream_t s, const uint32_t* labels, const TH* hess,
+                        const TN* norm, const uint16_t* img, int W, int H, int D, int is_ave,
                         int64_t* num_regions) {
   const i64 N = (i64)W * H * D;
   HIPCHK(ensure(ctx->rid, (size_t)N * 4));
@@ -141,8 +141,8 @@ int regional_features_t(hdog_ctx* ctx, hipStream_t s, const uint32_t* labels, co
   RegionAcc* acc = (RegionAcc*)ctx->acc.p;
   HIPCHK(launch_region_assign(labels, (const uint32_t*)ctx->bcnt.p, (uint32_t*)ctx->rid.p, acc, R,
                               N, s));
-  HIPCHK(launch_region_accum<TS>(labels, (const uint32_t*)ctx->rid.p, hess, norm, img, acc, R, W, H,
-                                 D, is_ave, s));
+  HIPCHK(launch_region_accum<TH, TN>(labels, (const uint32_t*)ctx->rid.p, hess, norm, img, acc, R,
+                                     W, H, D, is_ave, s));
   const i64 cap = ctx->out_cap;
   RegionOut o;
   o.label = (uint32_t*)ctx->o_label.p;
@@ -163,9 +163,9 @@ int regional_features_t(hdog_ctx* ctx, hipStream_t s, const uint32_t* labels, co
   return HDOG_OK;
 }
 
-template <typename TS>
-int run_stages(hdog_ctx* ctx, hipStream_t s, const hdog_params* p, const uint16_t* img, TS* norm,
-               TS* dog, TS* hess, uint8_t* pd, uint32_t* labels, int64_t* num_regions) {
+template <typename TN, typename TD, typename TH>
+int run_stages(hdog_ctx* ctx, hipStream_t s, const hdog_params* p, const uint16_t* img, TN* norm,
+               TD* dog, TH* hess, uint8_t* pd, uint32_t* labels, int64_t* num_regions) {
   const int W = p->width, H = p->height, D = p->depth;
   const i64 N = (i64)W * H * D;
   HIPCHK(ensure(ctx->tmp1, (size_t)N * 8));
@@ -177,15 +177,15 @@ int run_stages(hdog_ctx* ctx, hipStream_t s, const hdog_params* p, const uint16_
   ctx->ev_valid = false;
   ctx->pool.n = 0;
   mark(nullptr, s);
-  HIPCHK(launch_normalize<TS>(img, norm, ctx->tmp1.p, ctx->tmp2.p, p->min_intensity, W, H, D,
+  HIPCHK(launch_normalize<TN>(img, norm, ctx->tmp1.p, ctx->tmp2.p, p->min_intensity, W, H, D,
                               p->radius_xy, p->radius_z, s));
   const float nz = powf(ctx->sigma[0], p->gamma * 2) * powf(ctx->sigma[1], p->gamma);
-  HIPCHK(launch_dog<TS>(norm, dog, ctx->tmp1.p, ctx->tmp2.p, (const float*)ctx->weights.p, ctx->r,
-                        nz, W, H, D, s));
-  HIPCHK(launch_hessian<TS>(dog, hess, pd, W, H, D, s));
+  HIPCHK(launch_dog<TN, TD>(norm, dog, ctx->tmp1.p, ctx->tmp2.p, (const float*)ctx->weights.p,
+                            ctx->r, nz, W, H, D, s));
+  HIPCHK(launch_hessian<TD, TH>(dog, hess, pd, W, H, D, s));
   HIPCHK(launch_ccl(pd, labels, W, H, D, s));
-  int rc = regional_features_t<TS>(ctx, s, labels, hess, norm, img, W, H, D, p->is_ave_mode,
-                                   num_regions);
+  int rc = regional_features_t<TH, TN>(ctx, s, labels, hess, norm, img, W, H, D, p->is_ave_mode,
+                                       num_regions);
   if (rc) return rc;
   ctx->ev_valid = ctx->prof;
   return HDOG_OK;
@@ -199,7 +199,7 @@ int check_params(hdog_ctx* ctx, const hdog_params* p) {
       p->radius_z > kMaxNormRadius)
     return fail(ctx, HDOG_ERR_ARG, "normalization radii (%d, %d) outside [0, %d]", p->radius_xy,
                 p->radius_z, kMaxNormRadius);
-  if (p->precision < HDOG_FP32 || p->precision > HDOG_BF16)
+  if (p->precision < HDOG_FP32 || p->precision > HDOG_BF16_DOG32)
     return fail(ctx, HDOG_ERR_ARG, "unknown precision %d", p->precision);
   return HDOG_OK;
 }
@@ -336,7 +336,7 @@ int hdog_dog3d(hdog_ctx* ctx, void* stream, const float* d_norm, float* d_dog, i
   HIPCHK(ensure(ctx->tmp1, (size_t)N * 8));
   HIPCHK(ensure(ctx->tmp2, (size_t)N * 8));
   const float nz = powf(ctx->sigma[0], gamma * 2) * powf(ctx->sigma[1], gamma);
-  HIPCHK(launch_dog<float>(d_norm, d_dog, ctx->tmp1.p, ctx->tmp2.p, (const float*)ctx->weights.p,
+  HIPCHK(launch_dog<float, float>(d_norm, d_dog, ctx->tmp1.p, ctx->tmp2.p, (const float*)ctx->weights.p,
                            ctx->r, nz, width, height, depth, (hipStream_t)stream));
   return HDOG_OK;
 }
@@ -348,7 +348,8 @@ int hdog_hessian_pd(hdog_ctx* ctx, void* stream, const float* d_dog, float* d_he
   if (rc) return rc;
   if (!d_dog || !d_hessian || !d_pd) return fail(ctx, HDOG_ERR_ARG, "NULL device pointer");
   HIPCHK(hipSetDevice(ctx->device));
-  HIPCHK(launch_hessian<float>(d_dog, d_hessian, d_pd, width, height, depth, (hipStream_t)stream));
+  HIPCHK(launch_hessian<float, float>(d_dog, d_hessian, d_pd, width, height, depth,
+                                      (hipStream_t)stream));
   return HDOG_OK;
 }
 
@@ -372,8 +373,8 @@ int hdog_regional_features(hdog_ctx* ctx, void* stream, const uint32_t* d_labels
   if (rc) return rc;
   if (!d_labels || !d_hessian || !d_norm) return fail(ctx, HDOG_ERR_ARG, "NULL device pointer");
   HIPCHK(hipSetDevice(ctx->device));
-  return regional_features_t<float>(ctx, (hipStream_t)stream, d_labels, d_hessian, d_norm, d_img,
-                                    width, height, depth, is_ave_mode, num_regions);
+  return regional_features_t<float, float>(ctx, (hipStream_t)stream, d_labels, d_hessian, d_norm,
+                                           d_img, width, height, depth, is_ave_mode, num_regions);
 }
 
 int hdog_regions_download(hdog_ctx* ctx, void* stream, hdog_regions* out) {
@@ -428,24 +429,32 @@ int hdog_run_device(hdog_ctx* ctx, void* stream, const hdog_params* p, const uin
   HIPCHK(hipSetDevice(ctx->device));
   if ((rc = ensure_gauss(ctx, p))) return rc;
   const i64 N = (i64)p->width * p->height * p->depth;
-  const size_t es = p->precision == HDOG_FP32 ? 4 : 2;
+  const size_t es = p->precision == HDOG_FP32 ? 4 : 2;  // norm and hessian element size
+  const size_t ed = (p->precision == HDOG_FP16 || p->precision == HDOG_BF16) ? 2 : 4;  // dog
   if (!d_norm) { HIPCHK(ensure(ctx->norm, N * es)); d_norm = ctx->norm.p; }
-  if (!d_dog) { HIPCHK(ensure(ctx->dog, N * es)); d_dog = ctx->dog.p; }
+  if (!d_dog) { HIPCHK(ensure(ctx->dog, N * ed)); d_dog = ctx->dog.p; }
   if (!d_hessian) { HIPCHK(ensure(ctx->hess, 6 * N * es)); d_hessian = ctx->hess.p; }
   if (!d_pd) { HIPCHK(ensure(ctx->pd, N)); d_pd = (uint8_t*)ctx->pd.p; }
   if (!d_labels) { HIPCHK(ensure(ctx->labels, N * 4)); d_labels = (uint32_t*)ctx->labels.p; }
   hipStream_t s = (hipStream_t)stream;
+  typedef __half h16;
+  typedef __hip_bfloat16 b16;
   switch (p->precision) {
     case HDOG_FP16:
-      return run_stages<__half>(ctx, s, p, d_img, (__half*)d_norm, (__half*)d_dog,
-                                (__half*)d_hessian, d_pd, d_labels, num_regions);
+      return run_stages<h16, h16, h16>(ctx, s, p, d_img, (h16*)d_norm, (h16*)d_dog, (h16*)d_hessian,
+                                       d_pd, d_labels, num_regions);
     case HDOG_BF16:
-      return run_stages<__hip_bfloat16>(ctx, s, p, d_img, (__hip_bfloat16*)d_norm,
-                                        (__hip_bfloat16*)d_dog, (__hip_bfloat16*)d_hessian, d_pd,
-                                        d_labels, num_regions);
+      return run_stages<b16, b16, b16>(ctx, s, p, d_img, (b16*)d_norm, (b16*)d_dog, (b16*)d_hessian,
+                                       d_pd, d_labels, num_regions);
+    case HDOG_FP16_DOG32:
+      return run_stages<h16, float, h16>(ctx, s, p, d_img, (h16*)d_norm, (float*)d_dog,
+                                         (h16*)d_hessian, d_pd, d_labels, num_regions);
+    case HDOG_BF16_DOG32:
+      return run_stages<b16, float, b16>(ctx, s, p, d_img, (b16*)d_norm, (float*)d_dog,
+                                         (b16*)d_hessian, d_pd, d_labels, num_regions);
     default:
-      return run_stages<float>(ctx, s, p, d_img, (float*)d_norm, (float*)d_dog, (float*)d_hessian,
-                               d_pd, d_labels, num_regions);
+      return run_stages<float, float, float>(ctx, s, p, d_img, (float*)d_norm, (float*)d_dog,
+                                             (float*)d_hessian, d_pd, d_labels, num_regions);
   }
 }
 

@@ -114,10 +114,23 @@ __device__ __forceinline__ uint32_t lds_find(const uint32_t* lab, uint32_t x) {
   return x;
 }
 
+// union phase only: path halving keeps the LDS trees shallow (same argument as
+// find_root: a parent is only ever replaced by one of its ancestors)
+__device__ __forceinline__ uint32_t lds_find_halve(uint32_t* lab, uint32_t x) {
+  uint32_t p = lab[x];
+  while (p != x) {
+    const uint32_t gp = lab[p];
+    if (gp != p) lab[x] = gp;
+    x = p;
+    p = gp;
+  }
+  return x;
+}
+
 __device__ __forceinline__ uint32_t lds_unite(uint32_t* lab, uint32_t a, uint32_t b) {
   while (true) {
-    a = lds_find(lab, a);
-    b = lds_find(lab, b);
+    a = lds_find_halve(lab, a);
+    b = lds_find_halve(lab, b);
     if (a == b) return a;
     if (a < b) { const uint32_t t = a; a = b; b = t; }
     const uint32_t old = atomicMin(lab + a, b);
@@ -200,7 +213,7 @@ __global__ __launch_bounds__(NT) void k_ccl_merge(const uint8_t* __restrict__ pd
   const int by = b % nby; b /= nby;
   const int x0 = bx * CX, y0 = by * CY, z0 = (int)b * CZ;
   const i64 WH = (i64)W * H, N = WH * D;
-#pragma unroll
+#pragma unroll 1
   for (int k = 0; k < CPT; ++k) {
     const int v = threadIdx.x + k * NT;
     const int lx = v & (CX - 1), ly = (v / CX) & (CY - 1), lz = v / (CX * CY);
@@ -212,7 +225,10 @@ __global__ __launch_bounds__(NT) void k_ccl_merge(const uint8_t* __restrict__ pd
       continue;
     const i64 g = (i64)gz * WH + (i64)gy * W + gx;
     if (g == 0 || !pd[g]) continue;
-    uint32_t root = L[g];
+    // neighbours linked here: those leaving the tile or the volume (wrap); their mask bytes
+    // are loaded together, then their labels, before any union
+    i64 nj[13];
+    uint8_t m[13];
 #pragma unroll
     for (int q = 0; q < 13; ++q) {
       int dx, dy, dz;
@@ -220,10 +236,17 @@ __global__ __launch_bounds__(NT) void k_ccl_merge(const uint8_t* __restrict__ pd
       const int nx = lx + dx, ny = ly + dy, nz = lz + dz;
       const bool inside_volume = gx + dx >= 0 && gx + dx < W && gy + dy >= 0 && gy + dy < H &&
                                  gz + dz < D;
-      if (inside_volume && nx >= 0 && nx < CX && ny >= 0 && ny < CY && nz < CZ) continue;
-      const i64 j = g + (i64)dz * WH + (i64)dy * W + dx;  // linear offset, wraps like the reference
-      if (j < N && pd[j] && L[j] != root) root = unite(L, root, (uint32_t)j);
+      const bool local = inside_volume && nx >= 0 && nx < CX && ny >= 0 && ny < CY && nz < CZ;
+      nj[q] = g + (i64)dz * WH + (i64)dy * W + dx;  // linear offset, wraps like the reference
+      m[q] = (!local && nj[q] < N) ? pd[nj[q]] : (uint8_t)0;
     }
+    uint32_t lj[13];
+#pragma unroll
+    for (int q = 0; q < 13; ++q) lj[q] = m[q] ? L[nj[q]] : 0u;
+    uint32_t root = L[g];
+#pragma unroll
+    for (int q = 0; q < 13; ++q)
+      if (m[q] && lj[q] != root) root = unite(L, root, (uint32_t)nj[q]);
   }
 }
 

@@ -247,8 +247,8 @@ hipError_t gauss_line(const float2* src, float2* dst, TS* dog, const float* w1, 
 
 }  // namespace
 
-template <typename TS>
-hipError_t launch_dog(const TS* norm, TS* dog, void* tmp1, void* tmp2, const float* weights,
+template <typename TN, typename TD>
+hipError_t launch_dog(const TN* norm, TD* dog, void* tmp1, void* tmp2, const float* weights,
                       const int r[4], float normalizer, int W, int H, int D, hipStream_t s) {
   const i64 WH = (i64)W * H;
   const float* wxy1 = weights;
@@ -256,26 +256,28 @@ hipError_t launch_dog(const TS* norm, TS* dog, void* tmp1, void* tmp2, const flo
   const float* wxy2 = weights + 2 * kMaxTaps;
   const float* wz2 = weights + 3 * kMaxTaps;
   hipError_t e;
-  if ((e = gauss_x<TS>(norm, (float2*)tmp1, wxy1, wxy2, r[0], r[2], W, (i64)H * D, s)) !=
+  if ((e = gauss_x<TN>(norm, (float2*)tmp1, wxy1, wxy2, r[0], r[2], W, (i64)H * D, s)) !=
       hipSuccess)
     return e;
   mark("dog/gauss_x", s);
-  if ((e = gauss_line<false, TS>((const float2*)tmp1, (float2*)tmp2, nullptr, wxy1, wxy2, r[0],
+  if ((e = gauss_line<false, TD>((const float2*)tmp1, (float2*)tmp2, nullptr, wxy1, wxy2, r[0],
                                  r[2], 0.0f, W, H, W, WH, D, s)) != hipSuccess)
     return e;
   mark("dog/gauss_y", s);
-  e = gauss_line<true, TS>((const float2*)tmp2, nullptr, dog, wz1, wz2, r[1], r[3], normalizer, W,
+  e = gauss_line<true, TD>((const float2*)tmp2, nullptr, dog, wz1, wz2, r[1], r[3], normalizer, W,
                            D, WH, W, H, s);
   mark("dog/gauss_z_dog", s);
   return e;
 }
 
-template hipError_t launch_dog<float>(const float*, float*, void*, void*, const float*,
-                                      const int*, float, int, int, int, hipStream_t);
-template hipError_t launch_dog<__half>(const __half*, __half*, void*, void*, const float*,
-                                       const int*, float, int, int, int, hipStream_t);
-template hipError_t launch_dog<__hip_bfloat16>(const __hip_bfloat16*, __hip_bfloat16*, void*,
-                                               void*, const float*, const int*, float, int, int,
-                                               int, hipStream_t);
+#define HDOG_INST_DOG(TN, TD)                                                                \
+  template hipError_t launch_dog<TN, TD>(const TN*, TD*, void*, void*, const float*, const int*, \
+                                         float, int, int, int, hipStream_t);
+HDOG_INST_DOG(float, float)
+HDOG_INST_DOG(__half, __half)
+HDOG_INST_DOG(__hip_bfloat16, __hip_bfloat16)
+HDOG_INST_DOG(__half, float)
+HDOG_INST_DOG(__hip_bfloat16, float)
+#undef HDOG_INST_DOG
 
 }  // namespace hdog

@@ -91,11 +91,11 @@ hipError_t launch_normalize(const uint16_t* img, TS* norm, void* tmp1, void* tmp
                             float min_intensity, int W, int H, int D, int rxy, int rz,
                             hipStream_t s);
 // DoG stage; weights: device array [4][kMaxTaps] (xy1, z1, xy2, z2); tmp1/tmp2 >= 8*N bytes
-template <typename TS>
-hipError_t launch_dog(const TS* norm, TS* dog, void* tmp1, void* tmp2, const float* weights,
+template <typename TN, typename TD>
+hipError_t launch_dog(const TN* norm, TD* dog, void* tmp1, void* tmp2, const float* weights,
                       const int r[4], float normalizer, int W, int H, int D, hipStream_t s);
-template <typename TS>
-hipError_t launch_hessian(const TS* dog, TS* hess, uint8_t* pd, int W, int H, int D,
+template <typename TD, typename TH>
+hipError_t launch_hessian(const TD* dog, TH* hess, uint8_t* pd, int W, int H, int D,
                           hipStream_t s);
 // CCL: labels out; union-find in place
 hipError_t launch_ccl(const uint8_t* pd, uint32_t* labels, int W, int H, int D, hipStream_t s);
@@ -104,9 +104,9 @@ hipError_t launch_region_ids(const uint32_t* labels, uint32_t* rid, uint32_t* bl
                              uint32_t* d_R, i64 N, hipStream_t s, i64* nblocks_out);
 hipError_t launch_region_assign(const uint32_t* labels, const uint32_t* block_off, uint32_t* rid,
                                 RegionAcc* acc, i64 R, i64 N, hipStream_t s);
-template <typename TS>
-hipError_t launch_region_accum(const uint32_t* labels, const uint32_t* rid, const TS* hess,
-                               const TS* norm, const uint16_t* img, RegionAcc* acc, i64 R,
+template <typename TH, typename TN>
+hipError_t launch_region_accum(const uint32_t* labels, const uint32_t* rid, const TH* hess,
+                               const TN* norm, const uint16_t* img, RegionAcc* acc, i64 R,
                                int W, int H, int D, int is_ave, hipStream_t s);
 struct RegionOut {
   uint32_t* label; uint16_t* size; u64* voxels; float* cx; float* cy; float* cz;

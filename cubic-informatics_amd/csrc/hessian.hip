@@ -64,8 +64,8 @@ __device__ __forceinline__ uint8_t negdef_band(float xx, float xy, float xz, flo
 // Every dog value is read once per column (plus the xy halo, served by L2) instead of
 // (TZ+4)/TZ times; first derivatives outside the volume are zero as in the reference's
 // second pass.
-template <typename TS, bool BAND>
-__global__ __launch_bounds__(NT) void k_hessian(const TS* __restrict__ dog, TS* __restrict__ hess,
+template <typename TD, typename TS, bool BAND>
+__global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* __restrict__ hess,
                                                 uint8_t* __restrict__ pd, int W, int H, int D,
                                                 int nbx, int nby) {
   constexpr int DX = TX + 4, DY = TY + 4;        // dog slice with halo 2
@@ -195,23 +195,27 @@ __global__ __launch_bounds__(NT) void k_hessian(const TS* __restrict__ dog, TS* 
 
 }  // namespace
 
-template <typename TS>
-hipError_t launch_hessian(const TS* dog, TS* hess, uint8_t* pd, int W, int H, int D,
+template <typename TD, typename TH>
+hipError_t launch_hessian(const TD* dog, TH* hess, uint8_t* pd, int W, int H, int D,
                           hipStream_t s) {
   const int nbx = (W + TX - 1) / TX, nby = (H + TY - 1) / TY, nbz = (D + ZC - 1) / ZC;
   const i64 nblk = (i64)nbx * nby * nbz;
-  constexpr bool band = !std::is_same<TS, float>::value;
-  hipLaunchKernelGGL((k_hessian<TS, band>), dim3((unsigned)nblk), dim3(NT), 0, s, dog, hess, pd,
-                     W, H, D, nbx, nby);
+  // reduced-precision storage anywhere on the path: FP64 re-evaluation band for the minors
+  constexpr bool band = !std::is_same<TD, float>::value || !std::is_same<TH, float>::value;
+  hipLaunchKernelGGL((k_hessian<TD, TH, band>), dim3((unsigned)nblk), dim3(NT), 0, s, dog, hess,
+                     pd, W, H, D, nbx, nby);
   mark("hessian/hessian_pd", s);
   return hipGetLastError();
 }
 
-template hipError_t launch_hessian<float>(const float*, float*, uint8_t*, int, int, int,
-                                          hipStream_t);
-template hipError_t launch_hessian<__half>(const __half*, __half*, uint8_t*, int, int, int,
-                                           hipStream_t);
-template hipError_t launch_hessian<__hip_bfloat16>(const __hip_bfloat16*, __hip_bfloat16*,
-                                                   uint8_t*, int, int, int, hipStream_t);
+#define HDOG_INST_HESS(TD, TH)                                                          \
+  template hipError_t launch_hessian<TD, TH>(const TD*, TH*, uint8_t*, int, int, int, \
+                                             hipStream_t);
+HDOG_INST_HESS(float, float)
+HDOG_INST_HESS(__half, __half)
+HDOG_INST_HESS(__hip_bfloat16, __hip_bfloat16)
+HDOG_INST_HESS(float, __half)
+HDOG_INST_HESS(float, __hip_bfloat16)
+#undef HDOG_INST_HESS
 
 }  // namespace hdog

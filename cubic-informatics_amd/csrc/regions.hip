@@ -116,14 +116,13 @@ __device__ __forceinline__ float fdecode(uint32_t k) {
   return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
 }
 
-// A work-group takes a brick of 64 x BY x BZ voxels; each of its four waves walks the
-// rows (y, z) of its y.  Inside a row, runs of equal region id along x are reduced with a
-// segmented shuffle reduction (all lanes of a run finish on its last lane's value); the
-// first lane of each run then folds the run into a per-work-group hash table in LDS
-// keyed by region id.  Finally each occupied slot issues one set of global atomics, so
-// a region pays one set per brick it touches instead of one per voxel.  A run that
-// finds the table full falls back to global atomics directly (128 slots: at most 128
-// separate 26-connected regions fit in a 64 x 4 x 4 brick).
+// A work-group takes a brick of 64 x BY x BZ voxels (one row of 64 x per wave and z
+// step).  All loads of the brick are issued up front; every candidate voxel then folds
+// itself into a per-work-group hash table in LDS keyed by region id (LDS atomics, FP64
+// for the Hessian sums), and each occupied slot finally issues one set of global
+// atomics, so a region pays one set per brick it touches instead of one per voxel.  A
+// voxel that finds the table full falls back to global atomics directly (128 slots: at
+// most 128 separate 26-connected regions fit in a 64 x 4 x 4 brick).
 constexpr int BY = 4, BZ = 4, HSB = 7, HS = 1 << HSB;
 constexpr uint32_t EMPTY = 0xFFFFFFFFu;
 
@@ -150,11 +149,11 @@ __device__ __forceinline__ void flush_global(RegionAcc* a, const RunSums& v, int
   }
 }
 
-template <typename TS>
+template <typename TH, typename TN>
 __global__ __launch_bounds__(NT) void k_accum(const uint32_t* __restrict__ L,
                                               const uint32_t* __restrict__ rid,
-                                              const TS* __restrict__ hess,
-                                              const TS* __restrict__ norm,
+                                              const TH* __restrict__ hess,
+                                              const TN* __restrict__ norm,
                                               const uint16_t* __restrict__ img, RegionAcc* acc,
                                               int W, int H, int D, i64 N, int nbx, int nby,
                                               int is_ave) {
@@ -171,27 +170,25 @@ __global__ __launch_bounds__(NT) void k_accum(const uint32_t* __restrict__ L,
   unsigned b = blockIdx.x;
   const int bx = b % nbx; b /= nbx;
   const int by = b % nby; b /= nby;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int x = bx * 64 + lane;
-  const int y = by * BY + wave;
+  const int x = bx * 64 + (threadIdx.x & 63);
+  const int y = by * BY + (threadIdx.x >> 6);
   const int zb = (int)b * BZ;
   const i64 WH = (i64)W * H;
   const bool has_img = img != nullptr;
-  // issue the loads of all BZ rows before any use: three dependent rounds in total
+  // loads of all BZ rows issued before any use: two dependent rounds in total
   uint32_t l[BZ], r[BZ];
 #pragma unroll
   for (int zz = 0; zz < BZ; ++zz) {
     const int z = zb + zz;
     l[zz] = (x < W && y < H && z < D) ? L[(i64)z * WH + (i64)y * W + x] : 0u;
   }
-#pragma unroll
-  for (int zz = 0; zz < BZ; ++zz) r[zz] = l[zz] ? rid[l[zz]] : 0u;
   float hv[BZ][6], nv[BZ];
   uint32_t iv[BZ];
 #pragma unroll
   for (int zz = 0; zz < BZ; ++zz) {
     const i64 i = (i64)(zb + zz) * WH + (i64)y * W + x;
     const bool on = l[zz] != 0u;
+    r[zz] = on ? rid[l[zz]] : 0u;
 #pragma unroll
     for (int k = 0; k < 6; ++k) hv[zz][k] = on ? ld(hess, (i64)k * N + i) : 0.0f;
     nv[zz] = on ? ld(norm, i) : 0.0f;
@@ -200,42 +197,8 @@ __global__ __launch_bounds__(NT) void k_accum(const uint32_t* __restrict__ L,
   __syncthreads();  // table initialised
 #pragma unroll
   for (int zz = 0; zz < BZ; ++zz) {
-    const uint32_t lz = l[zz], rz = r[zz];
-    if (__ballot(lz != 0) == 0ull) continue;  // wave-uniform
-    const uint32_t rprev = __shfl_up(rz, 1, 64);
-    const bool head = lane == 0 || rz != rprev;
-    const unsigned long long heads = __ballot(head);
-    const unsigned long long later = lane == 63 ? 0ull : (heads & (~0ull << (lane + 1)));
-    const int run_end = later ? (__ffsll((long long)later) - 2) : 63;
-    RunSums v;
-    v.cnt = lz ? 1u : 0u;
-    v.sx = lz ? (uint32_t)x : 0u;
-    v.isum = iv[zz];
-    v.imax = iv[zz];
-    v.nkey = (lz && !is_ave) ? fkey(nv[zz]) : 0u;
-    v.ns = (lz && is_ave) ? (double)nv[zz] : 0.0;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) v.h[k] = (double)hv[zz][k];
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const bool take = lane + d <= run_end;
-      const uint32_t c2 = __shfl_down(v.cnt, d, 64), x2 = __shfl_down(v.sx, d, 64);
-      const uint32_t i2 = __shfl_down(v.isum, d, 64), m2 = __shfl_down(v.imax, d, 64);
-      const uint32_t k2 = __shfl_down(v.nkey, d, 64);
-      const double n2 = __shfl_down(v.ns, d, 64);
-      double h2[6];
-#pragma unroll
-      for (int k = 0; k < 6; ++k) h2[k] = __shfl_down(v.h[k], d, 64);
-      if (take) {
-        v.cnt += c2; v.sx += x2; v.isum += i2; v.imax = max(v.imax, m2);
-        v.nkey = max(v.nkey, k2); v.ns += n2;
-#pragma unroll
-        for (int k = 0; k < 6; ++k) v.h[k] += h2[k];
-      }
-    }
-    if (!head || !lz) continue;
-    v.sy = v.cnt * (uint32_t)y;
-    v.sz = v.cnt * (uint32_t)(zb + zz);
+    if (!l[zz]) continue;
+    const uint32_t rz = r[zz];
     uint32_t slot = (rz * 2654435761u) >> (32 - HSB);
     int probes = 0;
     bool placed = false;
@@ -245,23 +208,30 @@ __global__ __launch_bounds__(NT) void k_accum(const uint32_t* __restrict__ L,
       slot = (slot + 1) & (HS - 1);
       ++probes;
     }
-    if (!placed) {
+    if (!placed) {  // not reachable for this brick size (see above); kept for safety
+      RunSums v;
+      v.cnt = 1u; v.sx = (uint32_t)x; v.sy = (uint32_t)y; v.sz = (uint32_t)(zb + zz);
+      v.isum = iv[zz]; v.imax = iv[zz];
+      v.nkey = is_ave ? 0u : fkey(nv[zz]);
+      v.ns = is_ave ? (double)nv[zz] : 0.0;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) v.h[k] = (double)hv[zz][k];
       flush_global(acc + rz, v, is_ave, has_img);
       continue;
     }
-    atomicAdd(&tcnt[slot], v.cnt);
-    atomicAdd(&tsx[slot], v.sx);
-    atomicAdd(&tsy[slot], v.sy);
-    atomicAdd(&tsz[slot], v.sz);
+    atomicAdd(&tcnt[slot], 1u);
+    atomicAdd(&tsx[slot], (uint32_t)x);
+    atomicAdd(&tsy[slot], (uint32_t)y);
+    atomicAdd(&tsz[slot], (uint32_t)(zb + zz));
 #pragma unroll
-    for (int k = 0; k < 6; ++k) unsafeAtomicAdd(&th[k][slot], v.h[k]);
+    for (int k = 0; k < 6; ++k) unsafeAtomicAdd(&th[k][slot], (double)hv[zz][k]);
     if (is_ave)
-      unsafeAtomicAdd(&tns[slot], v.ns);
+      unsafeAtomicAdd(&tns[slot], (double)nv[zz]);
     else
-      atomicMax(&tnk[slot], v.nkey);
+      atomicMax(&tnk[slot], fkey(nv[zz]));
     if (has_img) {
-      atomicAdd(&tis[slot], v.isum);
-      atomicMax(&tim[slot], v.imax);
+      atomicAdd(&tis[slot], iv[zz]);
+      atomicMax(&tim[slot], iv[zz]);
     }
   }
   __syncthreads();
@@ -374,31 +344,28 @@ hipError_t launch_region_assign(const uint32_t* labels, const uint32_t* block_of
   return hipGetLastError();
 }
 
-template <typename TS>
-hipError_t launch_region_accum(const uint32_t* labels, const uint32_t* rid, const TS* hess,
-                               const TS* norm, const uint16_t* img, RegionAcc* acc, i64 R,
+template <typename TH, typename TN>
+hipError_t launch_region_accum(const uint32_t* labels, const uint32_t* rid, const TH* hess,
+                               const TN* norm, const uint16_t* img, RegionAcc* acc, i64 R,
                                int W, int H, int D, int is_ave, hipStream_t s) {
   (void)R;
   const int nbx = (W + 63) / 64, nby = (H + BY - 1) / BY, nbz = (D + BZ - 1) / BZ;
   const i64 nblk = (i64)nbx * nby * nbz;
   const i64 N = (i64)W * H * D;
-  hipLaunchKernelGGL((k_accum<TS>), dim3((unsigned)nblk), dim3(NT), 0, s, labels, rid, hess, norm,
-                     img, acc, W, H, D, N, nbx, nby, is_ave);
+  hipLaunchKernelGGL((k_accum<TH, TN>), dim3((unsigned)nblk), dim3(NT), 0, s, labels, rid, hess,
+                     norm, img, acc, W, H, D, N, nbx, nby, is_ave);
   mark("regions/accum", s);
   return hipGetLastError();
 }
 
-template hipError_t launch_region_accum<float>(const uint32_t*, const uint32_t*, const float*,
-                                               const float*, const uint16_t*, RegionAcc*, i64,
-                                               int, int, int, int, hipStream_t);
-template hipError_t launch_region_accum<__half>(const uint32_t*, const uint32_t*, const __half*,
-                                                const __half*, const uint16_t*, RegionAcc*, i64,
-                                                int, int, int, int, hipStream_t);
-template hipError_t launch_region_accum<__hip_bfloat16>(const uint32_t*, const uint32_t*,
-                                                        const __hip_bfloat16*,
-                                                        const __hip_bfloat16*, const uint16_t*,
-                                                        RegionAcc*, i64, int, int, int, int,
-                                                        hipStream_t);
+#define HDOG_INST_ACC(TH, TN)                                                                  \
+  template hipError_t launch_region_accum<TH, TN>(const uint32_t*, const uint32_t*, const TH*, \
+                                                  const TN*, const uint16_t*, RegionAcc*, i64,  \
+                                                  int, int, int, int, hipStream_t);
+HDOG_INST_ACC(float, float)
+HDOG_INST_ACC(__half, __half)
+HDOG_INST_ACC(__hip_bfloat16, __hip_bfloat16)
+#undef HDOG_INST_ACC
 
 hipError_t launch_region_finalize(const RegionAcc* acc, i64 R, int is_ave, RegionOut o, i64 pitch,
                                   hipStream_t s) {

@@ -24,9 +24,14 @@ from . import synth  # noqa: F401
 
 __all__ = ["HDoG", "HDoGError", "PRECISIONS"]
 
-PRECISIONS = {"fp32": C.HDOG_FP32, "fp16": C.HDOG_FP16, "bf16": C.HDOG_BF16}
-_TORCH_DTYPE = {C.HDOG_FP32: torch.float32, C.HDOG_FP16: torch.float16,
-                C.HDOG_BF16: torch.bfloat16}
+PRECISIONS = {"fp32": C.HDOG_FP32, "fp16": C.HDOG_FP16, "bf16": C.HDOG_BF16,
+              "fp16_dog32": C.HDOG_FP16_DOG32, "bf16_dog32": C.HDOG_BF16_DOG32}
+# storage dtypes (norm and hessian, dog) of each precision form
+_TORCH_DTYPE = {C.HDOG_FP32: (torch.float32, torch.float32),
+                C.HDOG_FP16: (torch.float16, torch.float16),
+                C.HDOG_BF16: (torch.bfloat16, torch.bfloat16),
+                C.HDOG_FP16_DOG32: (torch.float16, torch.float32),
+                C.HDOG_BF16_DOG32: (torch.bfloat16, torch.float32)}
 
 
 class HDoGError(RuntimeError):
@@ -185,11 +190,11 @@ class HDoG:
 
     def alloc_intermediates(self, shape, precision="fp32"):
         prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
-        dt = _TORCH_DTYPE[prec]
+        dt, dd = _TORCH_DTYPE[prec]
         D, H, W = shape
         kw = dict(device=self.device)
         return dict(norm=torch.empty((D, H, W), dtype=dt, **kw),
-                    dog=torch.empty((D, H, W), dtype=dt, **kw),
+                    dog=torch.empty((D, H, W), dtype=dd, **kw),
                     hess=torch.empty((6, D, H, W), dtype=dt, **kw),
                     pd=torch.empty((D, H, W), dtype=torch.uint8, **kw),
                     labels=torch.empty((D, H, W), dtype=torch.uint32, **kw))

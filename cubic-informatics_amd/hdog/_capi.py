@@ -20,7 +20,7 @@ if not os.path.exists(LIB_PATH):
 lib = ctypes.CDLL(LIB_PATH)
 
 HDOG_OK, HDOG_ERR_ARG, HDOG_ERR_HIP, HDOG_ERR_STATE, HDOG_ERR_CAPACITY = range(5)
-HDOG_FP32, HDOG_FP16, HDOG_BF16 = range(3)
+HDOG_FP32, HDOG_FP16, HDOG_BF16, HDOG_FP16_DOG32, HDOG_BF16_DOG32 = range(5)
 
 _vp = ctypes.c_void_p
 _i = ctypes.c_int

@@ -34,8 +34,16 @@ enum hdog_status {
 };
 
 /* Storage type of the float intermediates (normalized, DoG, Hessian).  Every
- * separable pass accumulates in FP32 whatever the storage type. */
-enum hdog_precision { HDOG_FP32 = 0, HDOG_FP16 = 1, HDOG_BF16 = 2 };
+ * separable pass accumulates in FP32 whatever the storage type.  The *_DOG32 forms keep
+ * the DoG volume in FP32 (its second differences feed the candidate test) and store the
+ * normalized volume and the Hessian elements in 2 bytes. */
+enum hdog_precision {
+  HDOG_FP32 = 0,
+  HDOG_FP16 = 1,        /* norm, dog, hessian in FP16 */
+  HDOG_BF16 = 2,        /* norm, dog, hessian in BF16 */
+  HDOG_FP16_DOG32 = 3,  /* norm, hessian in FP16; dog in FP32 */
+  HDOG_BF16_DOG32 = 4   /* norm, hessian in BF16; dog in FP32 */
+};
 
 /* Per-region results, structure of arrays indexed by region id r = 0..count-1 in
  * ascending label order (region 0 = background, never filled).  The caller supplies
@@ -142,8 +150,7 @@ int hdog_process_substack(hdog_ctx* ctx, void* stream, const hdog_params* p,
 
 /* Same pipeline on a volume already resident on the device.  The intermediates are
  * written to caller buffers when the pointers are non-NULL (else context scratch is
- * used); storage type follows p->precision for norm / dog / hessian (2 bytes per
- * element for FP16 and BF16).  Region results stay on the device as for
+ * used); storage types follow p->precision (enum hdog_precision).  Region results stay on the device as for
  * hdog_regional_features. */
 int hdog_run_device(hdog_ctx* ctx, void* stream, const hdog_params* p, const uint16_t* d_img,
                     void* d_norm, void* d_dog, void* d_hessian, uint8_t* d_pd,
