@@ -21,7 +21,7 @@ struct Buf {
 struct hdog_ctx {
   int device = 0;
   std::string err;
-  Buf tmp1, tmp2, rid, bcnt, dR, acc, weights, blobs;
+  Buf tmp1, tmp2, rid, bcnt, dR, acc, weights, blobs, sw, sums, ddog;
   Buf o_label, o_size, o_vox, o_cen, o_eig, o_nrm, o_hess, o_isum, o_imax;
   i64 out_cap = 0;
   Buf img, norm, dog, hess, pd, labels;  // substack buffers of hdog_process_substack
@@ -204,6 +204,50 @@ int check_params(hdog_ctx* ctx, const hdog_params* p) {
   return HDOG_OK;
 }
 
+// Normalised Gaussian weights exp(-i^2 / (2 sigma^2)) / Z on |i| <= r (FP64) and their
+// sigma-derivative (w_i / sigma^3) (i^2 - sum_k w_k k^2), stored as float.
+void soft_weights(double sigma, int r, float* w, float* dw) {
+  std::vector<double> e(2 * r + 1);
+  double Z = 0.0;
+  for (int i = -r; i <= r; ++i) Z += (e[i + r] = std::exp(-(double)i * i / (2.0 * sigma * sigma)));
+  double m2 = 0.0;
+  for (int i = -r; i <= r; ++i) m2 += e[i + r] / Z * (double)i * i;
+  for (int i = -r; i <= r; ++i) {
+    const double wi = e[i + r] / Z;
+    w[i + r] = (float)wi;
+    if (dw) dw[i + r] = (float)(wi / (sigma * sigma * sigma) * ((double)i * i - m2));
+  }
+}
+
+struct SoftSetup {
+  int r[6];
+  const float* w[6];   // x, y, z of sigma1, then of sigma2 (device)
+  const float* dw[6];  // their sigma-derivatives (device)
+  double s;            // normalizer (sx1 sy1 sz1)^gamma
+};
+
+int soft_setup(hdog_ctx* ctx, const double sig1[3], const double sig2[3], const int rad1[3],
+               const int rad2[3], double gamma, SoftSetup* S) {
+  std::vector<float> hw(12 * kMaxTaps, 0.0f);
+  for (int k = 0; k < 6; ++k) {
+    const double sg = k < 3 ? sig1[k] : sig2[k - 3];
+    const int r = k < 3 ? rad1[k] : rad2[k - 3];
+    if (!(sg > 0.0) || r < 0 || r > kMaxGaussRadius)
+      return fail(ctx, HDOG_ERR_ARG, "sigma %g / radius %d invalid (radius at most %d)", sg, r,
+                  kMaxGaussRadius);
+    S->r[k] = r;
+    soft_weights(sg, r, hw.data() + k * kMaxTaps, hw.data() + (6 + k) * kMaxTaps);
+  }
+  HIPCHK(ensure(ctx->sw, hw.size() * 4));
+  HIPCHK(hipMemcpy(ctx->sw.p, hw.data(), hw.size() * 4, hipMemcpyHostToDevice));
+  for (int k = 0; k < 6; ++k) {
+    S->w[k] = (const float*)ctx->sw.p + k * kMaxTaps;
+    S->dw[k] = (const float*)ctx->sw.p + (6 + k) * kMaxTaps;
+  }
+  S->s = std::pow(sig1[0] * sig1[1] * sig1[2], gamma);
+  return HDOG_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -229,6 +273,7 @@ int hdog_release_scratch(hdog_ctx* ctx) {
   if (!ctx) return HDOG_ERR_ARG;
   (void)hipSetDevice(ctx->device);
   for (Buf* b : {&ctx->tmp1, &ctx->tmp2, &ctx->rid, &ctx->bcnt, &ctx->dR, &ctx->acc, &ctx->blobs,
+                 &ctx->sw, &ctx->sums, &ctx->ddog,
                  &ctx->o_label, &ctx->o_size, &ctx->o_vox, &ctx->o_cen, &ctx->o_eig, &ctx->o_nrm,
                  &ctx->o_hess, &ctx->o_isum, &ctx->o_imax, &ctx->img, &ctx->norm, &ctx->dog,
                  &ctx->hess, &ctx->pd, &ctx->labels})
@@ -499,6 +544,95 @@ int hdog_synth_volume(hdog_ctx* ctx, void* stream, uint16_t* d_img, float* d_scr
   HIPCHK(launch_synth(d_img, d_scratch, width, height, depth, seed, mean0, mean1, sd,
                       (const double*)ctx->blobs.p, n_blobs, s));
   HIPCHK(hipStreamSynchronize(s));
+  return HDOG_OK;
+}
+
+int hdog_soft_forward(hdog_ctx* ctx, void* stream, const float* d_v, const uint8_t* d_target,
+                      int width, int height, int depth, const double sigma1[3],
+                      const double sigma2[3], const int radius1[3], const int radius2[3],
+                      double gamma, double tau, float* d_dog, uint8_t* d_mask, double* loss,
+                      int64_t* count) {
+  if (!ctx) return HDOG_ERR_ARG;
+  int rc = check_shape(ctx, width, height, depth);
+  if (rc) return rc;
+  if (!d_v || !d_target || !d_dog || !d_mask || !sigma1 || !sigma2 || !radius1 || !radius2 ||
+      !(tau > 0.0))
+    return fail(ctx, HDOG_ERR_ARG, "NULL pointer or non-positive tau");
+  HIPCHK(hipSetDevice(ctx->device));
+  SoftSetup S;
+  if ((rc = soft_setup(ctx, sigma1, sigma2, radius1, radius2, gamma, &S))) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const i64 N = (i64)width * height * depth;
+  HIPCHK(ensure(ctx->tmp1, (size_t)N * 8));
+  HIPCHK(ensure(ctx->tmp2, (size_t)N * 8));
+  HIPCHK(ensure(ctx->hess, (size_t)N * 24));
+  HIPCHK(ensure(ctx->sums, 64 * 8));
+  HIPCHK(launch_gauss_pair(d_v, ctx->tmp1.p, ctx->tmp2.p, S.w, S.r, (float)S.s, d_dog, nullptr,
+                           width, height, depth, s));
+  HIPCHK(launch_hessian<float, float>(d_dog, (float*)ctx->hess.p, d_mask, width, height, depth, s));
+  HIPCHK(launch_soft_loss((const float*)ctx->hess.p, d_mask, d_target, N, 1.0 / tau,
+                          (double*)ctx->sums.p, s));
+  double h[2];
+  HIPCHK(hipMemcpyAsync(h, ctx->sums.p, 16, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (count) *count = (int64_t)h[1];
+  if (loss) *loss = h[1] > 0 ? h[0] / h[1] : 0.0;
+  return HDOG_OK;
+}
+
+int hdog_soft_backward(hdog_ctx* ctx, void* stream, const float* d_v, const float* d_dog,
+                       const uint8_t* d_mask, const uint8_t* d_target, int width, int height,
+                       int depth, const double sigma1[3], const double sigma2[3],
+                       const int radius1[3], const int radius2[3], double gamma, double tau,
+                       int64_t count, double grad_out, float* d_grad_v, double grad_sigma1[3],
+                       double grad_sigma2[3], double* grad_tau) {
+  if (!ctx) return HDOG_ERR_ARG;
+  int rc = check_shape(ctx, width, height, depth);
+  if (rc) return rc;
+  if (!d_v || !d_dog || !d_mask || !d_target || !d_grad_v || !(tau > 0.0))
+    return fail(ctx, HDOG_ERR_ARG, "NULL pointer or non-positive tau");
+  HIPCHK(hipSetDevice(ctx->device));
+  SoftSetup S;
+  if ((rc = soft_setup(ctx, sigma1, sigma2, radius1, radius2, gamma, &S))) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  const i64 N = (i64)width * height * depth;
+  HIPCHK(ensure(ctx->tmp1, (size_t)N * 8));
+  HIPCHK(ensure(ctx->tmp2, (size_t)N * 8));
+  HIPCHK(ensure(ctx->hess, (size_t)N * 24));
+  HIPCHK(ensure(ctx->ddog, (size_t)N * 4));
+  HIPCHK(ensure(ctx->pd, (size_t)N));
+  HIPCHK(ensure(ctx->sums, 64 * 8));
+  double* sums = (double*)ctx->sums.p;
+  float* gh = (float*)ctx->hess.p;
+  float* ddog = (float*)ctx->ddog.p;
+  // H is recomputed from the checkpointed DoG; the mask from the forward pass is used
+  HIPCHK(launch_hessian<float, float>(d_dog, gh, (uint8_t*)ctx->pd.p, width, height, depth, s));
+  const double inv_count = count > 0 ? grad_out / (double)count : 0.0;
+  HIPCHK(launch_soft_grad_h(gh, d_mask, d_target, N, 1.0 / tau, inv_count, sums, s));
+  // P_x, P_y, P_z (3N floats) in tmp1, grown to 12 bytes per voxel
+  HIPCHK(ensure(ctx->tmp1, (size_t)N * 12));
+  HIPCHK(launch_hessian_adjoint(gh, (float*)ctx->tmp1.p, ddog, width, height, depth, s));
+  // dL/dv = s (G1 - G2) dL/ddog
+  HIPCHK(launch_gauss_pair(ddog, ctx->tmp1.p, ctx->tmp2.p, S.w, S.r, (float)S.s, d_grad_v, nullptr,
+                           width, height, depth, s));
+  double h[16];
+  HIPCHK(launch_dot(ddog, d_dog, N, sums + 1, s));  // <dL/ddog, dog>
+  for (int a = 0; a < 3; ++a) {
+    const float* w[6] = {S.w[0], S.w[1], S.w[2], S.w[3], S.w[4], S.w[5]};
+    w[a] = S.dw[a];
+    w[3 + a] = S.dw[3 + a];
+    HIPCHK(launch_gauss_pair(d_v, ctx->tmp1.p, ctx->tmp2.p, w, S.r, 0.0f, nullptr,
+                             (float2*)ctx->tmp1.p, width, height, depth, s));
+    HIPCHK(launch_dot_pair(ddog, (const float2*)ctx->tmp1.p, N, sums + 2 + 2 * a, s));
+  }
+  HIPCHK(hipMemcpyAsync(h, sums, 8 * 8, hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  if (grad_tau) *grad_tau = h[0];
+  for (int a = 0; a < 3; ++a) {
+    const double sg1 = sigma1[a];
+    if (grad_sigma1) grad_sigma1[a] = gamma / sg1 * h[1] + S.s * h[2 + 2 * a];
+    if (grad_sigma2) grad_sigma2[a] = -S.s * h[3 + 2 * a];
+  }
   return HDOG_OK;
 }
 

@@ -270,6 +270,28 @@ hipError_t launch_dog(const TN* norm, TD* dog, void* tmp1, void* tmp2, const flo
   return e;
 }
 
+// Separable pair of Gaussians with independent weights per axis (w[0..2] = x, y, z of the
+// first, w[3..5] of the second; radii r[6]).  With dog != nullptr the z pass writes
+// (G1 - G2) * nz; otherwise it writes the (G1, G2) pair to pair_out.  Used by the
+// differentiable form (anisotropic sigmas and their weight derivatives).
+hipError_t launch_gauss_pair(const float* src, void* tmp1, void* tmp2, const float* const w[6],
+                             const int r[6], float nz, float* dog, float2* pair_out, int W, int H,
+                             int D, hipStream_t s) {
+  const i64 WH = (i64)W * H;
+  hipError_t e;
+  if ((e = gauss_x<float>(src, (float2*)tmp1, w[0], w[3], r[0], r[3], W, (i64)H * D, s)) !=
+      hipSuccess)
+    return e;
+  if ((e = gauss_line<false, float>((const float2*)tmp1, (float2*)tmp2, nullptr, w[1], w[4], r[1],
+                                    r[4], 0.0f, W, H, W, WH, D, s)) != hipSuccess)
+    return e;
+  if (dog)
+    return gauss_line<true, float>((const float2*)tmp2, nullptr, dog, w[2], w[5], r[2], r[5], nz, W,
+                                   D, WH, W, H, s);
+  return gauss_line<false, float>((const float2*)tmp2, pair_out, nullptr, w[2], w[5], r[2], r[5],
+                                  0.0f, W, D, WH, W, H, s);
+}
+
 #define HDOG_INST_DOG(TN, TD)                                                                \
   template hipError_t launch_dog<TN, TD>(const TN*, TD*, void*, void*, const float*, const int*, \
                                          float, int, int, int, hipStream_t);

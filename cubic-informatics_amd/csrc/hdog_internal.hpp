@@ -94,6 +94,9 @@ hipError_t launch_normalize(const uint16_t* img, TS* norm, void* tmp1, void* tmp
 template <typename TN, typename TD>
 hipError_t launch_dog(const TN* norm, TD* dog, void* tmp1, void* tmp2, const float* weights,
                       const int r[4], float normalizer, int W, int H, int D, hipStream_t s);
+hipError_t launch_gauss_pair(const float* src, void* tmp1, void* tmp2, const float* const w[6],
+                             const int r[6], float nz, float* dog, float2* pair_out, int W, int H,
+                             int D, hipStream_t s);
 template <typename TD, typename TH>
 hipError_t launch_hessian(const TD* dog, TH* hess, uint8_t* pd, int W, int H, int D,
                           hipStream_t s);
@@ -114,6 +117,15 @@ struct RegionOut {
 };
 hipError_t launch_region_finalize(const RegionAcc* acc, i64 R, int is_ave, RegionOut o,
                                   i64 pitch, hipStream_t s);
+// differentiable blob response (soft.hip)
+hipError_t launch_soft_loss(const float* hess, const uint8_t* mask, const uint8_t* target, i64 N,
+                            double inv_tau, double* sums, hipStream_t s);
+hipError_t launch_soft_grad_h(float* hess, const uint8_t* mask, const uint8_t* target, i64 N,
+                              double inv_tau, double inv_count, double* sums, hipStream_t s);
+hipError_t launch_hessian_adjoint(const float* gh, float* P, float* out, int W, int H, int D,
+                                  hipStream_t s);
+hipError_t launch_dot_pair(const float* a, const float2* pair, i64 N, double* out, hipStream_t s);
+hipError_t launch_dot(const float* a, const float* b, i64 N, double* out, hipStream_t s);
 hipError_t launch_synth(uint16_t* img, float* scratch, int W, int H, int D, u64 seed,
                         double mean0, double mean1, double sd, const double* d_blobs,
                         i64 n_blobs, hipStream_t s);

@@ -76,8 +76,17 @@ hdog_synth_volume = _sig("hdog_synth_volume", [_vp, _vp, _vp, _vp, _i, _i, _i, c
 hdog_set_profiling = _sig("hdog_set_profiling", [_vp, _i])
 hdog_stage_times = _sig("hdog_stage_times", [_vp, ctypes.POINTER(_d), ctypes.POINTER(ctypes.c_char_p), _i])
 
+_d3 = ctypes.c_double * 3
+_i3 = ctypes.c_int * 3
+hdog_soft_forward = _sig("hdog_soft_forward", [_vp, _vp, _vp, _vp, _i, _i, _i, _d3, _d3, _i3, _i3,
+                                               _d, _d, _vp, _vp, ctypes.POINTER(_d),
+                                               ctypes.POINTER(_i64)])
+hdog_soft_backward = _sig("hdog_soft_backward", [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i, _d3, _d3,
+                                                 _i3, _i3, _d, _d, _i64, _d, _vp, _d3, _d3,
+                                                 ctypes.POINTER(_d)])
+
 EXPORTED = ["hdog_create", "hdog_destroy", "hdog_last_error", "hdog_release_scratch",
             "hdog_normalize3d", "hdog_init_gaussian3d", "hdog_dog3d", "hdog_hessian_pd",
             "hdog_ccl3d", "hdog_regional_features", "hdog_regions_download",
             "hdog_process_substack", "hdog_run_device", "hdog_synth_volume",
-            "hdog_set_profiling", "hdog_stage_times"]
+            "hdog_set_profiling", "hdog_stage_times", "hdog_soft_forward", "hdog_soft_backward"]

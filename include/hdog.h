@@ -165,6 +165,29 @@ int hdog_run_device(hdog_ctx* ctx, void* stream, const hdog_params* p, const uin
 int hdog_set_profiling(hdog_ctx* ctx, int on);
 int hdog_stage_times(hdog_ctx* ctx, double* ms, const char** names, int max_entries);
 
+/* ---- differentiable blob response ---------------------------------------------
+ * No counterpart in the reference, which only thresholds the Hessian.  For a float
+ * volume v (x-fastest), anisotropic sigmas sigma{1,2}[3] = (x, y, z) with fixed radii
+ * radius{1,2}[3], normalised weights exp(-i^2 / 2 sigma^2) / Z:
+ *   dog = (sx1 sy1 sz1)^gamma (G_sigma1 v - G_sigma2 v)      (zero padding)
+ *   H   = Hessian of dog by the detector's central differences, M = its candidate mask
+ *   loss = mean over M of BCE(sigmoid(-det(H) / tau), target)   (target: uint8 0/1)
+ * hdog_soft_forward writes the DoG (checkpoint for the backward pass) and M, and
+ * returns the loss and |M|; it synchronizes `stream`.  hdog_soft_backward recomputes H
+ * from the DoG, holds M fixed and returns grad_out * d loss / d (v, sigma1, sigma2, tau)
+ * through hand-written adjoints; it synchronizes `stream`. */
+int hdog_soft_forward(hdog_ctx* ctx, void* stream, const float* d_v, const uint8_t* d_target,
+                      int width, int height, int depth, const double sigma1[3],
+                      const double sigma2[3], const int radius1[3], const int radius2[3],
+                      double gamma, double tau, float* d_dog, uint8_t* d_mask, double* loss,
+                      int64_t* count);
+int hdog_soft_backward(hdog_ctx* ctx, void* stream, const float* d_v, const float* d_dog,
+                       const uint8_t* d_mask, const uint8_t* d_target, int width, int height,
+                       int depth, const double sigma1[3], const double sigma2[3],
+                       const int radius1[3], const int radius2[3], double gamma, double tau,
+                       int64_t count, double grad_out, float* d_grad_v, double grad_sigma1[3],
+                       double grad_sigma2[3], double* grad_tau);
+
 /* ---- synthetic input (benchmark / tests) ---------------------------------- */
 /* Renders noise + Gaussian blobs into d_img (depth x height x width uint16):
  * background mean0 + (mean1 - mean0) * z / (depth-1), sd `sd` (counter-based normal
