@@ -20,6 +20,11 @@ kernel (event-timed in a profiled pass after the timed one) against the 8 TB/s H
 peak; `pipeline_roofline` is the whole step against the 80 B/voxel (46 B/voxel for
 FP16/BF16 storage, 50 with an FP32 DoG) algorithmic traffic model.  `cpu_baseline` times the CPU oracle
 (OpenMP, all host threads) on one 32-slice substack of the same stack.
+
+`batched_form` is the many-stack form: 40 stacks of z ~ U{64..640} and y, x each from
+{512, 1024, 1536, 2160} (seeded), the same noise model and parameters at one blob per
+2e4 voxels, all resident on the device and run as one captured hipGraph launch per step,
+beside the same stacks run one hdog_run_device call at a time.
 """
 import argparse
 import json
@@ -131,6 +136,86 @@ def cpu_baseline(img, depth):
                        f"of the same stack, whole path, {t:.1f} s, {len(o['regions']['label'])} regions")
 
 
+def c3_tiles(n_tiles, seed=3):
+    """Stack shapes (z, y, x) of the batched workload."""
+    import numpy as np
+    rng = np.random.Generator(np.random.PCG64(seed))
+    z = rng.integers(64, 641, n_tiles)
+    y = rng.choice([512, 1024, 1536, 2160], n_tiles)
+    x = rng.choice([512, 1024, 1536, 2160], n_tiles)
+    return [(int(a), int(b), int(c)) for a, b, c in zip(z, y, x)]
+
+
+def run_batched(hd, forms, n_tiles, steps):
+    import torch
+    from hdog import synth
+    cfg = synth.CONFIGS["c1"]
+    shapes = c3_tiles(n_tiles)
+    imgs = []
+    t0 = time.perf_counter()
+    for t, sh in enumerate(shapes):
+        nb = int(sh[0] * sh[1] * sh[2] / 2e4)
+        blobs = synth.blob_table(sh, nb, 1000 + t, cfg["sigma"], cfg["zfactor"], cfg["amp"])
+        imgs.append(hd.synth_volume(sh, blobs, 1000 + t, cfg["mean0"], cfg["mean1"], cfg["sd"]))
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    Ns = [a * b * c for a, b, c in shapes]
+    total = sum(Ns)
+    free, tot = torch.cuda.mem_get_info()
+    log(f"device memory: {free / 2**30:.1f} GiB free of {tot / 2**30:.1f} GiB")
+    log(f"batched stacks: {n_tiles} stacks, {total / 1e9:.2f} Gvoxel (largest "
+        f"{max(Ns) / 1e9:.2f}) rendered in {time.perf_counter() - t0:.1f} s")
+    out = dict(stacks=n_tiles, total_gvoxel=total / 1e9, largest_gvoxel=max(Ns) / 1e9,
+               shapes_zyx=shapes, forms={})
+    stream = torch.cuda.current_stream()
+    for f in forms:
+        params = hd.make_params(shapes[0], C1["rxy"], C1["rz"], C1["min_intensity"],
+                                C1["sigma_xy1"], C1["sigma_z1"], C1["sigma_xy2"], C1["sigma_z2"],
+                                C1["truncate"], C1["gamma"], False, f)
+        t0 = time.perf_counter()
+        hd.batch_plan(imgs, params)
+        plan_s = time.perf_counter() - t0
+        hd.batch_run()
+        log(f"[batch {f}] planned + captured in {plan_s:.1f} s, warmup done")
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        ev0.record(stream)
+        for _ in range(steps):
+            hd.batch_run()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ms = ev0.elapsed_time(ev1) / steps
+        regions = sum(hd.batch_tile_info(t)[0] for t in range(n_tiles))
+        hd.batch_release()
+        log(f"[batch {f}] {steps} graph launches: {ms:.1f} ms each, {regions} regions")
+        # the same stacks through the per-stack entry point, one call (and host wait) each
+        ev0.record(stream)
+        per_regions = 0
+        for im in imgs:
+            per_regions += hd.run_device(im, _with_shape(params, im.shape)) - 1
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        ms_per = ev0.elapsed_time(ev1)
+        hd.release_scratch()
+        torch.cuda.empty_cache()
+        log(f"[batch {f}] per-stack calls: {ms_per:.1f} ms, {per_regions} regions")
+        out["forms"][f] = dict(value=total / (ms * 1e-3) / 1e9, ms_per_batch=ms,
+                               wall_ms_per_batch=wall * 1e3 / steps, plan_s=plan_s,
+                               num_regions=regions, per_stack_calls_ms=ms_per,
+                               per_stack_value=total / (ms_per * 1e-3) / 1e9,
+                               per_stack_regions_equal=per_regions == regions)
+    del imgs
+    torch.cuda.empty_cache()
+    return out
+
+
+def _with_shape(params, shape):
+    p = type(params).from_buffer_copy(params)
+    p.depth, p.height, p.width = shape
+    return p
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,6 +225,10 @@ def main():
                     help="storage forms to time; the first one gives `value`")
     ap.add_argument("--depth", type=int, default=512, help="stack depth (z) of the workload")
     ap.add_argument("--cpu-depth", type=int, default=32, help="substack depth of the CPU sample; 0 skips")
+    ap.add_argument("--batch-forms", default="fp32,fp16",
+                    help="storage forms of the batched many-stack run; empty skips it")
+    ap.add_argument("--batch-stacks", type=int, default=40)
+    ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device-to-device copy of MIB MiB (three launches), "
                          "the reference kernel for scaling HBM counters")
@@ -232,6 +321,17 @@ def main():
         out["cpu_baseline"] = cpu_baseline(img, args.cpu_depth)
     else:
         out["cpu_baseline"] = None
+    bforms = [f for f in args.batch_forms.split(",") if f]
+    if bforms:
+        img = res = r = None
+        a = b = rf = None  # noqa: F841 (drop the masks of the precision comparison)
+        hd.release_scratch()
+        torch.cuda.empty_cache()
+        try:
+            out["batched_form"] = run_batched(hd, bforms, args.batch_stacks, args.batch_steps)
+        except (RuntimeError, MemoryError) as e:  # HDoGError / torch OOM: keep the main line
+            log(f"batched form failed: {e}")
+            out["batched_form"] = {"error": str(e)}
     print(json.dumps(out), flush=True)
 
 

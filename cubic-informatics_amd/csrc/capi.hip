@@ -16,6 +16,20 @@ struct Buf {
   void* p = nullptr;
   size_t bytes = 0;
 };
+
+// A captured batch of resident stacks: it owns every buffer its graph points at.
+struct BatchState {
+  std::vector<hdog_tile> tiles;
+  hdog_params p{};
+  Buf tmp1, tmp2, norm, dog, hess, pd, labels, rid, bcnt, dR, acc, weights, tout, running;
+  Buf a_label, a_size, a_vox, a_cen, a_eig, a_nrm, a_hess, a_isum, a_imax;
+  i64 acc_cap = 0, arena_cap = 0;
+  hipStream_t cs = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  TileOut* h_tout = nullptr;  // pinned copy of the per-stack placements
+  bool ran = false;
+};
 }  // namespace
 
 struct hdog_ctx {
@@ -34,6 +48,7 @@ struct hdog_ctx {
   bool prof = false;        // record an event after every kernel of run_device
   Prof pool;
   bool ev_valid = false;
+  BatchState* batch = nullptr;
 };
 
 thread_local hdog::Prof* hdog::tl_prof = nullptr;
@@ -139,8 +154,8 @@ int regional_features_t(hdog_ctx* ctx, hipStream_t s, const uint32_t* labels, co
   int rc = ensure_regions(ctx, R);
   if (rc) return rc;
   RegionAcc* acc = (RegionAcc*)ctx->acc.p;
-  HIPCHK(launch_region_assign(labels, (const uint32_t*)ctx->bcnt.p, (uint32_t*)ctx->rid.p, acc, R,
-                              N, s));
+  HIPCHK(launch_region_assign(labels, (const uint32_t*)ctx->bcnt.p, (uint32_t*)ctx->rid.p, acc, R, R,
+                              nullptr, N, s));
   HIPCHK(launch_region_accum<TH, TN>(labels, (const uint32_t*)ctx->rid.p, hess, norm, img, acc, R,
                                      W, H, D, is_ave, s));
   const i64 cap = ctx->out_cap;
@@ -248,6 +263,99 @@ int soft_setup(hdog_ctx* ctx, const double sig1[3], const double sig2[3], const 
   return HDOG_OK;
 }
 
+
+// ---- batched stacks -------------------------------------------------------------
+
+void batch_free(BatchState* B) {
+  if (!B) return;
+  if (B->exec) (void)hipGraphExecDestroy(B->exec);
+  if (B->graph) (void)hipGraphDestroy(B->graph);
+  if (B->cs) (void)hipStreamDestroy(B->cs);
+  if (B->h_tout) (void)hipHostFree(B->h_tout);
+  for (Buf* b : {&B->tmp1, &B->tmp2, &B->norm, &B->dog, &B->hess, &B->pd, &B->labels, &B->rid,
+                 &B->bcnt, &B->dR, &B->acc, &B->weights, &B->tout, &B->running, &B->a_label,
+                 &B->a_size, &B->a_vox, &B->a_cen, &B->a_eig, &B->a_nrm, &B->a_hess, &B->a_isum,
+                 &B->a_imax})
+    release(*b);
+  delete B;
+}
+
+// Enqueues the whole pipeline of tile t on stream s with device-side region counts: no
+// host synchronization, allocation or memset, so it can be captured.
+template <typename TN, typename TD, typename TH>
+hipError_t batch_tile(hdog_ctx* ctx, BatchState* B, int t, hipStream_t s) {
+  const hdog_tile& T = B->tiles[t];
+  const int W = T.width, H = T.height, D = T.depth;
+  const i64 N = (i64)W * H * D;
+  const hdog_params* p = &B->p;
+  TN* norm = (TN*)B->norm.p;
+  TD* dog = (TD*)B->dog.p;
+  TH* hess = (TH*)B->hess.p;
+  uint8_t* pd = (uint8_t*)B->pd.p;
+  uint32_t* labels = (uint32_t*)B->labels.p;
+  hipError_t e;
+  if ((e = launch_normalize<TN>(T.image, norm, B->tmp1.p, B->tmp2.p, p->min_intensity, W, H, D,
+                                p->radius_xy, p->radius_z, s)) != hipSuccess)
+    return e;
+  const float nz = powf(ctx->sigma[0], p->gamma * 2) * powf(ctx->sigma[1], p->gamma);
+  if ((e = launch_dog<TN, TD>(norm, dog, B->tmp1.p, B->tmp2.p, (const float*)B->weights.p, ctx->r,
+                              nz, W, H, D, s)) != hipSuccess)
+    return e;
+  if ((e = launch_hessian<TD, TH>(dog, hess, pd, W, H, D, s)) != hipSuccess) return e;
+  if ((e = launch_ccl(pd, labels, W, H, D, s)) != hipSuccess) return e;
+  i64 nbo = 0;
+  uint32_t* dR = (uint32_t*)B->dR.p;
+  if ((e = launch_region_ids(labels, (uint32_t*)B->rid.p, (uint32_t*)B->bcnt.p, dR, N, s, &nbo)) !=
+      hipSuccess)
+    return e;
+  TileOut* tout = (TileOut*)B->tout.p + t;
+  if ((e = launch_tile_reserve(dR, B->acc_cap, B->arena_cap, (u64*)B->running.p, tout, s)) !=
+      hipSuccess)
+    return e;
+  RegionAcc* acc = (RegionAcc*)B->acc.p;
+  if ((e = launch_region_assign(labels, (const uint32_t*)B->bcnt.p, (uint32_t*)B->rid.p, acc,
+                                B->acc_cap, B->acc_cap, dR, N, s)) != hipSuccess)
+    return e;
+  if ((e = launch_region_accum<TH, TN>(labels, (const uint32_t*)B->rid.p, hess, norm, T.image, acc,
+                                       B->acc_cap, W, H, D, p->is_ave_mode, s)) != hipSuccess)
+    return e;
+  const i64 cap = B->arena_cap;
+  RegionOut o;
+  o.label = (uint32_t*)B->a_label.p;
+  o.size = (uint16_t*)B->a_size.p;
+  o.voxels = (u64*)B->a_vox.p;
+  o.cx = (float*)B->a_cen.p;
+  o.cy = o.cx + cap;
+  o.cz = o.cx + 2 * cap;
+  o.st = (float*)B->a_eig.p;
+  o.bl = o.st + cap;
+  o.nrm = (float*)B->a_nrm.p;
+  o.hess = (float*)B->a_hess.p;
+  o.isum = (u64*)B->a_isum.p;
+  o.imax = (uint16_t*)B->a_imax.p;
+  return launch_region_finalize(acc, B->acc_cap, p->is_ave_mode, o, cap, s, tout);
+}
+
+template <typename TN, typename TD, typename TH>
+hipError_t batch_capture(hdog_ctx* ctx, BatchState* B, hipStream_t s) {
+  hipError_t e = launch_fill_u64((u64*)B->running.p, 1, 0, s);
+  for (int t = 0; e == hipSuccess && t < (int)B->tiles.size(); ++t)
+    e = batch_tile<TN, TD, TH>(ctx, B, t, s);
+  return e;
+}
+
+hipError_t batch_capture_any(hdog_ctx* ctx, BatchState* B, hipStream_t s) {
+  typedef __half h16;
+  typedef __hip_bfloat16 b16;
+  switch (B->p.precision) {
+    case HDOG_FP16: return batch_capture<h16, h16, h16>(ctx, B, s);
+    case HDOG_BF16: return batch_capture<b16, b16, b16>(ctx, B, s);
+    case HDOG_FP16_DOG32: return batch_capture<h16, float, h16>(ctx, B, s);
+    case HDOG_BF16_DOG32: return batch_capture<b16, float, b16>(ctx, B, s);
+    default: return batch_capture<float, float, float>(ctx, B, s);
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -285,6 +393,7 @@ int hdog_release_scratch(hdog_ctx* ctx) {
 
 void hdog_destroy(hdog_ctx* ctx) {
   if (!ctx) return;
+  hdog_batch_release(ctx);
   hdog_release_scratch(ctx);
   if (ctx->prof)
     for (hipEvent_t& e : ctx->pool.ev) (void)hipEventDestroy(e);
@@ -633,6 +742,165 @@ int hdog_soft_backward(hdog_ctx* ctx, void* stream, const float* d_v, const floa
     if (grad_sigma1) grad_sigma1[a] = gamma / sg1 * h[1] + S.s * h[2 + 2 * a];
     if (grad_sigma2) grad_sigma2[a] = -S.s * h[3 + 2 * a];
   }
+  return HDOG_OK;
+}
+
+int hdog_batch_release(hdog_ctx* ctx) {
+  if (!ctx) return HDOG_ERR_ARG;
+  if (ctx->batch) {
+    (void)hipSetDevice(ctx->device);
+    batch_free(ctx->batch);
+    ctx->batch = nullptr;
+  }
+  return HDOG_OK;
+}
+
+int hdog_batch_plan(hdog_ctx* ctx, const hdog_params* p, const hdog_tile* tiles, int n_tiles,
+                    int64_t max_regions_per_stack, int64_t arena_capacity) {
+  if (!ctx) return HDOG_ERR_ARG;
+  if (!p || !tiles || n_tiles < 1) return fail(ctx, HDOG_ERR_ARG, "NULL params / tiles or no tiles");
+  if (max_regions_per_stack < 0 || arena_capacity < 0)
+    return fail(ctx, HDOG_ERR_ARG, "negative capacity");
+  HIPCHK(hipSetDevice(ctx->device));
+  hdog_batch_release(ctx);
+  i64 maxN = 0, totN = 0;
+  for (int t = 0; t < n_tiles; ++t) {
+    hdog_params q = *p;
+    q.width = tiles[t].width;
+    q.height = tiles[t].height;
+    q.depth = tiles[t].depth;
+    int rc = check_params(ctx, &q);
+    if (rc) return fail(ctx, rc, "stack %d: %s", t, std::string(ctx->err).c_str());
+    if (!tiles[t].image) return fail(ctx, HDOG_ERR_ARG, "stack %d: NULL image", t);
+    const i64 N = (i64)q.width * q.height * q.depth;
+    maxN = std::max(maxN, N);
+    totN += N;
+  }
+  int rc = ensure_gauss(ctx, p);
+  if (rc) return rc;
+  BatchState* B = new BatchState;
+  B->tiles.assign(tiles, tiles + n_tiles);
+  B->p = *p;
+  B->acc_cap = max_regions_per_stack > 0 ? max_regions_per_stack + 1 : maxN / 64 + 2;
+  B->arena_cap = arena_capacity > 0 ? arena_capacity : totN / 128 + n_tiles;
+  if (B->acc_cap > (i64)0xFFFFFFFFll) B->acc_cap = 0xFFFFFFFFll;
+  const size_t es = p->precision == HDOG_FP32 ? 4 : 2;
+  const size_t ed = (p->precision == HDOG_FP16 || p->precision == HDOG_BF16) ? 2 : 4;
+  const i64 cap = B->arena_cap;
+  const struct {
+    Buf* b;
+    size_t bytes;
+  } need[] = {{&B->tmp1, (size_t)maxN * 8}, {&B->tmp2, (size_t)maxN * 8},
+              {&B->norm, maxN * es}, {&B->dog, maxN * ed}, {&B->hess, 6 * maxN * es},
+              {&B->pd, (size_t)maxN}, {&B->labels, (size_t)maxN * 4}, {&B->rid, (size_t)maxN * 4},
+              {&B->bcnt, (size_t)((maxN + 4095) / 4096) * 4}, {&B->dR, 64},
+              {&B->acc, (size_t)B->acc_cap * sizeof(RegionAcc)}, {&B->weights, ctx->weights.bytes},
+              {&B->tout, (size_t)n_tiles * sizeof(TileOut)}, {&B->running, 64},
+              {&B->a_label, (size_t)cap * 4}, {&B->a_size, (size_t)cap * 2},
+              {&B->a_vox, (size_t)cap * 8}, {&B->a_cen, (size_t)cap * 12},
+              {&B->a_eig, (size_t)cap * 8}, {&B->a_nrm, (size_t)cap * 4},
+              {&B->a_hess, (size_t)cap * 24}, {&B->a_isum, (size_t)cap * 8},
+              {&B->a_imax, (size_t)cap * 2}};
+  auto bad = [&](hipError_t e, const char* what) {
+    batch_free(B);
+    return fail(ctx, HDOG_ERR_HIP, "batch plan: %s failed: %s", what, hipGetErrorString(e));
+  };
+  hipError_t e;
+  for (const auto& n : need)
+    if ((e = ensure(*n.b, n.bytes)) != hipSuccess) return bad(e, "device allocation");
+  if ((e = hipHostMalloc((void**)&B->h_tout, n_tiles * sizeof(TileOut), 0)) != hipSuccess)
+    return bad(e, "hipHostMalloc");
+  if ((e = hipMemcpy(B->weights.p, ctx->weights.p, ctx->weights.bytes, hipMemcpyDeviceToDevice)) !=
+      hipSuccess)
+    return bad(e, "weight copy");
+  if ((e = hipStreamCreateWithFlags(&B->cs, hipStreamNonBlocking)) != hipSuccess)
+    return bad(e, "hipStreamCreate");
+  if ((e = hipStreamBeginCapture(B->cs, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+    return bad(e, "hipStreamBeginCapture");
+  const hipError_t ec = batch_capture_any(ctx, B, B->cs);
+  e = hipStreamEndCapture(B->cs, &B->graph);
+  if (ec != hipSuccess) return bad(ec, "kernel launch during capture");
+  if (e != hipSuccess) return bad(e, "hipStreamEndCapture");
+  if ((e = hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0)) != hipSuccess)
+    return bad(e, "hipGraphInstantiate");
+  ctx->batch = B;
+  return HDOG_OK;
+}
+
+int hdog_batch_run(hdog_ctx* ctx, void* stream) {
+  if (!ctx) return HDOG_ERR_ARG;
+  BatchState* B = ctx->batch;
+  if (!B) return fail(ctx, HDOG_ERR_STATE, "hdog_batch_plan not called");
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t s = (hipStream_t)stream;
+  B->ran = false;
+  HIPCHK(hipGraphLaunch(B->exec, s));
+  HIPCHK(hipMemcpyAsync(B->h_tout, B->tout.p, B->tiles.size() * sizeof(TileOut),
+                        hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  B->ran = true;
+  for (size_t t = 0; t < B->tiles.size(); ++t)
+    if (B->h_tout[t].overflow)
+      return fail(ctx, HDOG_ERR_CAPACITY,
+                  "stack %d: %u regions do not fit (per-stack capacity %lld, arena %lld)", (int)t,
+                  B->h_tout[t].found, (long long)(B->acc_cap - 1), (long long)B->arena_cap);
+  return HDOG_OK;
+}
+
+int hdog_batch_tile_info(hdog_ctx* ctx, int tile, int64_t* num_regions, int* overflow) {
+  if (!ctx) return HDOG_ERR_ARG;
+  BatchState* B = ctx->batch;
+  if (!B || !B->ran) return fail(ctx, HDOG_ERR_STATE, "no completed hdog_batch_run");
+  if (tile < 0 || tile >= (int)B->tiles.size())
+    return fail(ctx, HDOG_ERR_ARG, "stack %d outside [0, %d)", tile, (int)B->tiles.size());
+  if (num_regions) *num_regions = B->h_tout[tile].found;
+  if (overflow) *overflow = (int)B->h_tout[tile].overflow;
+  return HDOG_OK;
+}
+
+int hdog_batch_download(hdog_ctx* ctx, void* stream, int tile, hdog_regions* out) {
+  if (!ctx) return HDOG_ERR_ARG;
+  BatchState* B = ctx->batch;
+  if (!B || !B->ran) return fail(ctx, HDOG_ERR_STATE, "no completed hdog_batch_run");
+  if (tile < 0 || tile >= (int)B->tiles.size() || !out)
+    return fail(ctx, HDOG_ERR_ARG, "stack %d outside [0, %d) or NULL output", tile,
+                (int)B->tiles.size());
+  const TileOut& T = B->h_tout[tile];
+  if (T.overflow)
+    return fail(ctx, HDOG_ERR_CAPACITY, "stack %d overflowed its region capacity", tile);
+  const i64 n = (i64)T.count, off = (i64)T.offset, R = n + 1;
+  out->count = R;
+  if (R > out->capacity)
+    return fail(ctx, HDOG_ERR_CAPACITY, "%lld regions exceed the caller capacity %lld",
+                (long long)R, (long long)out->capacity);
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t s = (hipStream_t)stream;
+  const i64 cap = B->arena_cap;
+  // entry 0 (background) is zero, entries 1..n come from arena slots off..off+n-1
+  auto cp = [&](void* dst, const void* src, size_t esz) -> hipError_t {
+    if (!dst) return hipSuccess;
+    std::memset(dst, 0, esz);
+    if (n == 0) return hipSuccess;
+    return hipMemcpyAsync((char*)dst + esz, (const char*)src + off * esz, n * esz,
+                          hipMemcpyDeviceToHost, s);
+  };
+  const float* cen = (const float*)B->a_cen.p;
+  const float* eig = (const float*)B->a_eig.p;
+  HIPCHK(cp(out->label, B->a_label.p, 4));
+  HIPCHK(cp(out->size, B->a_size.p, 2));
+  HIPCHK(cp(out->voxels, B->a_vox.p, 8));
+  HIPCHK(cp(out->centroid_x, cen, 4));
+  HIPCHK(cp(out->centroid_y, cen + cap, 4));
+  HIPCHK(cp(out->centroid_z, cen + 2 * cap, 4));
+  HIPCHK(cp(out->structureness, eig, 4));
+  HIPCHK(cp(out->blobness, eig + cap, 4));
+  HIPCHK(cp(out->max_normalized, B->a_nrm.p, 4));
+  if (out->hessian)
+    for (int k = 0; k < 6; ++k)
+      HIPCHK(cp(out->hessian + k * out->capacity, (const float*)B->a_hess.p + k * cap, 4));
+  HIPCHK(cp(out->intensity_sum, B->a_isum.p, 8));
+  HIPCHK(cp(out->intensity_max, B->a_imax.p, 2));
+  HIPCHK(hipStreamSynchronize(s));
   return HDOG_OK;
 }
 

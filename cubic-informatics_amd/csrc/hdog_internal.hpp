@@ -105,18 +105,34 @@ hipError_t launch_ccl(const uint8_t* pd, uint32_t* labels, int W, int H, int D, 
 // region numbering: fills rid[root] for every root, block counts, and *d_R = R
 hipError_t launch_region_ids(const uint32_t* labels, uint32_t* rid, uint32_t* block_cnt,
                              uint32_t* d_R, i64 N, hipStream_t s, i64* nblocks_out);
+// R is the host's region count, or, when d_R is non-null, read on the device as
+// min(*d_R, cap) (graph-captured batches, where the host never sees R); ids >= cap are skipped.
 hipError_t launch_region_assign(const uint32_t* labels, const uint32_t* block_off, uint32_t* rid,
-                                RegionAcc* acc, i64 R, i64 N, hipStream_t s);
+                                RegionAcc* acc, i64 R, i64 cap, const uint32_t* d_R, i64 N,
+                                hipStream_t s);
 template <typename TH, typename TN>
 hipError_t launch_region_accum(const uint32_t* labels, const uint32_t* rid, const TH* hess,
-                               const TN* norm, const uint16_t* img, RegionAcc* acc, i64 R,
+                               const TN* norm, const uint16_t* img, RegionAcc* acc, i64 cap,
                                int W, int H, int D, int is_ave, hipStream_t s);
 struct RegionOut {
   uint32_t* label; uint16_t* size; u64* voxels; float* cx; float* cy; float* cz;
   float* st; float* bl; float* nrm; float* hess; u64* isum; uint16_t* imax;
 };
+// Placement of one stack's regions in a batch's region arena (device memory): regions
+// 1..count of the stack occupy arena slots offset .. offset + count - 1.
+struct TileOut {
+  u64 offset, count;
+  uint32_t overflow, found;  // found: regions of the stack, also when they did not fit
+};
+// With t == nullptr: region r goes to slot r, r < R.  With t: regions 1..t->count go to
+// slots t->offset + r - 1 (nothing when t->overflow); R then only sizes the grid.
 hipError_t launch_region_finalize(const RegionAcc* acc, i64 R, int is_ave, RegionOut o,
-                                  i64 pitch, hipStream_t s);
+                                  i64 pitch, hipStream_t s, const TileOut* t = nullptr);
+// t->offset = *running, t->count = *d_R - 1, *running += t->count; overflow (count 0) when
+// *d_R > acc_cap or the arena of arena_cap slots would overflow
+hipError_t launch_tile_reserve(const uint32_t* d_R, i64 acc_cap, i64 arena_cap, u64* running,
+                               TileOut* t, hipStream_t s);
+hipError_t launch_fill_u64(u64* p, i64 n, u64 value, hipStream_t s);
 // differentiable blob response (soft.hip)
 hipError_t launch_soft_loss(const float* hess, const uint8_t* mask, const uint8_t* target, i64 N,
                             double inv_tau, double* sums, hipStream_t s);

@@ -75,7 +75,7 @@ __global__ __launch_bounds__(1024) void k_scan_counts(uint32_t* cnt, i64 nb, uin
 __global__ __launch_bounds__(NT) void k_assign_ids(const uint32_t* __restrict__ L, i64 N,
                                                    const uint32_t* __restrict__ off,
                                                    uint32_t* __restrict__ rid,
-                                                   RegionAcc* __restrict__ acc) {
+                                                   RegionAcc* __restrict__ acc, i64 cap) {
   __shared__ uint32_t wsum[4];
   const i64 base = (i64)blockIdx.x * BLK;
   uint32_t run = off[blockIdx.x];
@@ -95,14 +95,16 @@ __global__ __launch_bounds__(NT) void k_assign_ids(const uint32_t* __restrict__ 
     if (isroot) {
       const uint32_t r = 1 + run + woff + pre;
       rid[i] = r;
-      acc[r].root = (uint32_t)i;
+      if ((i64)r < cap) acc[r].root = (uint32_t)i;
     }
     run += tot;
     __syncthreads();
   }
 }
 
-__global__ __launch_bounds__(NT) void k_zero_acc(RegionAcc* acc, i64 R) {
+__global__ __launch_bounds__(NT) void k_zero_acc(RegionAcc* acc, i64 R, const uint32_t* d_R,
+                                                  i64 cap) {
+  if (d_R) R = std::min<i64>((i64)*d_R, cap);
   const i64 nwords = R * (i64)(sizeof(RegionAcc) / 8);
   unsigned long long* p = (unsigned long long*)acc;
   for (i64 i = (i64)blockIdx.x * NT + threadIdx.x; i < nwords; i += (i64)gridDim.x * NT) p[i] = 0;
@@ -155,8 +157,8 @@ __global__ __launch_bounds__(NT) void k_accum(const uint32_t* __restrict__ L,
                                               const TH* __restrict__ hess,
                                               const TN* __restrict__ norm,
                                               const uint16_t* __restrict__ img, RegionAcc* acc,
-                                              int W, int H, int D, i64 N, int nbx, int nby,
-                                              int is_ave) {
+                                              uint32_t cap, int W, int H, int D, i64 N, int nbx,
+                                              int nby, int is_ave) {
   __shared__ uint32_t key[HS];
   __shared__ uint32_t tcnt[HS], tsx[HS], tsy[HS], tsz[HS], tis[HS], tim[HS], tnk[HS];
   __shared__ double th[6][HS], tns[HS];
@@ -197,7 +199,7 @@ __global__ __launch_bounds__(NT) void k_accum(const uint32_t* __restrict__ L,
   __syncthreads();  // table initialised
 #pragma unroll
   for (int zz = 0; zz < BZ; ++zz) {
-    if (!l[zz]) continue;
+    if (!l[zz] || r[zz] >= cap) continue;
     const uint32_t rz = r[zz];
     uint32_t slot = (rz * 2654435761u) >> (32 - HSB);
     int probes = 0;
@@ -279,8 +281,16 @@ __device__ void eigen_features(const float h6[6], float* st, float* bl) {
 }
 
 __global__ __launch_bounds__(NT) void k_finalize(const RegionAcc* __restrict__ acc, i64 R,
-                                                 int is_ave, RegionOut o, i64 pitch) {
-  for (i64 r = (i64)blockIdx.x * NT + threadIdx.x; r < R; r += (i64)gridDim.x * NT) {
+                                                 int is_ave, RegionOut o, i64 pitch,
+                                                 const TileOut* __restrict__ tile) {
+  i64 base = 0, first = 0;
+  if (tile) {
+    if (tile->overflow) return;
+    R = (i64)tile->count + 1;
+    base = (i64)tile->offset - 1;  // region r >= 1 goes to slot offset + r - 1
+    first = 1;
+  }
+  for (i64 r = (i64)blockIdx.x * NT + threadIdx.x + first; r < R; r += (i64)gridDim.x * NT) {
     if (r == 0) {
       if (o.label) o.label[0] = 0;
       if (o.size) o.size[0] = 0;
@@ -294,24 +304,42 @@ __global__ __launch_bounds__(NT) void k_finalize(const RegionAcc* __restrict__ a
       continue;
     }
     const RegionAcc& a = acc[r];
+    const i64 q = base + r;
     const uint16_t s16 = (uint16_t)a.cnt;
     const float fs = (float)s16;
-    if (o.label) o.label[r] = a.root;
-    if (o.size) o.size[r] = s16;
-    if (o.voxels) o.voxels[r] = a.cnt;
+    if (o.label) o.label[q] = a.root;
+    if (o.size) o.size[q] = s16;
+    if (o.voxels) o.voxels[q] = a.cnt;
     if (o.cx) {
-      o.cx[r] = (float)a.sx / fs;
-      o.cy[r] = (float)a.sy / fs;
-      o.cz[r] = (float)a.sz / fs;
+      o.cx[q] = (float)a.sx / fs;
+      o.cy[q] = (float)a.sy / fs;
+      o.cz[q] = (float)a.sz / fs;
     }
     float h6[6];
     for (int k = 0; k < 6; ++k) h6[k] = (float)a.h[k];
-    if (o.hess) for (int k = 0; k < 6; ++k) o.hess[k * pitch + r] = h6[k];
-    if (o.st) eigen_features(h6, o.st + r, o.bl + r);
-    if (o.nrm) o.nrm[r] = is_ave ? (float)a.nsum : fdecode(a.nmax_key);
-    if (o.isum) o.isum[r] = a.isum;
-    if (o.imax) o.imax[r] = (uint16_t)a.imax;
+    if (o.hess) for (int k = 0; k < 6; ++k) o.hess[k * pitch + q] = h6[k];
+    if (o.st) eigen_features(h6, o.st + q, o.bl + q);
+    if (o.nrm) o.nrm[q] = is_ave ? (float)a.nsum : fdecode(a.nmax_key);
+    if (o.isum) o.isum[q] = a.isum;
+    if (o.imax) o.imax[q] = (uint16_t)a.imax;
   }
+}
+
+__global__ void k_tile_reserve(const uint32_t* __restrict__ d_R, i64 acc_cap, i64 arena_cap,
+                               u64* running, TileOut* t) {
+  if (threadIdx.x != 0) return;
+  const i64 R = (i64)*d_R;
+  const u64 off = *running;
+  const bool over = R > acc_cap || (i64)off + (R - 1) > arena_cap;
+  t->offset = off;
+  t->count = over ? 0 : (u64)(R - 1);
+  t->overflow = over ? 1u : 0u;
+  t->found = (uint32_t)(R - 1);
+  if (!over) *running = off + (u64)(R - 1);
+}
+
+__global__ __launch_bounds__(NT) void k_fill_u64(u64* p, i64 n, u64 v) {
+  for (i64 i = (i64)blockIdx.x * NT + threadIdx.x; i < n; i += (i64)gridDim.x * NT) p[i] = v;
 }
 
 }  // namespace
@@ -330,30 +358,33 @@ hipError_t launch_region_ids(const uint32_t* labels, uint32_t* rid, uint32_t* bl
 }
 
 hipError_t launch_region_assign(const uint32_t* labels, const uint32_t* block_off, uint32_t* rid,
-                                RegionAcc* acc, i64 R, i64 N, hipStream_t s) {
-  const i64 zb = std::min<i64>((R * (i64)(sizeof(RegionAcc) / 8) + NT - 1) / NT, 1 << 16);
+                                RegionAcc* acc, i64 R, i64 cap, const uint32_t* d_R, i64 N,
+                                hipStream_t s) {
+  const i64 words = (d_R ? cap : std::min(R, cap)) * (i64)(sizeof(RegionAcc) / 8);
+  const i64 zb = std::min<i64>((words + NT - 1) / NT, 1 << 16);
   mark(nullptr, s);  // the host waited for the region count: start a new interval
-  hipLaunchKernelGGL(k_zero_acc, dim3((unsigned)std::max<i64>(zb, 1)), dim3(NT), 0, s, acc, R);
+  hipLaunchKernelGGL(k_zero_acc, dim3((unsigned)std::max<i64>(zb, 1)), dim3(NT), 0, s, acc, R, d_R,
+                     cap);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   mark("regions/zero", s);
   const i64 nb = (N + BLK - 1) / BLK;
   hipLaunchKernelGGL(k_assign_ids, dim3((unsigned)nb), dim3(NT), 0, s, labels, N, block_off, rid,
-                     acc);
+                     acc, cap);
   mark("regions/assign_ids", s);
   return hipGetLastError();
 }
 
 template <typename TH, typename TN>
 hipError_t launch_region_accum(const uint32_t* labels, const uint32_t* rid, const TH* hess,
-                               const TN* norm, const uint16_t* img, RegionAcc* acc, i64 R,
+                               const TN* norm, const uint16_t* img, RegionAcc* acc, i64 cap,
                                int W, int H, int D, int is_ave, hipStream_t s) {
-  (void)R;
   const int nbx = (W + 63) / 64, nby = (H + BY - 1) / BY, nbz = (D + BZ - 1) / BZ;
   const i64 nblk = (i64)nbx * nby * nbz;
   const i64 N = (i64)W * H * D;
   hipLaunchKernelGGL((k_accum<TH, TN>), dim3((unsigned)nblk), dim3(NT), 0, s, labels, rid, hess,
-                     norm, img, acc, W, H, D, N, nbx, nby, is_ave);
+                     norm, img, acc, (uint32_t)std::min<i64>(cap, 0xFFFFFFFFll), W, H, D, N, nbx,
+                     nby, is_ave);
   mark("regions/accum", s);
   return hipGetLastError();
 }
@@ -368,10 +399,22 @@ HDOG_INST_ACC(__hip_bfloat16, __hip_bfloat16)
 #undef HDOG_INST_ACC
 
 hipError_t launch_region_finalize(const RegionAcc* acc, i64 R, int is_ave, RegionOut o, i64 pitch,
-                                  hipStream_t s) {
+                                  hipStream_t s, const TileOut* t) {
   const i64 nb = std::max<i64>(1, std::min<i64>((R + NT - 1) / NT, 1 << 16));
-  hipLaunchKernelGGL(k_finalize, dim3((unsigned)nb), dim3(NT), 0, s, acc, R, is_ave, o, pitch);
+  hipLaunchKernelGGL(k_finalize, dim3((unsigned)nb), dim3(NT), 0, s, acc, R, is_ave, o, pitch, t);
   mark("regions/finalize", s);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_reserve(const uint32_t* d_R, i64 acc_cap, i64 arena_cap, u64* running,
+                               TileOut* t, hipStream_t s) {
+  hipLaunchKernelGGL(k_tile_reserve, dim3(1), dim3(64), 0, s, d_R, acc_cap, arena_cap, running, t);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(u64* p, i64 n, u64 value, hipStream_t s) {
+  const i64 nb = std::max<i64>(1, std::min<i64>((n + NT - 1) / NT, 1 << 16));
+  hipLaunchKernelGGL(k_fill_u64, dim3((unsigned)nb), dim3(NT), 0, s, p, n, value);
   return hipGetLastError();
 }
 

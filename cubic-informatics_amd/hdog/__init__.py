@@ -35,7 +35,9 @@ _TORCH_DTYPE = {C.HDOG_FP32: (torch.float32, torch.float32),
 
 
 class HDoGError(RuntimeError):
-    pass
+    def __init__(self, message, status=None):
+        super().__init__(message)
+        self.status = status
 
 
 def _ptr(t):
@@ -62,6 +64,10 @@ class HDoG:
         self.r_z2 = None
         self._last_R = 0
 
+    def release_scratch(self):
+        """Free the context's device scratch (not a batch plan, not the caller's buffers)."""
+        self._check(C.hdog_release_scratch(self._h))
+
     def close(self):
         if getattr(self, "_h", None):
             C.hdog_destroy(self._h)
@@ -76,7 +82,7 @@ class HDoG:
     # -- helpers -----------------------------------------------------------------
     def _check(self, rc):
         if rc != C.HDOG_OK:
-            raise HDoGError(C.hdog_last_error(self._h).decode() or f"status {rc}")
+            raise HDoGError(C.hdog_last_error(self._h).decode() or f"status {rc}", rc)
 
     def _stream(self):
         return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
@@ -149,9 +155,8 @@ class HDoG:
         self._last_R = R.value
         return self.download_regions()
 
-    def download_regions(self):
-        """Copy the last region result to host numpy arrays (index 0 = background)."""
-        R = self._last_R
+    @staticmethod
+    def _region_arrays(R):
         o = dict(label=np.zeros(R, np.uint32), size16=np.zeros(R, np.uint16),
                  count=np.zeros(R, np.uint64), cx=np.zeros(R, np.float32),
                  cy=np.zeros(R, np.float32), cz=np.zeros(R, np.float32),
@@ -166,11 +171,53 @@ class HDoG:
                      ("norm", "max_normalized"), ("hsum", "hessian"), ("isum", "intensity_sum"),
                      ("imax", "intensity_max")):
             setattr(reg, f, o[k].ctypes.data)
-        self._check(C.hdog_regions_download(self._h, self._stream(), ctypes.byref(reg)))
+        return o, reg
+
+    @staticmethod
+    def _finish_regions(o):
         o["centroid"] = np.stack([o["cx"], o["cy"], o["cz"]], axis=1)
         o["eig"] = np.stack([o["structureness"], o["blobness"]], axis=1)
         o["hsum"] = np.ascontiguousarray(o["hsum"].T)
         return o
+
+    def download_regions(self):
+        """Copy the last region result to host numpy arrays (index 0 = background)."""
+        o, reg = self._region_arrays(self._last_R)
+        self._check(C.hdog_regions_download(self._h, self._stream(), ctypes.byref(reg)))
+        return self._finish_regions(o)
+
+    # -- batched stacks (one captured graph) -------------------------------------
+    def batch_plan(self, images, params, max_regions_per_stack=0, arena_capacity=0):
+        """Capture the pipeline of every stack in `images` (uint16/int16 device tensors of
+        shape (D, H, W), kept alive by this object) into one graph.  `params` from
+        make_params; its sizes are ignored."""
+        self._batch_images = [im if im.is_contiguous() else im.contiguous() for im in images]
+        tiles = (C.Tile * len(images))()
+        for t, im in enumerate(self._batch_images):
+            D, H, W = im.shape
+            tiles[t].image = im.data_ptr()
+            tiles[t].width, tiles[t].height, tiles[t].depth = W, H, D
+        self._check(C.hdog_batch_plan(self._h, ctypes.byref(params), tiles, len(images),
+                                      int(max_regions_per_stack), int(arena_capacity)))
+
+    def batch_run(self):
+        self._check(C.hdog_batch_run(self._h, self._stream()))
+
+    def batch_tile_info(self, tile):
+        n = ctypes.c_int64()
+        over = ctypes.c_int()
+        self._check(C.hdog_batch_tile_info(self._h, int(tile), ctypes.byref(n), ctypes.byref(over)))
+        return n.value, bool(over.value)
+
+    def batch_download(self, tile):
+        n, _ = self.batch_tile_info(tile)
+        o, reg = self._region_arrays(n + 1)
+        self._check(C.hdog_batch_download(self._h, self._stream(), int(tile), ctypes.byref(reg)))
+        return self._finish_regions(o)
+
+    def batch_release(self):
+        self._check(C.hdog_batch_release(self._h))
+        self._batch_images = None
 
     # -- whole path ---------------------------------------------------------------
     @staticmethod

@@ -43,6 +43,10 @@ class Params(ctypes.Structure):
                 ("is_ave_mode", _i), ("precision", _i)]
 
 
+class Tile(ctypes.Structure):
+    _fields_ = [("image", _vp), ("width", _i), ("height", _i), ("depth", _i)]
+
+
 def _sig(name, args):
     fn = getattr(lib, name)
     fn.argtypes = args
@@ -70,6 +74,13 @@ hdog_process_substack = _sig("hdog_process_substack", [_vp, _vp, ctypes.POINTER(
                                                        ctypes.POINTER(Regions)])
 hdog_run_device = _sig("hdog_run_device", [_vp, _vp, ctypes.POINTER(Params), _vp, _vp, _vp, _vp,
                                            _vp, _vp, ctypes.POINTER(_i64)])
+hdog_batch_plan = _sig("hdog_batch_plan", [_vp, ctypes.POINTER(Params), ctypes.POINTER(Tile), _i,
+                                           _i64, _i64])
+hdog_batch_run = _sig("hdog_batch_run", [_vp, _vp])
+hdog_batch_tile_info = _sig("hdog_batch_tile_info", [_vp, _i, ctypes.POINTER(_i64),
+                                                     ctypes.POINTER(_i)])
+hdog_batch_download = _sig("hdog_batch_download", [_vp, _vp, _i, ctypes.POINTER(Regions)])
+hdog_batch_release = _sig("hdog_batch_release", [_vp])
 hdog_synth_volume = _sig("hdog_synth_volume", [_vp, _vp, _vp, _vp, _i, _i, _i, ctypes.c_uint64,
                                                _d, _d, _d, _vp, _i64])
 
@@ -89,4 +100,6 @@ EXPORTED = ["hdog_create", "hdog_destroy", "hdog_last_error", "hdog_release_scra
             "hdog_normalize3d", "hdog_init_gaussian3d", "hdog_dog3d", "hdog_hessian_pd",
             "hdog_ccl3d", "hdog_regional_features", "hdog_regions_download",
             "hdog_process_substack", "hdog_run_device", "hdog_synth_volume",
-            "hdog_set_profiling", "hdog_stage_times", "hdog_soft_forward", "hdog_soft_backward"]
+            "hdog_set_profiling", "hdog_stage_times", "hdog_soft_forward", "hdog_soft_backward",
+            "hdog_batch_plan", "hdog_batch_run", "hdog_batch_tile_info", "hdog_batch_download",
+            "hdog_batch_release"]

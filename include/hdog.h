@@ -156,6 +156,37 @@ int hdog_run_device(hdog_ctx* ctx, void* stream, const hdog_params* p, const uin
                     void* d_norm, void* d_dog, void* d_hessian, uint8_t* d_pd,
                     uint32_t* d_labels, int64_t* num_regions);
 
+/* ---- batched stacks ------------------------------------------------------------
+ * Many whole stacks (tiles of one brain) resident on the device, run as ONE captured
+ * hipGraph launch.  Replaces the per-stack fan-out of script/HDoG_gpu.py:67-104, which
+ * starts one HDoG3D_main process per stack and runs each as depth-32 substacks.
+ *
+ * hdog_batch_plan allocates every buffer for the largest stack, a per-stack region
+ * accumulator of max_regions_per_stack entries (0: largest N / 64) and a region arena of
+ * arena_capacity records shared by all stacks (0: total N / 128), and captures the
+ * pipeline of p (sizes ignored, precision honoured) for every tile into one graph.  The
+ * tiles' images must stay valid and unchanged in shape while the plan exists; their
+ * contents may change between runs.  hdog_batch_run launches the graph on `stream`,
+ * waits for it and returns HDOG_ERR_CAPACITY (naming the stack) when a stack's regions
+ * did not fit; the other stacks' results are still valid.  Region ids are per stack,
+ * labels are linear indices within the stack.  One plan per context; a new plan or
+ * hdog_batch_release frees the previous one.  Per-stack calls on the same context do
+ * not disturb a plan (it owns its buffers). */
+typedef struct hdog_tile {
+  const uint16_t* image;  /* device, depth x height x width, x-fastest */
+  int width, height, depth;
+} hdog_tile;
+int hdog_batch_plan(hdog_ctx* ctx, const hdog_params* p, const hdog_tile* tiles, int n_tiles,
+                    int64_t max_regions_per_stack, int64_t arena_capacity);
+int hdog_batch_run(hdog_ctx* ctx, void* stream);
+/* After hdog_batch_run: regions found in stack `tile` (excluding the background) and
+ * whether they fit (1 = overflow, nothing stored). */
+int hdog_batch_tile_info(hdog_ctx* ctx, int tile, int64_t* num_regions, int* overflow);
+/* Copies stack `tile`'s regions into *out in the layout of hdog_regions_download
+ * (entry 0 = background, zero). */
+int hdog_batch_download(hdog_ctx* ctx, void* stream, int tile, hdog_regions* out);
+int hdog_batch_release(hdog_ctx* ctx);
+
 /* ---- kernel timing ----------------------------------------------------------- */
 /* With profiling on, hdog_run_device / hdog_process_substack record a HIP event on the
  * caller's stream after every kernel they enqueue; hdog_stage_times waits for the last

@@ -188,9 +188,10 @@ def test_pipeline_is_deterministic(hd):
 
 
 # measured candidate-set agreement of the reduced-precision storage forms with FP32
-# (Jaccard of the masks); the bars are what the current kernels reach, below the
-# 99.9 % goal for the precision form
-JACCARD_BAR = {"fp16": 0.995, "bf16": 0.96, "fp16_dog32": 0.995, "bf16_dog32": 0.96}
+# (Jaccard of the masks on the 128x256x256 stack: fp16 0.9960, bf16 0.9695, fp16_dog32
+# 0.9995, bf16_dog32 0.9961); fp16_dog32 meets the 99.9 % goal, the others are held at
+# what the kernels reach
+JACCARD_BAR = {"fp16": 0.995, "bf16": 0.965, "fp16_dog32": 0.999, "bf16_dog32": 0.995}
 
 
 @pytest.mark.parametrize("precision", ["fp16", "bf16", "fp16_dog32", "bf16_dog32"])
