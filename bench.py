@@ -18,7 +18,9 @@ storage forms are measured in the same run and reported under "precision_forms" 
 their candidate-set agreement (Jaccard) against the FP32 mask.  `roofline` is the dominant
 kernel (event-timed in a profiled pass after the timed one) against the 8 TB/s HBM
 peak; `pipeline_roofline` is the whole step against the 80 B/voxel (46 B/voxel for
-FP16/BF16 storage, 50 with an FP32 DoG) algorithmic traffic model.  `cpu_baseline` times the CPU oracle
+FP16/BF16 storage, 50 with an FP32 DoG) stage-boundary traffic model, and beside it
+the model of what the step actually moves: the Hessian handed to the region pass as
+28-byte records of the candidate voxels only (about 32 B/voxel at 6.6 % candidates).  `cpu_baseline` times the CPU oracle
 (OpenMP, all host threads) on one 32-slice substack of the same stack.
 
 `batched_form` is the many-stack form: 40 stacks of z ~ U{64..640} and y, x each from
@@ -41,23 +43,28 @@ HBM_PEAK_GBS = 8000.0
 # read or write, once.  sn, sd, sh: element size of the stored normalized volume, DoG and
 # Hessian elements for the storage form being measured.
 KERNEL_BYTES = {
-    "normalize/minmax_z": lambda sn, sd, sh: 2 + 4,
-    "normalize/minmax_y": lambda sn, sd, sh: 4 + 4,
-    "normalize/minmax_x_box_x": lambda sn, sd, sh: 4 + 8,
-    "normalize/box_y": lambda sn, sd, sh: 8 + 8,
-    "normalize/box_z_norm": lambda sn, sd, sh: 8 + 2 + sn,
-    "normalize/copy": lambda sn, sd, sh: 2 + sn,
-    "dog/gauss_x": lambda sn, sd, sh: sn + 8,
-    "dog/gauss_y": lambda sn, sd, sh: 8 + 8,
-    "dog/gauss_z_dog": lambda sn, sd, sh: 8 + sd,
-    "hessian/hessian_pd": lambda sn, sd, sh: sd + 6 * sh + 1,
-    "ccl/local": lambda sn, sd, sh: 1 + 4, "ccl/merge": lambda sn, sd, sh: 1,
-    "ccl/init": lambda sn, sd, sh: 1 + 4, "ccl/union": lambda sn, sd, sh: 1,
-    "ccl/flatten": lambda sn, sd, sh: 4 + 4,
-    "regions/count_roots": lambda sn, sd, sh: 4, "regions/scan": lambda sn, sd, sh: 0,
-    "regions/zero": lambda sn, sd, sh: 0, "regions/assign_ids": lambda sn, sd, sh: 4,
-    "regions/accum": lambda sn, sd, sh: 4 + 6 * sh + sn,
-    "regions/finalize": lambda sn, sd, sh: 0,
+    "normalize/minmax_z": lambda sn, sd, sh, cf: 2 + 4,
+    "normalize/minmax_y": lambda sn, sd, sh, cf: 4 + 4,
+    "normalize/minmax_x_box_x": lambda sn, sd, sh, cf: 4 + 8,
+    "normalize/box_y": lambda sn, sd, sh, cf: 8 + 8,
+    "normalize/box_z_norm": lambda sn, sd, sh, cf: 8 + 2 + sn,
+    "normalize/copy": lambda sn, sd, sh, cf: 2 + sn,
+    "dog/gauss_x": lambda sn, sd, sh, cf: sn + 8,
+    "dog/gauss_y": lambda sn, sd, sh, cf: 8 + 8,
+    "dog/gauss_z_dog": lambda sn, sd, sh, cf: 8 + sd,
+    # mask, plus a 28-byte record (index + six elements) per candidate voxel
+    "hessian/hessian_pd": lambda sn, sd, sh, cf: sd + 1 + 28 * cf,
+    "ccl/local": lambda sn, sd, sh, cf: 1 + 4, "ccl/merge": lambda sn, sd, sh, cf: 1,
+    "ccl/init": lambda sn, sd, sh, cf: 1 + 4, "ccl/union": lambda sn, sd, sh, cf: 1,
+    "ccl/flatten": lambda sn, sd, sh, cf: 4 + 4,
+    "regions/count_roots": lambda sn, sd, sh, cf: 4,
+    "regions/flatten_count_roots": lambda sn, sd, sh, cf: 4 + 4 * cf,
+    "regions/scan": lambda sn, sd, sh, cf: 0,
+    "regions/zero": lambda sn, sd, sh, cf: 0, "regions/assign_ids": lambda sn, sd, sh, cf: 4,
+    "regions/accum": lambda sn, sd, sh, cf: 4 + 6 * sh + sn,
+    # per candidate: its record, label, region id, normalized and raw intensity
+    "regions/accum_candidates": lambda sn, sd, sh, cf: cf * (28 + 4 + 4 + sn + 2),
+    "regions/finalize": lambda sn, sd, sh, cf: 0,
 }
 # (norm, dog, hessian) element sizes of each storage form
 FORM_SIZES = {"fp32": (4, 4, 4), "fp16": (2, 2, 2), "bf16": (2, 2, 2),
@@ -69,6 +76,15 @@ def pipeline_bytes(form):
     read by Hessian) + 6 Hessian elements (write, read by regions) + mask 2 + labels 8."""
     sn, sd, sh = FORM_SIZES[form]
     return 2 + 3 * sn + 2 * sd + 12 * sh + 2 + 8
+
+
+def record_bytes(form, cf):
+    """The same boundaries as the step runs them, with the Hessian passed on as 28-byte
+    records of the candidate voxels (fraction cf) instead of six volumes: img 2 + norm
+    (write, read by DoG) 2 sn + dog (write, read by the Hessian pass) 2 sd + mask 2 +
+    labels 8 + per candidate: record write and read 56, label 4, norm sn, img 2."""
+    sn, sd, sh = FORM_SIZES[form]
+    return 2 + 2 * sn + 2 * sd + 2 + 8 + cf * (56 + 4 + sn + 2)
 
 
 C1 = dict(name="c1", rxy=10, rz=10, min_intensity=0, sigma_xy1=2.0, sigma_z1=1.6,
@@ -85,6 +101,9 @@ def run_form(hd, img, shape, precision, steps, warmup, keep_pd=False):
                             C1["sigma_z1"], C1["sigma_xy2"], C1["sigma_z2"], C1["truncate"],
                             C1["gamma"], False, precision)
     bufs = hd.alloc_intermediates(shape, precision)
+    # the step's product is the region arrays: the Hessian volumes are not requested, so the
+    # library does not materialize them (the region pass reads candidate records)
+    del bufs["hess"]
     hd.set_profiling(False)
     R = 0
     for i in range(warmup):
@@ -276,7 +295,8 @@ def main():
     value = N / (ms_step * 1e-3) / 1e9
     kernels = r["kernels"]
     dom = max(kernels, key=kernels.get)
-    kb = KERNEL_BYTES.get(dom, lambda sn, sd, sh: 0)(*FORM_SIZES[main_f])
+    cf = r["cand"] / N
+    kb = KERNEL_BYTES.get(dom, lambda sn, sd, sh, cf: 0)(*FORM_SIZES[main_f], cf)
     achieved = kb * N / (kernels[dom] * 1e-3) / 1e9
     pipe_gbs = pipeline_bytes(main_f) * N / (ms_step * 1e-3) / 1e9
 
@@ -309,7 +329,10 @@ def main():
                      "alg_bytes_per_voxel": kb,
                      "avg_ms": kernels[dom]},
         "pipeline_roofline": {"bytes_per_voxel": pipeline_bytes(main_f), "achieved": pipe_gbs,
-                              "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": pipe_gbs / HBM_PEAK_GBS},
+                              "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": pipe_gbs / HBM_PEAK_GBS,
+                              "record_model_bytes_per_voxel": record_bytes(main_f, cf),
+                              "record_model_frac": record_bytes(main_f, cf) * N
+                              / (ms_step * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "num_regions": r["R"],
         "candidate_fraction": r["cand"] / N,
         "kernels_ms": {k: round(v, 4) for k, v in kernels.items()},

@@ -21,7 +21,8 @@ struct Buf {
 struct BatchState {
   std::vector<hdog_tile> tiles;
   hdog_params p{};
-  Buf tmp1, tmp2, norm, dog, hess, pd, labels, rid, bcnt, dR, acc, weights, tout, running;
+  Buf tmp1, tmp2, norm, dog, pd, labels, rid, bcnt, dR, acc, weights, tout, running, cand,
+      cand_cnt;
   Buf a_label, a_size, a_vox, a_cen, a_eig, a_nrm, a_hess, a_isum, a_imax;
   i64 acc_cap = 0, arena_cap = 0;
   hipStream_t cs = nullptr;
@@ -35,7 +36,7 @@ struct BatchState {
 struct hdog_ctx {
   int device = 0;
   std::string err;
-  Buf tmp1, tmp2, rid, bcnt, dR, acc, weights, blobs, sw, sums, ddog;
+  Buf tmp1, tmp2, rid, bcnt, dR, acc, weights, blobs, sw, sums, ddog, cand, cand_cnt;
   Buf o_label, o_size, o_vox, o_cen, o_eig, o_nrm, o_hess, o_isum, o_imax;
   i64 out_cap = 0;
   Buf img, norm, dog, hess, pd, labels;  // substack buffers of hdog_process_substack
@@ -136,10 +137,21 @@ int ensure_regions(hdog_ctx* ctx, i64 R) {
   return HDOG_OK;
 }
 
+// Candidate records of the Hessian pass, when the pipeline produced them.
+struct CandRecords {
+  const float* rec = nullptr;
+  const uint32_t* cnt = nullptr;
+  i64 ntiles = 0;
+};
+
+// With cand.rec the region sums come from the candidate records (hess unused), else from
+// the materialized Hessian.  flatten: `labels` come from launch_ccl(flatten = false) and
+// are finished here (written); otherwise they are final and only read.
 template <typename TH, typename TN>
-int regional_features_t(hdog_ctx* ctx, hipStream_t s, const uint32_t* labels, const TH* hess,
+int regional_features_t(hdog_ctx* ctx, hipStream_t s, uint32_t* labels, const TH* hess,
                         const TN* norm, const uint16_t* img, int W, int H, int D, int is_ave,
-                        int64_t* num_regions) {
+                        int64_t* num_regions, CandRecords cand = CandRecords(),
+                        bool flatten = false) {
   const i64 N = (i64)W * H * D;
   HIPCHK(ensure(ctx->rid, (size_t)N * 4));
   const i64 nb = (N + 4095) / 4096;
@@ -147,7 +159,7 @@ int regional_features_t(hdog_ctx* ctx, hipStream_t s, const uint32_t* labels, co
   HIPCHK(ensure(ctx->dR, 64));
   i64 nbo = 0;
   HIPCHK(launch_region_ids(labels, (uint32_t*)ctx->rid.p, (uint32_t*)ctx->bcnt.p,
-                           (uint32_t*)ctx->dR.p, N, s, &nbo));
+                           (uint32_t*)ctx->dR.p, N, s, &nbo, flatten));
   HIPCHK(hipMemcpyAsync(ctx->h_R, ctx->dR.p, 4, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   const i64 R = *ctx->h_R;
@@ -156,8 +168,13 @@ int regional_features_t(hdog_ctx* ctx, hipStream_t s, const uint32_t* labels, co
   RegionAcc* acc = (RegionAcc*)ctx->acc.p;
   HIPCHK(launch_region_assign(labels, (const uint32_t*)ctx->bcnt.p, (uint32_t*)ctx->rid.p, acc, R, R,
                               nullptr, N, s));
-  HIPCHK(launch_region_accum<TH, TN>(labels, (const uint32_t*)ctx->rid.p, hess, norm, img, acc, R,
-                                     W, H, D, is_ave, s));
+  if (cand.rec)
+    HIPCHK(launch_region_accum_cand<TN>(cand.rec, cand.cnt, cand.ntiles, labels,
+                                        (const uint32_t*)ctx->rid.p, norm, img, acc, R, W, H,
+                                        is_ave, s));
+  else
+    HIPCHK(launch_region_accum<TH, TN>(labels, (const uint32_t*)ctx->rid.p, hess, norm, img, acc, R,
+                                       W, H, D, is_ave, s));
   const i64 cap = ctx->out_cap;
   RegionOut o;
   o.label = (uint32_t*)ctx->o_label.p;
@@ -197,10 +214,19 @@ int run_stages(hdog_ctx* ctx, hipStream_t s, const hdog_params* p, const uint16_
   const float nz = powf(ctx->sigma[0], p->gamma * 2) * powf(ctx->sigma[1], p->gamma);
   HIPCHK(launch_dog<TN, TD>(norm, dog, ctx->tmp1.p, ctx->tmp2.p, (const float*)ctx->weights.p,
                             ctx->r, nz, W, H, D, s));
-  HIPCHK(launch_hessian<TD, TH>(dog, hess, pd, W, H, D, s));
-  HIPCHK(launch_ccl(pd, labels, W, H, D, s));
+  // the Hessian pass also leaves compact candidate records, which the region pass reads
+  // instead of the six element volumes (those are written only when hess is non-NULL)
+  CandRecords cand;
+  cand.ntiles = hessian_tiles(W, H, D);
+  HIPCHK(ensure(ctx->cand, (size_t)cand.ntiles * kCandTile * 7 * 4));
+  HIPCHK(ensure(ctx->cand_cnt, (size_t)cand.ntiles * 4));
+  cand.rec = (const float*)ctx->cand.p;
+  cand.cnt = (const uint32_t*)ctx->cand_cnt.p;
+  HIPCHK(launch_hessian<TD, TH>(dog, hess, pd, W, H, D, s, (float*)ctx->cand.p,
+                                (uint32_t*)ctx->cand_cnt.p));
+  HIPCHK(launch_ccl(pd, labels, W, H, D, s, false));  // flattened by the region numbering
   int rc = regional_features_t<TH, TN>(ctx, s, labels, hess, norm, img, W, H, D, p->is_ave_mode,
-                                       num_regions);
+                                       num_regions, cand, true);
   if (rc) return rc;
   ctx->ev_valid = ctx->prof;
   return HDOG_OK;
@@ -272,7 +298,8 @@ void batch_free(BatchState* B) {
   if (B->graph) (void)hipGraphDestroy(B->graph);
   if (B->cs) (void)hipStreamDestroy(B->cs);
   if (B->h_tout) (void)hipHostFree(B->h_tout);
-  for (Buf* b : {&B->tmp1, &B->tmp2, &B->norm, &B->dog, &B->hess, &B->pd, &B->labels, &B->rid,
+  for (Buf* b : {&B->tmp1, &B->tmp2, &B->norm, &B->dog, &B->pd, &B->labels, &B->rid, &B->cand,
+                 &B->cand_cnt,
                  &B->bcnt, &B->dR, &B->acc, &B->weights, &B->tout, &B->running, &B->a_label,
                  &B->a_size, &B->a_vox, &B->a_cen, &B->a_eig, &B->a_nrm, &B->a_hess, &B->a_isum,
                  &B->a_imax})
@@ -290,7 +317,7 @@ hipError_t batch_tile(hdog_ctx* ctx, BatchState* B, int t, hipStream_t s) {
   const hdog_params* p = &B->p;
   TN* norm = (TN*)B->norm.p;
   TD* dog = (TD*)B->dog.p;
-  TH* hess = (TH*)B->hess.p;
+  TH* hess = nullptr;  // not materialized: the region pass reads the candidate records
   uint8_t* pd = (uint8_t*)B->pd.p;
   uint32_t* labels = (uint32_t*)B->labels.p;
   hipError_t e;
@@ -301,11 +328,15 @@ hipError_t batch_tile(hdog_ctx* ctx, BatchState* B, int t, hipStream_t s) {
   if ((e = launch_dog<TN, TD>(norm, dog, B->tmp1.p, B->tmp2.p, (const float*)B->weights.p, ctx->r,
                               nz, W, H, D, s)) != hipSuccess)
     return e;
-  if ((e = launch_hessian<TD, TH>(dog, hess, pd, W, H, D, s)) != hipSuccess) return e;
-  if ((e = launch_ccl(pd, labels, W, H, D, s)) != hipSuccess) return e;
+  const i64 ntiles = hessian_tiles(W, H, D);
+  if ((e = launch_hessian<TD, TH>(dog, hess, pd, W, H, D, s, (float*)B->cand.p,
+                                  (uint32_t*)B->cand_cnt.p)) != hipSuccess)
+    return e;
+  if ((e = launch_ccl(pd, labels, W, H, D, s, false)) != hipSuccess) return e;
   i64 nbo = 0;
   uint32_t* dR = (uint32_t*)B->dR.p;
-  if ((e = launch_region_ids(labels, (uint32_t*)B->rid.p, (uint32_t*)B->bcnt.p, dR, N, s, &nbo)) !=
+  if ((e = launch_region_ids(labels, (uint32_t*)B->rid.p, (uint32_t*)B->bcnt.p, dR, N, s, &nbo,
+                             true)) !=
       hipSuccess)
     return e;
   TileOut* tout = (TileOut*)B->tout.p + t;
@@ -316,8 +347,9 @@ hipError_t batch_tile(hdog_ctx* ctx, BatchState* B, int t, hipStream_t s) {
   if ((e = launch_region_assign(labels, (const uint32_t*)B->bcnt.p, (uint32_t*)B->rid.p, acc,
                                 B->acc_cap, B->acc_cap, dR, N, s)) != hipSuccess)
     return e;
-  if ((e = launch_region_accum<TH, TN>(labels, (const uint32_t*)B->rid.p, hess, norm, T.image, acc,
-                                       B->acc_cap, W, H, D, p->is_ave_mode, s)) != hipSuccess)
+  if ((e = launch_region_accum_cand<TN>((const float*)B->cand.p, (const uint32_t*)B->cand_cnt.p,
+                                        ntiles, labels, (const uint32_t*)B->rid.p, norm, T.image,
+                                        acc, B->acc_cap, W, H, p->is_ave_mode, s)) != hipSuccess)
     return e;
   const i64 cap = B->arena_cap;
   RegionOut o;
@@ -384,7 +416,7 @@ int hdog_release_scratch(hdog_ctx* ctx) {
                  &ctx->sw, &ctx->sums, &ctx->ddog,
                  &ctx->o_label, &ctx->o_size, &ctx->o_vox, &ctx->o_cen, &ctx->o_eig, &ctx->o_nrm,
                  &ctx->o_hess, &ctx->o_isum, &ctx->o_imax, &ctx->img, &ctx->norm, &ctx->dog,
-                 &ctx->hess, &ctx->pd, &ctx->labels})
+                 &ctx->hess, &ctx->pd, &ctx->labels, &ctx->cand, &ctx->cand_cnt})
     release(*b);
   ctx->out_cap = 0;
   ctx->R = 0;
@@ -448,6 +480,31 @@ int hdog_normalize3d(hdog_ctx* ctx, void* stream, const uint16_t* d_img, float* 
   HIPCHK(ensure(ctx->tmp2, (size_t)N * 8));
   HIPCHK(launch_normalize<float>(d_img, d_norm, ctx->tmp1.p, ctx->tmp2.p, min_intensity, width,
                                  height, depth, radius_xy, radius_z, (hipStream_t)stream));
+  return HDOG_OK;
+}
+
+int hdog_normalize3d_ed(hdog_ctx* ctx, void* stream, const uint16_t* d_img, float* d_norm,
+                        float* d_erosion, float* d_dilation, float min_intensity, int width,
+                        int height, int depth, int radius_xy, int radius_z) {
+  if (!ctx) return HDOG_ERR_ARG;
+  int rc = check_shape(ctx, width, height, depth);
+  if (rc) return rc;
+  if (!d_img || !d_norm || !d_erosion || !d_dilation)
+    return fail(ctx, HDOG_ERR_ARG, "NULL device pointer");
+  if (radius_xy < 0 || radius_z < 0 || radius_xy > kMaxNormRadius || radius_z > kMaxNormRadius)
+    return fail(ctx, HDOG_ERR_ARG, "normalization radii (%d, %d) outside [0, %d]", radius_xy,
+                radius_z, kMaxNormRadius);
+  HIPCHK(hipSetDevice(ctx->device));
+  const i64 N = (i64)width * height * depth;
+  hipStream_t s = (hipStream_t)stream;
+  HIPCHK(ensure(ctx->tmp1, (size_t)N * 8));
+  HIPCHK(ensure(ctx->tmp2, (size_t)N * 8));
+  if (radius_xy == 0 || radius_z == 0) {  // thresholded copy: no filters run, E = D = 0
+    HIPCHK(hipMemsetAsync(d_erosion, 0, (size_t)N * 4, s));
+    HIPCHK(hipMemsetAsync(d_dilation, 0, (size_t)N * 4, s));
+  }
+  HIPCHK(launch_normalize<float>(d_img, d_norm, ctx->tmp1.p, ctx->tmp2.p, min_intensity, width,
+                                 height, depth, radius_xy, radius_z, s, d_erosion, d_dilation));
   return HDOG_OK;
 }
 
@@ -527,8 +584,10 @@ int hdog_regional_features(hdog_ctx* ctx, void* stream, const uint32_t* d_labels
   if (rc) return rc;
   if (!d_labels || !d_hessian || !d_norm) return fail(ctx, HDOG_ERR_ARG, "NULL device pointer");
   HIPCHK(hipSetDevice(ctx->device));
-  return regional_features_t<float, float>(ctx, (hipStream_t)stream, d_labels, d_hessian, d_norm,
-                                           d_img, width, height, depth, is_ave_mode, num_regions);
+  // flatten = false: the caller's labels are only read
+  return regional_features_t<float, float>(ctx, (hipStream_t)stream, const_cast<uint32_t*>(d_labels),
+                                           d_hessian, d_norm, d_img, width, height, depth,
+                                           is_ave_mode, num_regions);
 }
 
 int hdog_regions_download(hdog_ctx* ctx, void* stream, hdog_regions* out) {
@@ -587,7 +646,7 @@ int hdog_run_device(hdog_ctx* ctx, void* stream, const hdog_params* p, const uin
   const size_t ed = (p->precision == HDOG_FP16 || p->precision == HDOG_BF16) ? 2 : 4;  // dog
   if (!d_norm) { HIPCHK(ensure(ctx->norm, N * es)); d_norm = ctx->norm.p; }
   if (!d_dog) { HIPCHK(ensure(ctx->dog, N * ed)); d_dog = ctx->dog.p; }
-  if (!d_hessian) { HIPCHK(ensure(ctx->hess, 6 * N * es)); d_hessian = ctx->hess.p; }
+  // no Hessian buffer requested: it is not materialized (the region pass recomputes it)
   if (!d_pd) { HIPCHK(ensure(ctx->pd, N)); d_pd = (uint8_t*)ctx->pd.p; }
   if (!d_labels) { HIPCHK(ensure(ctx->labels, N * 4)); d_labels = (uint32_t*)ctx->labels.p; }
   hipStream_t s = (hipStream_t)stream;
@@ -763,7 +822,7 @@ int hdog_batch_plan(hdog_ctx* ctx, const hdog_params* p, const hdog_tile* tiles,
     return fail(ctx, HDOG_ERR_ARG, "negative capacity");
   HIPCHK(hipSetDevice(ctx->device));
   hdog_batch_release(ctx);
-  i64 maxN = 0, totN = 0;
+  i64 maxN = 0, totN = 0, maxT = 0;
   for (int t = 0; t < n_tiles; ++t) {
     hdog_params q = *p;
     q.width = tiles[t].width;
@@ -774,6 +833,7 @@ int hdog_batch_plan(hdog_ctx* ctx, const hdog_params* p, const hdog_tile* tiles,
     if (!tiles[t].image) return fail(ctx, HDOG_ERR_ARG, "stack %d: NULL image", t);
     const i64 N = (i64)q.width * q.height * q.depth;
     maxN = std::max(maxN, N);
+    maxT = std::max(maxT, hessian_tiles(q.width, q.height, q.depth));
     totN += N;
   }
   int rc = ensure_gauss(ctx, p);
@@ -791,7 +851,7 @@ int hdog_batch_plan(hdog_ctx* ctx, const hdog_params* p, const hdog_tile* tiles,
     Buf* b;
     size_t bytes;
   } need[] = {{&B->tmp1, (size_t)maxN * 8}, {&B->tmp2, (size_t)maxN * 8},
-              {&B->norm, maxN * es}, {&B->dog, maxN * ed}, {&B->hess, 6 * maxN * es},
+              {&B->norm, maxN * es}, {&B->dog, maxN * ed},
               {&B->pd, (size_t)maxN}, {&B->labels, (size_t)maxN * 4}, {&B->rid, (size_t)maxN * 4},
               {&B->bcnt, (size_t)((maxN + 4095) / 4096) * 4}, {&B->dR, 64},
               {&B->acc, (size_t)B->acc_cap * sizeof(RegionAcc)}, {&B->weights, ctx->weights.bytes},
@@ -800,7 +860,8 @@ int hdog_batch_plan(hdog_ctx* ctx, const hdog_params* p, const hdog_tile* tiles,
               {&B->a_vox, (size_t)cap * 8}, {&B->a_cen, (size_t)cap * 12},
               {&B->a_eig, (size_t)cap * 8}, {&B->a_nrm, (size_t)cap * 4},
               {&B->a_hess, (size_t)cap * 24}, {&B->a_isum, (size_t)cap * 8},
-              {&B->a_imax, (size_t)cap * 2}};
+              {&B->a_imax, (size_t)cap * 2},
+              {&B->cand, (size_t)maxT * kCandTile * 7 * 4}, {&B->cand_cnt, (size_t)maxT * 4}};
   auto bad = [&](hipError_t e, const char* what) {
     batch_free(B);
     return fail(ctx, HDOG_ERR_HIP, "batch plan: %s failed: %s", what, hipGetErrorString(e));

@@ -42,15 +42,6 @@ __device__ __forceinline__ uint32_t find_root(uint32_t* L, uint32_t x) {
   return x;
 }
 
-__device__ __forceinline__ uint32_t find_root_readonly(const uint32_t* L, uint32_t x) {
-  uint32_t p = L[x];
-  while (p != x) {
-    x = p;
-    p = L[x];
-  }
-  return x;
-}
-
 // links the sets of a and b; returns the root of the union at the time of the link
 __device__ uint32_t unite(uint32_t* L, uint32_t a, uint32_t b) {
   while (true) {
@@ -103,7 +94,15 @@ __global__ __launch_bounds__(NT) void k_ccl_flatten(uint32_t* L, i64 N) {
 // links, with the global union-find, only the pairs that leave the tile or wrap.  For
 // width, height >= 2 the 13 forward neighbours (dz = 1; dz = 0, dy = 1; dz = dy = 0,
 // dx = 1) are exactly the positive linear offsets of the reference's adjacency.
-constexpr int CX = 32, CY = 8, CZ = 8, CV = CX * CY * CZ, CPT = CV / NT;
+#ifndef HDOG_CCL_CY
+#define HDOG_CCL_CY 8
+#endif
+#ifndef HDOG_CCL_CZ
+#define HDOG_CCL_CZ 8
+#endif
+constexpr int CX = 32, CY = HDOG_CCL_CY, CZ = HDOG_CCL_CZ, CV = CX * CY * CZ;
+constexpr int NTC = CV >= 4096 ? 512 : 256, CPT = CV / NTC;
+static_assert(CV % NTC == 0 && (CY & (CY - 1)) == 0 && (CZ & (CZ - 1)) == 0, "tile shape");
 
 __device__ __forceinline__ uint32_t lds_find(const uint32_t* lab, uint32_t x) {
   uint32_t p = lab[x];
@@ -145,7 +144,7 @@ __device__ __forceinline__ void fwd_offset(int k, int& dx, int& dy, int& dz) {
   else { dz = 0; dy = 0; dx = 1; }
 }
 
-__global__ __launch_bounds__(NT) void k_ccl_local(const uint8_t* __restrict__ pd,
+__global__ __launch_bounds__(NTC) void k_ccl_local(const uint8_t* __restrict__ pd,
                                                   uint32_t* __restrict__ L, int W, int H, int D,
                                                   int nbx, int nby) {
   constexpr uint32_t NONE = 0xFFFFFFFFu;
@@ -158,21 +157,21 @@ __global__ __launch_bounds__(NT) void k_ccl_local(const uint8_t* __restrict__ pd
   uint8_t fg[CPT];
 #pragma unroll
   for (int k = 0; k < CPT; ++k) {
-    const int v = threadIdx.x + k * NT;
+    const int v = threadIdx.x + k * NTC;
     const int gx = x0 + (v & (CX - 1)), gy = y0 + ((v / CX) & (CY - 1)), gz = z0 + v / (CX * CY);
     const i64 g = (i64)gz * WH + (i64)gy * W + gx;
     fg[k] = (gx < W && gy < H && gz < D && g > 0) ? pd[g] : 0;
   }
 #pragma unroll
   for (int k = 0; k < CPT; ++k) {
-    const int v = threadIdx.x + k * NT;
+    const int v = threadIdx.x + k * NTC;
     lab[v] = fg[k] ? (uint32_t)v : NONE;
   }
   __syncthreads();
 #pragma unroll 1
   for (int k = 0; k < CPT; ++k) {
     if (!fg[k]) continue;
-    const int v = threadIdx.x + k * NT;
+    const int v = threadIdx.x + k * NTC;
     const int lx = v & (CX - 1), ly = (v / CX) & (CY - 1), lz = v / (CX * CY);
     uint32_t root = (uint32_t)v;  // a representative of v's set, refreshed by each link
 #pragma unroll
@@ -191,7 +190,7 @@ __global__ __launch_bounds__(NT) void k_ccl_local(const uint8_t* __restrict__ pd
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < CPT; ++k) {
-    const int v = threadIdx.x + k * NT;
+    const int v = threadIdx.x + k * NTC;
     const int gx = x0 + (v & (CX - 1)), gy = y0 + ((v / CX) & (CY - 1)), gz = z0 + v / (CX * CY);
     if (gx >= W || gy >= H || gz >= D) continue;
     const i64 g = (i64)gz * WH + (i64)gy * W + gx;
@@ -206,23 +205,56 @@ __global__ __launch_bounds__(NT) void k_ccl_local(const uint8_t* __restrict__ pd
   }
 }
 
-__global__ __launch_bounds__(NT) void k_ccl_merge(const uint8_t* __restrict__ pd, uint32_t* L,
-                                                  int W, int H, int D, int nbx, int nby) {
+// Kernel 2 visits only the voxels that can have a neighbour outside their tile or the
+// volume: with the tile clipped to the volume as ex x ey x ez, its top z-face (lz = ez-1)
+// and the rim of every lower plane (lx in {0, ex-1} or ly in {0, ey-1}); every other
+// voxel's 13 forward neighbours lie inside the clipped tile.  Those voxels are numbered
+// densely so that each wave carries only rim work.
+__device__ __forceinline__ void rim_voxel(int t, int ex, int ey, int ez, int& lx, int& ly,
+                                          int& lz) {
+  const int top = ex * ey;
+  if (t < top) {
+    lz = ez - 1;
+    ly = t / ex;
+    lx = t - ly * ex;
+    return;
+  }
+  t -= top;
+  const int rows = (ey >= 2 ? 2 : 1) * ex, nc = ex >= 2 ? 2 : 1;
+  const int ring = rows + (ey >= 2 ? ey - 2 : 0) * nc;
+  lz = t / ring;
+  const int r = t - lz * ring;
+  if (r < rows) {
+    ly = r < ex ? 0 : ey - 1;
+    lx = r < ex ? r : r - ex;
+  } else {
+    const int r2 = r - rows;
+    ly = 1 + r2 / nc;
+    lx = (r2 % nc) ? ex - 1 : 0;
+  }
+}
+
+__device__ __forceinline__ int rim_count(int ex, int ey, int ez) {
+  const int rows = (ey >= 2 ? 2 : 1) * ex, nc = ex >= 2 ? 2 : 1;
+  return ex * ey + (ez - 1) * (rows + (ey >= 2 ? ey - 2 : 0) * nc);
+}
+
+constexpr int NTM = 256;
+
+__global__ __launch_bounds__(NTM) void k_ccl_merge(const uint8_t* __restrict__ pd, uint32_t* L,
+                                                   int W, int H, int D, int nbx, int nby) {
   unsigned b = blockIdx.x;
   const int bx = b % nbx; b /= nbx;
   const int by = b % nby; b /= nby;
   const int x0 = bx * CX, y0 = by * CY, z0 = (int)b * CZ;
+  const int ex = min(CX, W - x0), ey = min(CY, H - y0), ez = min(CZ, D - z0);
   const i64 WH = (i64)W * H, N = WH * D;
+  const int F = rim_count(ex, ey, ez);
 #pragma unroll 1
-  for (int k = 0; k < CPT; ++k) {
-    const int v = threadIdx.x + k * NT;
-    const int lx = v & (CX - 1), ly = (v / CX) & (CY - 1), lz = v / (CX * CY);
+  for (int t = threadIdx.x; t < F; t += NTM) {
+    int lx, ly, lz;
+    rim_voxel(t, ex, ey, ez, lx, ly, lz);
     const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
-    if (gx >= W || gy >= H || gz >= D) continue;
-    // interior: every forward neighbour is inside the tile and the volume (linked locally)
-    if (lx >= 1 && lx <= CX - 2 && ly >= 1 && ly <= CY - 2 && lz <= CZ - 2 && gx >= 1 &&
-        gx <= W - 2 && gy >= 1 && gy <= H - 2 && gz <= D - 2)
-      continue;
     const i64 g = (i64)gz * WH + (i64)gy * W + gx;
     if (g == 0 || !pd[g]) continue;
     // neighbours linked here: those leaving the tile or the volume (wrap); their mask bytes
@@ -234,25 +266,32 @@ __global__ __launch_bounds__(NT) void k_ccl_merge(const uint8_t* __restrict__ pd
       int dx, dy, dz;
       fwd_offset(q, dx, dy, dz);
       const int nx = lx + dx, ny = ly + dy, nz = lz + dz;
-      const bool inside_volume = gx + dx >= 0 && gx + dx < W && gy + dy >= 0 && gy + dy < H &&
-                                 gz + dz < D;
-      const bool local = inside_volume && nx >= 0 && nx < CX && ny >= 0 && ny < CY && nz < CZ;
+      const bool local = nx >= 0 && nx < ex && ny >= 0 && ny < ey && nz < ez;
       nj[q] = g + (i64)dz * WH + (i64)dy * W + dx;  // linear offset, wraps like the reference
       m[q] = (!local && nj[q] < N) ? pd[nj[q]] : (uint8_t)0;
     }
     uint32_t lj[13];
 #pragma unroll
     for (int q = 0; q < 13; ++q) lj[q] = m[q] ? L[nj[q]] : 0u;
+    // the outside neighbours mostly belong to one or two tile-local components: link each
+    // distinct neighbour label once (labels are tile-local roots after kernel 1)
     uint32_t root = L[g];
+    uint32_t seen0 = root, seen1 = 0u, seen2 = 0u;
 #pragma unroll
-    for (int q = 0; q < 13; ++q)
-      if (m[q] && lj[q] != root) root = unite(L, root, (uint32_t)nj[q]);
+    for (int q = 0; q < 13; ++q) {
+      const uint32_t l = lj[q];
+      if (!m[q] || l == seen0 || l == seen1 || l == seen2 || l == root) continue;
+      root = unite(L, root, l);
+      seen2 = seen1;
+      seen1 = l;
+    }
   }
 }
 
 }  // namespace
 
-hipError_t launch_ccl(const uint8_t* pd, uint32_t* labels, int W, int H, int D, hipStream_t s) {
+hipError_t launch_ccl(const uint8_t* pd, uint32_t* labels, int W, int H, int D, hipStream_t s,
+                      bool flatten) {
   const i64 WH = (i64)W * H, N = WH * D;
   Offsets offs{};
   {
@@ -276,11 +315,11 @@ hipError_t launch_ccl(const uint8_t* pd, uint32_t* labels, int W, int H, int D, 
   if (W >= 2 && H >= 2) {
     const int nbx = (W + CX - 1) / CX, nby = (H + CY - 1) / CY, nbz = (D + CZ - 1) / CZ;
     const i64 nt = (i64)nbx * nby * nbz;
-    hipLaunchKernelGGL(k_ccl_local, dim3((unsigned)nt), dim3(NT), 0, s, pd, labels, W, H, D, nbx,
+    hipLaunchKernelGGL(k_ccl_local, dim3((unsigned)nt), dim3(NTC), 0, s, pd, labels, W, H, D, nbx,
                        nby);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     mark("ccl/local", s);
-    hipLaunchKernelGGL(k_ccl_merge, dim3((unsigned)nt), dim3(NT), 0, s, pd, labels, W, H, D, nbx,
+    hipLaunchKernelGGL(k_ccl_merge, dim3((unsigned)nt), dim3(NTM), 0, s, pd, labels, W, H, D, nbx,
                        nby);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     mark("ccl/merge", s);
@@ -292,6 +331,7 @@ hipError_t launch_ccl(const uint8_t* pd, uint32_t* labels, int W, int H, int D, 
     if ((e = hipGetLastError()) != hipSuccess) return e;
     mark("ccl/union", s);
   }
+  if (!flatten) return hipSuccess;
   hipLaunchKernelGGL(k_ccl_flatten, dim3((unsigned)nb), dim3(NT), 0, s, labels, N);
   mark("ccl/flatten", s);
   return hipGetLastError();

@@ -15,6 +15,14 @@ namespace hdog {
 constexpr int kMaxGaussRadius = 40;            // 2r+1 <= 81 taps per axis
 constexpr int kMaxTaps = 2 * kMaxGaussRadius + 1;
 constexpr int kMaxNormRadius = 200;
+// Hessian tiles: 64 x 8 outputs per work-group over 16 slices.  Candidate records are kept
+// per tile in a block of kCandTile slots x 7 words (voxel index, then the six elements).
+constexpr int kHessTX = 64, kHessTY = 8, kHessTZ = 16;
+constexpr int kCandTile = kHessTX * kHessTY * kHessTZ;
+inline i64 hessian_tiles(int W, int H, int D) {
+  return (i64)((W + kHessTX - 1) / kHessTX) * ((H + kHessTY - 1) / kHessTY) *
+         ((D + kHessTZ - 1) / kHessTZ);
+}
 
 // ---- storage types of float intermediates --------------------------------------
 template <typename T> __device__ __forceinline__ float ld(const T* p, i64 i);
@@ -84,12 +92,41 @@ inline void mark(const char* name, hipStream_t s) {
   }
 }
 
+// One zero-padded central difference ((0 + hi) - lo) * 0.5 in the reference's order; the
+// Hessian kernel and the region pass that recomputes the Hessian share it.
+__device__ __forceinline__ float cdiff_half(float hi, float lo) {
+  float s = 0.0f;
+  s += hi;
+  s -= lo;
+  s *= 0.5f;
+  return s;
+}
+
+// Root of x in a union-find forest stored as parent indices (parents only decrease, a
+// root is its own parent).  Read-only: safe while other threads replace parents by
+// ancestors.
+__device__ __forceinline__ uint32_t find_root_readonly(const uint32_t* L, uint32_t x) {
+  uint32_t p = L[x];
+  while (p != x) {
+    x = p;
+    p = L[x];
+  }
+  return x;
+}
+
+// v rounded to the storage type T and widened back
+template <typename T> __device__ __forceinline__ float round_to(float v) {
+  const T t = cvt<T>(v);
+  return ld(&t, 0);
+}
+
 // ---- launchers (return hipSuccess or the launch error) ----------------------------
-// normalize stage; tmp1/tmp2 each hold >= 8*N bytes
+// normalize stage; tmp1/tmp2 each hold >= 8*N bytes.  ero / dil (optional): the box means
+// of the min and max filters, E and D, as separate float volumes.
 template <typename TS>
 hipError_t launch_normalize(const uint16_t* img, TS* norm, void* tmp1, void* tmp2,
                             float min_intensity, int W, int H, int D, int rxy, int rz,
-                            hipStream_t s);
+                            hipStream_t s, float* ero = nullptr, float* dil = nullptr);
 // DoG stage; weights: device array [4][kMaxTaps] (xy1, z1, xy2, z2); tmp1/tmp2 >= 8*N bytes
 template <typename TN, typename TD>
 hipError_t launch_dog(const TN* norm, TD* dog, void* tmp1, void* tmp2, const float* weights,
@@ -98,13 +135,20 @@ hipError_t launch_gauss_pair(const float* src, void* tmp1, void* tmp2, const flo
                              const int r[6], float nz, float* dog, float2* pair_out, int W, int H,
                              int D, hipStream_t s);
 template <typename TD, typename TH>
+// hess may be NULL (mask only).  cand / cand_cnt (optional): per Hessian tile, the number
+// of candidate voxels and their records (index, six elements rounded to TH, as float);
+// cand holds hessian_tiles() * kCandTile * 7 words.
 hipError_t launch_hessian(const TD* dog, TH* hess, uint8_t* pd, int W, int H, int D,
-                          hipStream_t s);
+                          hipStream_t s, float* cand = nullptr, uint32_t* cand_cnt = nullptr);
 // CCL: labels out; union-find in place
-hipError_t launch_ccl(const uint8_t* pd, uint32_t* labels, int W, int H, int D, hipStream_t s);
+// flatten = false leaves non-root voxels pointing at an ancestor of their root; the region
+// numbering (launch_region_ids with flatten = true) then finishes the labels in its pass.
+hipError_t launch_ccl(const uint8_t* pd, uint32_t* labels, int W, int H, int D, hipStream_t s,
+                      bool flatten = true);
 // region numbering: fills rid[root] for every root, block counts, and *d_R = R
-hipError_t launch_region_ids(const uint32_t* labels, uint32_t* rid, uint32_t* block_cnt,
-                             uint32_t* d_R, i64 N, hipStream_t s, i64* nblocks_out);
+hipError_t launch_region_ids(uint32_t* labels, uint32_t* rid, uint32_t* block_cnt,
+                             uint32_t* d_R, i64 N, hipStream_t s, i64* nblocks_out,
+                             bool flatten = false);
 // R is the host's region count, or, when d_R is non-null, read on the device as
 // min(*d_R, cap) (graph-captured batches, where the host never sees R); ids >= cap are skipped.
 hipError_t launch_region_assign(const uint32_t* labels, const uint32_t* block_off, uint32_t* rid,
@@ -114,6 +158,12 @@ template <typename TH, typename TN>
 hipError_t launch_region_accum(const uint32_t* labels, const uint32_t* rid, const TH* hess,
                                const TN* norm, const uint16_t* img, RegionAcc* acc, i64 cap,
                                int W, int H, int D, int is_ave, hipStream_t s);
+// Region sums from the candidate records of launch_hessian (ntiles = hessian_tiles()).
+template <typename TN>
+hipError_t launch_region_accum_cand(const float* cand, const uint32_t* cand_cnt, i64 ntiles,
+                                    const uint32_t* labels, const uint32_t* rid, const TN* norm,
+                                    const uint16_t* img, RegionAcc* acc, i64 cap, int W, int H,
+                                    int is_ave, hipStream_t s);
 struct RegionOut {
   uint32_t* label; uint16_t* size; u64* voxels; float* cx; float* cy; float* cz;
   float* st; float* bl; float* nrm; float* hess; u64* isum; uint16_t* imax;

@@ -22,16 +22,10 @@
 namespace hdog {
 namespace {
 
-constexpr int TX = 64, TY = 8, ZC = 16;
+constexpr int TX = kHessTX, TY = kHessTY, ZC = kHessTZ;
 constexpr int NT = 256;
 
-__device__ __forceinline__ float g2(float hi, float lo) {
-  float s = 0.0f;
-  s += hi;
-  s -= lo;
-  s *= 0.5f;
-  return s;
-}
+__device__ __forceinline__ float g2(float hi, float lo) { return cdiff_half(hi, lo); }
 
 __device__ __forceinline__ uint8_t negdef32(float xx, float xy, float xz, float yy, float yz,
                                             float zz) {
@@ -67,7 +61,8 @@ __device__ __forceinline__ uint8_t negdef_band(float xx, float xy, float xz, flo
 template <typename TD, typename TS, bool BAND>
 __global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* __restrict__ hess,
                                                 uint8_t* __restrict__ pd, int W, int H, int D,
-                                                int nbx, int nby) {
+                                                int nbx, int nby, float* __restrict__ cand,
+                                                uint32_t* __restrict__ cand_cnt) {
   constexpr int DX = TX + 4, DY = TY + 4;        // dog slice with halo 2
   constexpr int XX = TX + 2, XY = TY + 2;        // Hx slice with halo 1 in x and y
   constexpr int YX = TX, YY = TY + 2;            // Hy slice with halo 1 in y
@@ -75,6 +70,10 @@ __global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* 
   __shared__ float sHx[3][XY * XX];
   __shared__ float sHy[3][YY * YX];
   __shared__ float sHz[3][TY * TX];
+  __shared__ uint32_t s_cnt;  // candidate records written by this work-group
+  if (threadIdx.x == 0) s_cnt = 0;
+  float* rec = cand ? cand + (size_t)blockIdx.x * kCandTile * 7 : nullptr;
+  const int lane = threadIdx.x & 63;
   unsigned b = blockIdx.x;
   const int bx = b % nbx; b /= nbx;
   const int by = b % nby; b /= nby;
@@ -166,6 +165,9 @@ __global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* 
       for (int k = 0; k < TY / (NT / TX); ++k) {
         const int ty = ty0 + k * (NT / TX);
         const int gy = y0 + ty;
+        bool is_cand = false;
+        float h6[6];
+        uint32_t cidx = 0;
         if (gx < W && gy < H) {
           const int cx = (ty + 1) * XX + tx + 1;  // Hx centre
           const int cy = (ty + 1) * YX + tx;      // Hy centre
@@ -177,13 +179,37 @@ __global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* 
           const float yz = g2(hyp[cy], hym[cy]);
           const float zz = g2(hzp[cz], hzm[cz]);
           const i64 idx = (i64)z * WH + (i64)gy * W + gx;
-          hess[idx] = cvt<TS>(xx);
-          hess[N + idx] = cvt<TS>(xy);
-          hess[2 * N + idx] = cvt<TS>(xz);
-          hess[3 * N + idx] = cvt<TS>(yy);
-          hess[4 * N + idx] = cvt<TS>(yz);
-          hess[5 * N + idx] = cvt<TS>(zz);
-          pd[idx] = BAND ? negdef_band(xx, xy, xz, yy, yz, zz) : negdef32(xx, xy, xz, yy, yz, zz);
+          if (hess) {  // NULL: mask only (the region pass recomputes the elements)
+            hess[idx] = cvt<TS>(xx);
+            hess[N + idx] = cvt<TS>(xy);
+            hess[2 * N + idx] = cvt<TS>(xz);
+            hess[3 * N + idx] = cvt<TS>(yy);
+            hess[4 * N + idx] = cvt<TS>(yz);
+            hess[5 * N + idx] = cvt<TS>(zz);
+          }
+          const uint8_t m =
+              BAND ? negdef_band(xx, xy, xz, yy, yz, zz) : negdef32(xx, xy, xz, yy, yz, zz);
+          pd[idx] = m;
+          is_cand = m != 0 && rec != nullptr;
+          if (is_cand) {
+            h6[0] = round_to<TS>(xx); h6[1] = round_to<TS>(xy); h6[2] = round_to<TS>(xz);
+            h6[3] = round_to<TS>(yy); h6[4] = round_to<TS>(yz); h6[5] = round_to<TS>(zz);
+            cidx = (uint32_t)idx;
+          }
+        }
+        // candidate records: one LDS reservation per wave (a wave is one row of 64 x),
+        // consecutive slots for its candidates in lane order
+        const unsigned long long bal = __ballot(is_cand);
+        if (bal) {
+          uint32_t base = 0;
+          if (lane == 0) base = atomicAdd(&s_cnt, (uint32_t)__popcll(bal));
+          base = __shfl(base, 0, 64);
+          if (is_cand) {
+            const uint32_t slot = base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+            rec[slot] = __uint_as_float(cidx);
+#pragma unroll
+            for (int q = 0; q < 6; ++q) rec[(size_t)kCandTile * (1 + q) + slot] = h6[q];
+          }
         }
       }
     }
@@ -191,26 +217,27 @@ __global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* 
     fetch(s + 2);
     __syncthreads();
   }
+  if (cand_cnt && threadIdx.x == 0) cand_cnt[blockIdx.x] = s_cnt;
 }
 
 }  // namespace
 
 template <typename TD, typename TH>
 hipError_t launch_hessian(const TD* dog, TH* hess, uint8_t* pd, int W, int H, int D,
-                          hipStream_t s) {
+                          hipStream_t s, float* cand, uint32_t* cand_cnt) {
   const int nbx = (W + TX - 1) / TX, nby = (H + TY - 1) / TY, nbz = (D + ZC - 1) / ZC;
   const i64 nblk = (i64)nbx * nby * nbz;
   // reduced-precision storage anywhere on the path: FP64 re-evaluation band for the minors
   constexpr bool band = !std::is_same<TD, float>::value || !std::is_same<TH, float>::value;
   hipLaunchKernelGGL((k_hessian<TD, TH, band>), dim3((unsigned)nblk), dim3(NT), 0, s, dog, hess,
-                     pd, W, H, D, nbx, nby);
+                     pd, W, H, D, nbx, nby, cand, cand_cnt);
   mark("hessian/hessian_pd", s);
   return hipGetLastError();
 }
 
 #define HDOG_INST_HESS(TD, TH)                                                          \
   template hipError_t launch_hessian<TD, TH>(const TD*, TH*, uint8_t*, int, int, int, \
-                                             hipStream_t);
+                                             hipStream_t, float*, uint32_t*);
 HDOG_INST_HESS(float, float)
 HDOG_INST_HESS(__half, __half)
 HDOG_INST_HESS(__hip_bfloat16, __hip_bfloat16)

@@ -395,12 +395,22 @@ hipError_t box_line(const float2* src, float2* dst, TS* norm, const uint16_t* im
   });
 }
 
+__global__ __launch_bounds__(NT) void k_split_ed(const float2* __restrict__ ed,
+                                                 float* __restrict__ ero, float* __restrict__ dil,
+                                                 i64 N) {
+  for (i64 i = (i64)blockIdx.x * NT + threadIdx.x; i < N; i += (i64)gridDim.x * NT) {
+    const float2 v = ed[i];
+    ero[i] = v.x;
+    dil[i] = v.y;
+  }
+}
+
 }  // namespace
 
 template <typename TS>
 hipError_t launch_normalize(const uint16_t* img, TS* norm, void* tmp1, void* tmp2,
                             float min_intensity, int W, int H, int D, int rxy, int rz,
-                            hipStream_t s) {
+                            hipStream_t s, float* ero, float* dil) {
   const i64 WH = (i64)W * H, N = WH * D;
   if (rxy == 0 || rz == 0) {
     const i64 nb = std::min<i64>((N + NT - 1) / NT, 1 << 20);
@@ -436,6 +446,15 @@ hipError_t launch_normalize(const uint16_t* img, TS* norm, void* tmp1, void* tmp
                                W, WH, D, rxy, s)) != hipSuccess)
     return e;
   mark("normalize/box_y", s);
+  if (ero && dil) {  // E and D on request: the same z pass without normalization, then split
+    if ((e = box_line<false, TS>((const float2*)tmp2, (float2*)tmp1, nullptr, nullptr, 0.0f, W, D,
+                                 WH, W, H, rz, s)) != hipSuccess)
+      return e;
+    const i64 nb = std::min<i64>((N + NT - 1) / NT, 1 << 20);
+    hipLaunchKernelGGL(k_split_ed, dim3((unsigned)nb), dim3(NT), 0, s, (const float2*)tmp1, ero,
+                       dil, N);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
   e = box_line<true, TS>((const float2*)tmp2, nullptr, norm, img, min_intensity, W, D, WH, W, H,
                          rz, s);
   mark("normalize/box_z_norm", s);
@@ -443,11 +462,11 @@ hipError_t launch_normalize(const uint16_t* img, TS* norm, void* tmp1, void* tmp
 }
 
 template hipError_t launch_normalize<float>(const uint16_t*, float*, void*, void*, float, int,
-                                            int, int, int, int, hipStream_t);
+                                            int, int, int, int, hipStream_t, float*, float*);
 template hipError_t launch_normalize<__half>(const uint16_t*, __half*, void*, void*, float, int,
-                                             int, int, int, int, hipStream_t);
+                                             int, int, int, int, hipStream_t, float*, float*);
 template hipError_t launch_normalize<__hip_bfloat16>(const uint16_t*, __hip_bfloat16*, void*,
                                                      void*, float, int, int, int, int, int,
-                                                     hipStream_t);
+                                                     hipStream_t, float*, float*);
 
 }  // namespace hdog

@@ -34,41 +34,87 @@ __device__ __forceinline__ int block_sum_int(int v, int* sh) {
   return t;
 }
 
-__global__ __launch_bounds__(NT) void k_count_roots(const uint32_t* __restrict__ L, i64 N,
-                                                    uint32_t* __restrict__ cnt) {
+// Per block of BLK voxels: the number of roots (L[i] == i, i > 0).  FLATTEN: also points
+// every other foreground voxel at its root (labels leave the union-find with parents that
+// are only ancestors).  A root is its own parent before and after, so counting and
+// flattening can share the pass.
+template <bool FLATTEN>
+__global__ __launch_bounds__(NT) void k_count_roots(uint32_t* L, i64 N, uint32_t* __restrict__ cnt) {
   __shared__ int sh[4];
   const i64 base = (i64)blockIdx.x * BLK;
-  int c = 0;
+  uint32_t v[ITEMS];
+#pragma unroll
   for (int k = 0; k < ITEMS; ++k) {
     const i64 i = base + (i64)k * NT + threadIdx.x;
-    if (i > 0 && i < N && L[i] == (uint32_t)i) ++c;
+    v[k] = i < N ? L[i] : 0u;
+  }
+  int c = 0;
+#pragma unroll
+  for (int k = 0; k < ITEMS; ++k) {
+    const i64 i = base + (i64)k * NT + threadIdx.x;
+    if (i <= 0 || i >= N || v[k] == 0u) continue;
+    if (v[k] == (uint32_t)i) {
+      ++c;
+    } else if (FLATTEN) {
+      const uint32_t r = find_root_readonly(L, v[k]);
+      if (r != v[k]) L[i] = r;
+    }
   }
   const int t = block_sum_int(c, sh);
   if (threadIdx.x == 0) cnt[blockIdx.x] = (uint32_t)t;
 }
 
-// exclusive scan of cnt[0..nb) in place by one work-group of 1024 threads; *d_R = 1 + total
+// Exclusive scan of cnt[0..nb) in place by one work-group of 1024 threads, 4096 entries
+// per step (4 consecutive per thread, coalesced), the next step's entries loaded before
+// the current step is scanned; *d_R = 1 + total.  Totals stay below 2^32 (N < 2^32).
 __global__ __launch_bounds__(1024) void k_scan_counts(uint32_t* cnt, i64 nb, uint32_t* d_R) {
-  __shared__ unsigned long long part[1024];
-  const i64 per = (nb + 1023) / 1024;
-  const i64 lo = threadIdx.x * per, hi = std::min<i64>(lo + per, nb);
-  unsigned long long s = 0;
-  for (i64 i = lo; i < hi; ++i) s += cnt[i];
-  part[threadIdx.x] = s;
-  __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    unsigned long long v = threadIdx.x >= o ? part[threadIdx.x - o] : 0;
+  constexpr int V = 4, TILE = 1024 * V;
+  __shared__ uint32_t wsum[16];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t carry = 0;
+  uint32_t cur[V], nxt[V];
+  auto load = [&](i64 base, uint32_t* dst) {
+    const i64 i0 = base + (i64)threadIdx.x * V;
+#pragma unroll
+    for (int j = 0; j < V; ++j) dst[j] = (i0 + j < nb) ? cnt[i0 + j] : 0u;
+  };
+  load(0, cur);
+  for (i64 base = 0; base < nb; base += TILE) {
+    if (base + TILE < nb) load(base + TILE, nxt);
+    uint32_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < V; ++j) sum += cur[j];
+    uint32_t x = sum;  // inclusive scan within the wave
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
     __syncthreads();
-    part[threadIdx.x] += v;
+    if (wave == 0) {
+      uint32_t w = lane < 16 ? wsum[lane] : 0u;
+#pragma unroll
+      for (int o = 1; o < 16; o <<= 1) {
+        const uint32_t y = __shfl_up(w, o, 64);
+        if (lane >= o) w += y;
+      }
+      if (lane < 16) wsum[lane] = w;
+    }
     __syncthreads();
+    uint32_t run = carry + (x - sum) + (wave ? wsum[wave - 1] : 0u);
+    const i64 i0 = base + (i64)threadIdx.x * V;
+#pragma unroll
+    for (int j = 0; j < V; ++j) {
+      if (i0 + j < nb) cnt[i0 + j] = run;
+      run += cur[j];
+    }
+    carry += wsum[15];
+    __syncthreads();  // wsum is rewritten by the next step
+#pragma unroll
+    for (int j = 0; j < V; ++j) cur[j] = nxt[j];
   }
-  unsigned long long run = threadIdx.x ? part[threadIdx.x - 1] : 0;
-  for (i64 i = lo; i < hi; ++i) {
-    const uint32_t c = cnt[i];
-    cnt[i] = (uint32_t)run;
-    run += c;
-  }
-  if (threadIdx.x == 1023) *d_R = (uint32_t)(1 + part[1023]);
+  if (threadIdx.x == 0) *d_R = 1u + carry;
 }
 
 // rid[root] = 1 + rank of the root in linear order; acc[rid].root = root
@@ -118,16 +164,9 @@ __device__ __forceinline__ float fdecode(uint32_t k) {
   return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
 }
 
-// A work-group takes a brick of 64 x BY x BZ voxels (one row of 64 x per wave and z
-// step).  All loads of the brick are issued up front; every candidate voxel then folds
-// itself into a per-work-group hash table in LDS keyed by region id (LDS atomics, FP64
-// for the Hessian sums), and each occupied slot finally issues one set of global
-// atomics, so a region pays one set per brick it touches instead of one per voxel.  A
-// voxel that finds the table full falls back to global atomics directly (128 slots: at
-// most 128 separate 26-connected regions fit in a 64 x 4 x 4 brick).
-constexpr int BY = 4, BZ = 4, HSB = 7, HS = 1 << HSB;
 constexpr uint32_t EMPTY = 0xFFFFFFFFu;
 
+// Partial sums of one region, then one set of global atomics per region and work-group.
 struct RunSums {
   uint32_t cnt, sx, sy, sz, isum, imax, nkey;
   double h[6], ns;
@@ -151,6 +190,83 @@ __device__ __forceinline__ void flush_global(RegionAcc* a, const RunSums& v, int
   }
 }
 
+// Per-work-group hash table in LDS keyed by region id (open addressing, LDS atomics, FP64
+// for the Hessian and normalized sums).  Each occupied slot ends as one set of global
+// atomics, so a region pays one set per work-group that touches it instead of one per
+// voxel; a voxel that finds the table full goes straight to the global atomics.  The
+// u32 partial sums cannot overflow for the <= 8192 voxels a work-group folds in.
+template <int S>
+struct LdsTable {
+  uint32_t key[S];
+  uint32_t cnt[S], sx[S], sy[S], sz[S], is[S], im[S], nk[S];
+  double h[6][S], ns[S];
+
+  __device__ void init() {
+    for (int t = threadIdx.x; t < S; t += blockDim.x) {
+      key[t] = EMPTY;
+      cnt[t] = sx[t] = sy[t] = sz[t] = is[t] = im[t] = nk[t] = 0u;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) h[k][t] = 0.0;
+      ns[t] = 0.0;
+    }
+  }
+
+  __device__ void add(RegionAcc* acc, uint32_t r, uint32_t x, uint32_t y, uint32_t z,
+                      const float* hv, float nv, uint32_t iv, int is_ave, bool has_img) {
+    uint32_t slot = (r * 2654435761u) >> (32 - __builtin_ctz(S));
+    bool placed = false;
+    for (int probes = 0; probes < S; ++probes) {
+      const uint32_t k0 = atomicCAS(&key[slot], EMPTY, r);
+      if (k0 == EMPTY || k0 == r) { placed = true; break; }
+      slot = (slot + 1) & (S - 1);
+    }
+    if (!placed) {
+      RunSums v;
+      v.cnt = 1u; v.sx = x; v.sy = y; v.sz = z;
+      v.isum = iv; v.imax = iv;
+      v.nkey = is_ave ? 0u : fkey(nv);
+      v.ns = is_ave ? (double)nv : 0.0;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) v.h[k] = (double)hv[k];
+      flush_global(acc + r, v, is_ave, has_img);
+      return;
+    }
+    atomicAdd(&cnt[slot], 1u);
+    atomicAdd(&sx[slot], x);
+    atomicAdd(&sy[slot], y);
+    atomicAdd(&sz[slot], z);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) unsafeAtomicAdd(&h[k][slot], (double)hv[k]);
+    if (is_ave)
+      unsafeAtomicAdd(&ns[slot], (double)nv);
+    else
+      atomicMax(&nk[slot], fkey(nv));
+    if (has_img) {
+      atomicAdd(&is[slot], iv);
+      atomicMax(&im[slot], iv);
+    }
+  }
+
+  __device__ void flush(RegionAcc* acc, int is_ave, bool has_img) {
+    for (int t = threadIdx.x; t < S; t += blockDim.x) {
+      const uint32_t rr = key[t];
+      if (rr == EMPTY) continue;
+      RunSums v;
+      v.cnt = cnt[t]; v.sx = sx[t]; v.sy = sy[t]; v.sz = sz[t];
+      v.isum = is[t]; v.imax = im[t]; v.nkey = nk[t]; v.ns = ns[t];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) v.h[k] = h[k][t];
+      flush_global(acc + rr, v, is_ave, has_img);
+    }
+  }
+};
+
+// Region sums from the materialized Hessian (the per-stage interface).  A work-group takes
+// a brick of 64 x BY x BZ voxels (one row of 64 x per wave and z step); all loads of the
+// brick are issued up front, then every candidate voxel folds itself into the table
+// (128 slots: at most 128 separate 26-connected regions fit in a 64 x 4 x 4 brick).
+constexpr int BY = 4, BZ = 4;
+
 template <typename TH, typename TN>
 __global__ __launch_bounds__(NT) void k_accum(const uint32_t* __restrict__ L,
                                               const uint32_t* __restrict__ rid,
@@ -159,16 +275,8 @@ __global__ __launch_bounds__(NT) void k_accum(const uint32_t* __restrict__ L,
                                               const uint16_t* __restrict__ img, RegionAcc* acc,
                                               uint32_t cap, int W, int H, int D, i64 N, int nbx,
                                               int nby, int is_ave) {
-  __shared__ uint32_t key[HS];
-  __shared__ uint32_t tcnt[HS], tsx[HS], tsy[HS], tsz[HS], tis[HS], tim[HS], tnk[HS];
-  __shared__ double th[6][HS], tns[HS];
-  for (int t = threadIdx.x; t < HS; t += NT) {
-    key[t] = EMPTY;
-    tcnt[t] = tsx[t] = tsy[t] = tsz[t] = tis[t] = tim[t] = tnk[t] = 0u;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) th[k][t] = 0.0;
-    tns[t] = 0.0;
-  }
+  __shared__ LdsTable<128> tab;
+  tab.init();
   unsigned b = blockIdx.x;
   const int bx = b % nbx; b /= nbx;
   const int by = b % nby; b /= nby;
@@ -177,7 +285,6 @@ __global__ __launch_bounds__(NT) void k_accum(const uint32_t* __restrict__ L,
   const int zb = (int)b * BZ;
   const i64 WH = (i64)W * H;
   const bool has_img = img != nullptr;
-  // loads of all BZ rows issued before any use: two dependent rounds in total
   uint32_t l[BZ], r[BZ];
 #pragma unroll
   for (int zz = 0; zz < BZ; ++zz) {
@@ -198,55 +305,47 @@ __global__ __launch_bounds__(NT) void k_accum(const uint32_t* __restrict__ L,
   }
   __syncthreads();  // table initialised
 #pragma unroll
-  for (int zz = 0; zz < BZ; ++zz) {
-    if (!l[zz] || r[zz] >= cap) continue;
-    const uint32_t rz = r[zz];
-    uint32_t slot = (rz * 2654435761u) >> (32 - HSB);
-    int probes = 0;
-    bool placed = false;
-    while (probes < HS) {
-      const uint32_t k0 = atomicCAS(&key[slot], EMPTY, rz);
-      if (k0 == EMPTY || k0 == rz) { placed = true; break; }
-      slot = (slot + 1) & (HS - 1);
-      ++probes;
-    }
-    if (!placed) {  // not reachable for this brick size (see above); kept for safety
-      RunSums v;
-      v.cnt = 1u; v.sx = (uint32_t)x; v.sy = (uint32_t)y; v.sz = (uint32_t)(zb + zz);
-      v.isum = iv[zz]; v.imax = iv[zz];
-      v.nkey = is_ave ? 0u : fkey(nv[zz]);
-      v.ns = is_ave ? (double)nv[zz] : 0.0;
+  for (int zz = 0; zz < BZ; ++zz)
+    if (l[zz] && r[zz] < cap)
+      tab.add(acc, r[zz], (uint32_t)x, (uint32_t)y, (uint32_t)(zb + zz), hv[zz], nv[zz], iv[zz],
+              is_ave, has_img);
+  __syncthreads();
+  tab.flush(acc, is_ave, has_img);
+}
+
+// Region sums from the candidate records of the Hessian kernel: one work-group per
+// Hessian tile folds that tile's records (voxel index + six elements, SoA) into the table,
+// gathering the label, normalized and raw intensity of each candidate voxel; no pass
+// over the non-candidate voxels.
+template <typename TN>
+__global__ __launch_bounds__(NT) void k_accum_cand(const float* __restrict__ cand,
+                                                   const uint32_t* __restrict__ cand_cnt,
+                                                   const uint32_t* __restrict__ L,
+                                                   const uint32_t* __restrict__ rid,
+                                                   const TN* __restrict__ norm,
+                                                   const uint16_t* __restrict__ img,
+                                                   RegionAcc* acc, uint32_t cap, uint32_t W,
+                                                   uint32_t WH, int is_ave) {
+  const uint32_t n = cand_cnt[blockIdx.x];
+  if (n == 0) return;  // uniform over the work-group
+  __shared__ LdsTable<256> tab;
+  tab.init();
+  const bool has_img = img != nullptr;
+  const float* t = cand + (size_t)blockIdx.x * kCandTile * 7;
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < n; i += NT) {
+    const uint32_t idx = __float_as_uint(t[i]);
+    float hv[6];
 #pragma unroll
-      for (int k = 0; k < 6; ++k) v.h[k] = (double)hv[zz][k];
-      flush_global(acc + rz, v, is_ave, has_img);
-      continue;
-    }
-    atomicAdd(&tcnt[slot], 1u);
-    atomicAdd(&tsx[slot], (uint32_t)x);
-    atomicAdd(&tsy[slot], (uint32_t)y);
-    atomicAdd(&tsz[slot], (uint32_t)(zb + zz));
-#pragma unroll
-    for (int k = 0; k < 6; ++k) unsafeAtomicAdd(&th[k][slot], (double)hv[zz][k]);
-    if (is_ave)
-      unsafeAtomicAdd(&tns[slot], (double)nv[zz]);
-    else
-      atomicMax(&tnk[slot], fkey(nv[zz]));
-    if (has_img) {
-      atomicAdd(&tis[slot], iv[zz]);
-      atomicMax(&tim[slot], iv[zz]);
-    }
+    for (int k = 0; k < 6; ++k) hv[k] = t[(size_t)kCandTile * (1 + k) + i];
+    const uint32_t r = rid[L[idx]];
+    const float nv = ld(norm, (i64)idx);
+    const uint32_t iv = has_img ? (uint32_t)img[idx] : 0u;
+    const uint32_t z = idx / WH, rem = idx - z * WH, y = rem / W, x = rem - y * W;
+    if (r < cap) tab.add(acc, r, x, y, z, hv, nv, iv, is_ave, has_img);
   }
   __syncthreads();
-  for (int t = threadIdx.x; t < HS; t += NT) {
-    const uint32_t rr = key[t];
-    if (rr == EMPTY) continue;
-    RunSums v;
-    v.cnt = tcnt[t]; v.sx = tsx[t]; v.sy = tsy[t]; v.sz = tsz[t];
-    v.isum = tis[t]; v.imax = tim[t]; v.nkey = tnk[t]; v.ns = tns[t];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) v.h[k] = th[k][t];
-    flush_global(acc + rr, v, is_ave, has_img);
-  }
+  tab.flush(acc, is_ave, has_img);
 }
 
 // src/3D/Eigenvalues.cu:32-94 (Eberly's closed form, FP64)
@@ -344,14 +443,22 @@ __global__ __launch_bounds__(NT) void k_fill_u64(u64* p, i64 n, u64 v) {
 
 }  // namespace
 
-hipError_t launch_region_ids(const uint32_t* labels, uint32_t* rid, uint32_t* block_cnt,
-                             uint32_t* d_R, i64 N, hipStream_t s, i64* nblocks_out) {
+hipError_t launch_region_ids(uint32_t* labels, uint32_t* rid, uint32_t* block_cnt,
+                             uint32_t* d_R, i64 N, hipStream_t s, i64* nblocks_out,
+                             bool flatten) {
+  (void)rid;
   const i64 nb = (N + BLK - 1) / BLK;
   *nblocks_out = nb;
-  hipLaunchKernelGGL(k_count_roots, dim3((unsigned)nb), dim3(NT), 0, s, labels, N, block_cnt);
+  if (flatten) {
+    hipLaunchKernelGGL(k_count_roots<true>, dim3((unsigned)nb), dim3(NT), 0, s, labels, N,
+                       block_cnt);
+  } else {
+    hipLaunchKernelGGL(k_count_roots<false>, dim3((unsigned)nb), dim3(NT), 0, s, labels, N,
+                       block_cnt);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  mark("regions/count_roots", s);
+  mark(flatten ? "regions/flatten_count_roots" : "regions/count_roots", s);
   hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, block_cnt, nb, d_R);
   mark("regions/scan", s);
   return hipGetLastError();
@@ -389,6 +496,18 @@ hipError_t launch_region_accum(const uint32_t* labels, const uint32_t* rid, cons
   return hipGetLastError();
 }
 
+template <typename TN>
+hipError_t launch_region_accum_cand(const float* cand, const uint32_t* cand_cnt, i64 ntiles,
+                                    const uint32_t* labels, const uint32_t* rid, const TN* norm,
+                                    const uint16_t* img, RegionAcc* acc, i64 cap, int W, int H,
+                                    int is_ave, hipStream_t s) {
+  hipLaunchKernelGGL((k_accum_cand<TN>), dim3((unsigned)ntiles), dim3(NT), 0, s, cand, cand_cnt,
+                     labels, rid, norm, img, acc, (uint32_t)std::min<i64>(cap, 0xFFFFFFFFll),
+                     (uint32_t)W, (uint32_t)((i64)W * H), is_ave);
+  mark("regions/accum_candidates", s);
+  return hipGetLastError();
+}
+
 #define HDOG_INST_ACC(TH, TN)                                                                  \
   template hipError_t launch_region_accum<TH, TN>(const uint32_t*, const uint32_t*, const TH*, \
                                                   const TN*, const uint16_t*, RegionAcc*, i64,  \
@@ -397,6 +516,15 @@ HDOG_INST_ACC(float, float)
 HDOG_INST_ACC(__half, __half)
 HDOG_INST_ACC(__hip_bfloat16, __hip_bfloat16)
 #undef HDOG_INST_ACC
+#define HDOG_INST_ACCC(TN)                                                                  \
+  template hipError_t launch_region_accum_cand<TN>(const float*, const uint32_t*, i64,      \
+                                                   const uint32_t*, const uint32_t*, const TN*, \
+                                                   const uint16_t*, RegionAcc*, i64, int, int, \
+                                                   int, hipStream_t);
+HDOG_INST_ACCC(float)
+HDOG_INST_ACCC(__half)
+HDOG_INST_ACCC(__hip_bfloat16)
+#undef HDOG_INST_ACCC
 
 hipError_t launch_region_finalize(const RegionAcc* acc, i64 R, int is_ave, RegionOut o, i64 pitch,
                                   hipStream_t s, const TileOut* t) {

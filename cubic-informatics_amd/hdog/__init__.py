@@ -105,6 +105,18 @@ class HDoG:
                                        float(min_intensity), W, H, D, int(radius_xy), int(radius_z)))
         return out
 
+    def Normalize3DFilterWithBounds(self, img, min_intensity, radius_xy, radius_z):
+        """Normalize3DFilter plus its erosion_l / dilation_l volumes (box means of the min and
+        max filters): returns (norm, E, D), float32 (D,H,W) each."""
+        img = self._dev(img, torch.uint16)
+        W, H, D = _shape(img)
+        norm, ero, dil = (torch.empty(img.shape, dtype=torch.float32, device=self.device)
+                          for _ in range(3))
+        self._check(C.hdog_normalize3d_ed(self._h, self._stream(), _ptr(img), _ptr(norm), _ptr(ero),
+                                          _ptr(dil), float(min_intensity), W, H, D, int(radius_xy),
+                                          int(radius_z)))
+        return norm, ero, dil
+
     def initGaussian3DKernel(self, sigma_xy1, sigma_z1, sigma_xy2, sigma_z2, truncate=2.0):
         """src/3D/DoG3DFilter.cu:394-415; returns the z radius of sigma2."""
         r = ctypes.c_int()

@@ -10,7 +10,8 @@ import os
 import torch  # noqa: F401  (loads the HIP runtime libhdog.so binds to)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libhdog.so")
+# HDOG_LIBRARY selects another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("HDOG_LIBRARY") or os.path.join(_HERE, "libhdog.so")
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(
@@ -63,6 +64,8 @@ lib.hdog_last_error.restype = ctypes.c_char_p
 hdog_last_error = lib.hdog_last_error
 hdog_release_scratch = _sig("hdog_release_scratch", [_vp])
 hdog_normalize3d = _sig("hdog_normalize3d", [_vp, _vp, _vp, _vp, _f, _i, _i, _i, _i, _i])
+hdog_normalize3d_ed = _sig("hdog_normalize3d_ed", [_vp, _vp, _vp, _vp, _vp, _vp, _f, _i, _i, _i,
+                                                   _i, _i])
 hdog_init_gaussian3d = _sig("hdog_init_gaussian3d", [_vp, _f, _f, _f, _f, _d, ctypes.POINTER(_i)])
 hdog_dog3d = _sig("hdog_dog3d", [_vp, _vp, _vp, _vp, _i, _i, _i, _f])
 hdog_hessian_pd = _sig("hdog_hessian_pd", [_vp, _vp, _vp, _vp, _vp, _i, _i, _i])
@@ -97,7 +100,7 @@ hdog_soft_backward = _sig("hdog_soft_backward", [_vp, _vp, _vp, _vp, _vp, _vp, _
                                                  ctypes.POINTER(_d)])
 
 EXPORTED = ["hdog_create", "hdog_destroy", "hdog_last_error", "hdog_release_scratch",
-            "hdog_normalize3d", "hdog_init_gaussian3d", "hdog_dog3d", "hdog_hessian_pd",
+            "hdog_normalize3d", "hdog_normalize3d_ed", "hdog_init_gaussian3d", "hdog_dog3d", "hdog_hessian_pd",
             "hdog_ccl3d", "hdog_regional_features", "hdog_regions_download",
             "hdog_process_substack", "hdog_run_device", "hdog_synth_volume",
             "hdog_set_profiling", "hdog_stage_times", "hdog_soft_forward", "hdog_soft_backward",

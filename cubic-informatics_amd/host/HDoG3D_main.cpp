@@ -113,7 +113,7 @@ int main(int argc, char** argv) {
   hp.truncate = p.truncate;
   hp.gamma = p.gamma_n;
   hp.is_ave_mode = p.is_ave_mode ? 1 : 0;
-  hp.precision = p.precision == "fp16" ? HDOG_FP16 : (p.precision == "bf16" ? HDOG_BF16 : HDOG_FP32);
+  hp.precision = precision_code(p.precision);
 
   const size_t plane = (size_t)p.width * p.height;
   std::vector<uint16_t> buf[2] = {std::vector<uint16_t>(plane * p.depth),

@@ -18,6 +18,7 @@
 #include <string>
 #include <vector>
 
+#include "hdog.h"
 #include "json.hpp"
 
 struct HostParams {
@@ -40,6 +41,16 @@ struct HostParams {
   std::vector<std::vector<std::string>> src_paths;
   std::vector<std::string> dst_paths;
 };
+
+// enum hdog_precision of an HDoG_param.precision string, -1 when unknown
+inline int precision_code(const std::string& s) {
+  if (s == "fp32") return HDOG_FP32;
+  if (s == "fp16") return HDOG_FP16;
+  if (s == "bf16") return HDOG_BF16;
+  if (s == "fp16_dog32") return HDOG_FP16_DOG32;
+  if (s == "bf16_dog32") return HDOG_BF16_DOG32;
+  return -1;
+}
 
 // Exit code 2 for every parameter-file problem (the reference exits 2 when the file
 // cannot be opened, src/3D/utils.h:96-100, and aborts on a missing key).
@@ -93,8 +104,8 @@ inline bool load_params(const std::string& fname, HostParams& p, std::string& wh
     return false;
   }
   if (p.width < 1 || p.height < 1 || p.depth < 1) { why = "non-positive image shape"; return false; }
-  if (p.precision != "fp32" && p.precision != "fp16" && p.precision != "bf16") {
-    why = "HDoG_param.precision must be fp32, fp16 or bf16";
+  if (precision_code(p.precision) < 0) {
+    why = "HDoG_param.precision must be fp32, fp16, bf16, fp16_dog32 or bf16_dog32";
     return false;
   }
   return true;

@@ -97,6 +97,14 @@ int hdog_normalize3d(hdog_ctx* ctx, void* stream, const uint16_t* d_img, float* 
                      float min_intensity, int width, int height, int depth,
                      int radius_xy, int radius_z);
 
+/* As hdog_normalize3d, and also writes the box means of the min filter (E) and of the
+ * max filter (D) that the normalization divides by: the erosion_l / dilation_l volumes
+ * of Normalize3DFilter (src/3D/NormalizeFilter.cu:603-654) that test/Normalize_test.cpp
+ * dumps (:105-125).  With a zero radius no filter runs and E = D = 0. */
+int hdog_normalize3d_ed(hdog_ctx* ctx, void* stream, const uint16_t* d_img, float* d_norm,
+                        float* d_erosion, float* d_dilation, float min_intensity, int width,
+                        int height, int depth, int radius_xy, int radius_z);
+
 /* Gaussian weights for sigma1 (xy, z) and sigma2 (xy, z); *r_z2 receives the z radius
  * of sigma2, which sets the substack depth margin.  Replaces initGaussian3DKernel,
  * src/3D/DoG3DFilter.cuh:5-9 (src/3D/DoG3DFilter.cu:372-415) with the truncation
@@ -150,8 +158,12 @@ int hdog_process_substack(hdog_ctx* ctx, void* stream, const hdog_params* p,
 
 /* Same pipeline on a volume already resident on the device.  The intermediates are
  * written to caller buffers when the pointers are non-NULL (else context scratch is
- * used); storage types follow p->precision (enum hdog_precision).  Region results stay on the device as for
- * hdog_regional_features. */
+ * used); storage types follow p->precision (enum hdog_precision).  With d_hessian NULL
+ * the six Hessian volumes are not materialized at all: the candidate pass writes the mask
+ * and a 28-byte record (voxel index, six elements) per candidate voxel, which the region
+ * pass reads instead of the volumes (identical sums; about 48 of the 80 bytes per voxel
+ * of FP32 traffic saved at a few percent candidates).  Region results stay on the device
+ * as for hdog_regional_features. */
 int hdog_run_device(hdog_ctx* ctx, void* stream, const hdog_params* p, const uint16_t* d_img,
                     void* d_norm, void* d_dog, void* d_hessian, uint8_t* d_pd,
                     uint32_t* d_labels, int64_t* num_regions);
