@@ -149,7 +149,7 @@ __global__ __launch_bounds__(NTC) void k_ccl_local(const uint8_t* __restrict__ p
                                                   int nbx, int nby) {
   constexpr uint32_t NONE = 0xFFFFFFFFu;
   __shared__ uint32_t lab[CV];
-  unsigned b = blockIdx.x;
+  unsigned b = xcd_tile(blockIdx.x, gridDim.x);
   const int bx = b % nbx; b /= nbx;
   const int by = b % nby; b /= nby;
   const int x0 = bx * CX, y0 = by * CY, z0 = (int)b * CZ;
@@ -240,10 +240,19 @@ __device__ __forceinline__ int rim_count(int ex, int ey, int ez) {
 }
 
 constexpr int NTM = 256;
+constexpr int KR = 4;  // rim voxels per thread per pass: a 32 x 8 x 8 tile has <= 788
 
+// Kernel 2.  Background voxels carry label 0 after kernel 1, so labels double as the
+// mask: round 1 loads the labels of up to KR rim voxels per thread, round 2 the labels of
+// their neighbours outside the clipped tile, all in flight together; then each distinct
+// neighbour label is linked with the voxel's own label by the global union-find (every
+// voxel of a tile-local component carries the same label, so few links remain).  Labels
+// read here may already have been linked by other work-groups; any value read is an
+// ancestor, so every link still joins the right sets.
 __global__ __launch_bounds__(NTM) void k_ccl_merge(const uint8_t* __restrict__ pd, uint32_t* L,
                                                    int W, int H, int D, int nbx, int nby) {
-  unsigned b = blockIdx.x;
+  (void)pd;
+  unsigned b = xcd_tile(blockIdx.x, gridDim.x);
   const int bx = b % nbx; b /= nbx;
   const int by = b % nby; b /= nby;
   const int x0 = bx * CX, y0 = by * CY, z0 = (int)b * CZ;
@@ -251,39 +260,45 @@ __global__ __launch_bounds__(NTM) void k_ccl_merge(const uint8_t* __restrict__ p
   const i64 WH = (i64)W * H, N = WH * D;
   const int F = rim_count(ex, ey, ez);
 #pragma unroll 1
-  for (int t = threadIdx.x; t < F; t += NTM) {
-    int lx, ly, lz;
-    rim_voxel(t, ex, ey, ez, lx, ly, lz);
-    const int gx = x0 + lx, gy = y0 + ly, gz = z0 + lz;
-    const i64 g = (i64)gz * WH + (i64)gy * W + gx;
-    if (g == 0 || !pd[g]) continue;
-    // neighbours linked here: those leaving the tile or the volume (wrap); their mask bytes
-    // are loaded together, then their labels, before any union
-    i64 nj[13];
-    uint8_t m[13];
+  for (int t0 = 0; t0 < F; t0 += KR * NTM) {
+    int lx[KR], ly[KR], lz[KR];
+    i64 g[KR];
+    uint32_t own[KR];
 #pragma unroll
-    for (int q = 0; q < 13; ++q) {
-      int dx, dy, dz;
-      fwd_offset(q, dx, dy, dz);
-      const int nx = lx + dx, ny = ly + dy, nz = lz + dz;
-      const bool local = nx >= 0 && nx < ex && ny >= 0 && ny < ey && nz < ez;
-      nj[q] = g + (i64)dz * WH + (i64)dy * W + dx;  // linear offset, wraps like the reference
-      m[q] = (!local && nj[q] < N) ? pd[nj[q]] : (uint8_t)0;
+    for (int k = 0; k < KR; ++k) {
+      const int t = t0 + threadIdx.x + k * NTM;
+      own[k] = 0u;
+      g[k] = 0;
+      lx[k] = ly[k] = lz[k] = 0;
+      if (t < F) {
+        rim_voxel(t, ex, ey, ez, lx[k], ly[k], lz[k]);
+        g[k] = (i64)(z0 + lz[k]) * WH + (i64)(y0 + ly[k]) * W + (x0 + lx[k]);
+        own[k] = L[g[k]];  // 0: background (voxel 0 is always background)
+      }
     }
-    uint32_t lj[13];
+    uint32_t lj[KR][13];
 #pragma unroll
-    for (int q = 0; q < 13; ++q) lj[q] = m[q] ? L[nj[q]] : 0u;
-    // the outside neighbours mostly belong to one or two tile-local components: link each
-    // distinct neighbour label once (labels are tile-local roots after kernel 1)
-    uint32_t root = L[g];
-    uint32_t seen0 = root, seen1 = 0u, seen2 = 0u;
+    for (int k = 0; k < KR; ++k)
 #pragma unroll
-    for (int q = 0; q < 13; ++q) {
-      const uint32_t l = lj[q];
-      if (!m[q] || l == seen0 || l == seen1 || l == seen2 || l == root) continue;
-      root = unite(L, root, l);
-      seen2 = seen1;
-      seen1 = l;
+      for (int q = 0; q < 13; ++q) {
+        int dx, dy, dz;
+        fwd_offset(q, dx, dy, dz);
+        const int nx = lx[k] + dx, ny = ly[k] + dy, nz = lz[k] + dz;
+        const bool local = nx >= 0 && nx < ex && ny >= 0 && ny < ey && nz < ez;
+        const i64 nj = g[k] + (i64)dz * WH + (i64)dy * W + dx;  // wraps like the reference
+        lj[k][q] = (own[k] && !local && nj < N) ? L[nj] : 0u;
+      }
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+      uint32_t seen1 = 0u, seen2 = 0u;
+#pragma unroll
+      for (int q = 0; q < 13; ++q) {
+        const uint32_t l = lj[k][q];
+        if (!l || l == own[k] || l == seen1 || l == seen2) continue;
+        unite(L, own[k], l);
+        seen2 = seen1;
+        seen1 = l;
+      }
     }
   }
 }

@@ -30,8 +30,9 @@ __global__ __launch_bounds__(NT) void k_gauss_x(const TS* __restrict__ src, floa
   extern __shared__ float smf[];
   const int R = max(r1, r2);
   const int span = C + 2 * R;
-  const int ct = blockIdx.x % nct;
-  const i64 row0 = (i64)(blockIdx.x / nct) * ROWS;
+  const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int ct = tile % nct;
+  const i64 row0 = (i64)(tile / nct) * ROWS;
   const int c0 = ct * C;
   stage_tile<8, NT, float>(
       ROWS * span,
@@ -109,8 +110,9 @@ __global__ __launch_bounds__(NT) void k_gauss_x_r(const TS* __restrict__ src,
   constexpr int SPAN = C + 2 * RP;
   constexpr int NV = (RUN + 2 * RP) / 4;  // float4 reads per window
   __shared__ __align__(16) float T[ROWS * SPAN];
-  const int ct = blockIdx.x % nct;
-  const i64 row0 = (i64)(blockIdx.x / nct) * ROWS;
+  const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int ct = tile % nct;
+  const i64 row0 = (i64)(tile / nct) * ROWS;
   const int c0 = ct * C;
   stage_tile<8, NT, float>(
       ROWS * SPAN,

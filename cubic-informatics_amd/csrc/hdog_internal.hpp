@@ -92,6 +92,17 @@ inline void mark(const char* name, hipStream_t s) {
   }
 }
 
+// Work-group b of n runs on XCD b % 8 (the dispatcher deals work-groups to the 8 XCDs in
+// turn, each XCD with its own L2).  Returns the logical tile for b such that every XCD
+// walks one contiguous range of tiles in order: tiles that share cache lines (halo rows,
+// 128-byte lines split between narrow tiles) then meet in the same L2.  A bijection of
+// [0, n) for any n.
+__device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned n) {
+  constexpr unsigned X = 8;
+  const unsigned q = n / X, r = n % X, j = b % X;
+  return j * q + min(j, r) + b / X;
+}
+
 // One zero-padded central difference ((0 + hi) - lo) * 0.5 in the reference's order; the
 // Hessian kernel and the region pass that recomputes the Hessian share it.
 __device__ __forceinline__ float cdiff_half(float hi, float lo) {

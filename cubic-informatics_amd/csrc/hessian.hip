@@ -72,9 +72,10 @@ __global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* 
   __shared__ float sHz[3][TY * TX];
   __shared__ uint32_t s_cnt;  // candidate records written by this work-group
   if (threadIdx.x == 0) s_cnt = 0;
-  float* rec = cand ? cand + (size_t)blockIdx.x * kCandTile * 7 : nullptr;
+  const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+  float* rec = cand ? cand + (size_t)tile * kCandTile * 7 : nullptr;
   const int lane = threadIdx.x & 63;
-  unsigned b = blockIdx.x;
+  unsigned b = tile;
   const int bx = b % nbx; b /= nbx;
   const int by = b % nby; b /= nby;
   const int x0 = bx * TX, y0 = by * TY;
@@ -217,7 +218,7 @@ __global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* 
     fetch(s + 2);
     __syncthreads();
   }
-  if (cand_cnt && threadIdx.x == 0) cand_cnt[blockIdx.x] = s_cnt;
+  if (cand_cnt && threadIdx.x == 0) cand_cnt[tile] = s_cnt;
 }
 
 }  // namespace

@@ -108,8 +108,9 @@ __global__ __launch_bounds__(NT) void k_minmax_x_box(const uint32_t* __restrict_
   const int spanmax = C + 4 * r;
   uint32_t* A = smem;
   uint32_t* P = smem + ROWS * spanmax;
-  const int ct = blockIdx.x % nct;
-  const i64 row0 = (i64)(blockIdx.x / nct) * ROWS;
+  const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int ct = tile % nct;
+  const i64 row0 = (i64)(tile / nct) * ROWS;
   const int c0 = ct * C;
   const int qa = max(0, c0 - r), qb = min(W, c0 + C + r);
   const int nq = qb - qa;

@@ -277,7 +277,7 @@ __global__ __launch_bounds__(NT) void k_accum(const uint32_t* __restrict__ L,
                                               int nby, int is_ave) {
   __shared__ LdsTable<128> tab;
   tab.init();
-  unsigned b = blockIdx.x;
+  unsigned b = xcd_tile(blockIdx.x, gridDim.x);
   const int bx = b % nbx; b /= nbx;
   const int by = b % nby; b /= nby;
   const int x = bx * 64 + (threadIdx.x & 63);
@@ -326,12 +326,13 @@ __global__ __launch_bounds__(NT) void k_accum_cand(const float* __restrict__ can
                                                    const uint16_t* __restrict__ img,
                                                    RegionAcc* acc, uint32_t cap, uint32_t W,
                                                    uint32_t WH, int is_ave) {
-  const uint32_t n = cand_cnt[blockIdx.x];
+  const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t n = cand_cnt[tile];
   if (n == 0) return;  // uniform over the work-group
   __shared__ LdsTable<256> tab;
   tab.init();
   const bool has_img = img != nullptr;
-  const float* t = cand + (size_t)blockIdx.x * kCandTile * 7;
+  const float* t = cand + (size_t)tile * kCandTile * 7;
   __syncthreads();
   for (uint32_t i = threadIdx.x; i < n; i += NT) {
     const uint32_t idx = __float_as_uint(t[i]);
