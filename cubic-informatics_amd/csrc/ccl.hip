@@ -94,15 +94,8 @@ __global__ __launch_bounds__(NT) void k_ccl_flatten(uint32_t* L, i64 N) {
 // links, with the global union-find, only the pairs that leave the tile or wrap.  For
 // width, height >= 2 the 13 forward neighbours (dz = 1; dz = 0, dy = 1; dz = dy = 0,
 // dx = 1) are exactly the positive linear offsets of the reference's adjacency.
-#ifndef HDOG_CCL_CY
-#define HDOG_CCL_CY 8
-#endif
-#ifndef HDOG_CCL_CZ
-#define HDOG_CCL_CZ 8
-#endif
-constexpr int CX = 32, CY = HDOG_CCL_CY, CZ = HDOG_CCL_CZ, CV = CX * CY * CZ;
-constexpr int NTC = CV >= 4096 ? 512 : 256, CPT = CV / NTC;
-static_assert(CV % NTC == 0 && (CY & (CY - 1)) == 0 && (CZ & (CZ - 1)) == 0, "tile shape");
+constexpr int CX = 32, CY = 8, CZ = 8, CV = CX * CY * CZ, NTC = 256, CPT = CV / NTC;
+static_assert(CV % NTC == 0, "tile shape");
 
 __device__ __forceinline__ uint32_t lds_find(const uint32_t* lab, uint32_t x) {
   uint32_t p = lab[x];
@@ -144,43 +137,63 @@ __device__ __forceinline__ void fwd_offset(int k, int& dx, int& dy, int& dz) {
   else { dz = 0; dy = 0; dx = 1; }
 }
 
+// Kernel 1.  With 256 threads on a 32 x 8 x 8 tile, thread t owns the column (t % 32,
+// t / 32) of the tile and k = 0..7 walks its z; every index is 32-bit (N < 2^32).  The
+// foreground voxels are compacted into an LDS list (wave ballots), so the union phase runs
+// one dense pass over the few foreground voxels instead of eight divergent passes over all.
+static_assert(NTC == CX * CY && CPT == CZ, "k_ccl_local maps one thread to one z column");
+
 __global__ __launch_bounds__(NTC) void k_ccl_local(const uint8_t* __restrict__ pd,
                                                   uint32_t* __restrict__ L, int W, int H, int D,
                                                   int nbx, int nby) {
   constexpr uint32_t NONE = 0xFFFFFFFFu;
   __shared__ uint32_t lab[CV];
+  __shared__ uint16_t list[CV];
+  __shared__ uint32_t nfg;
+  if (threadIdx.x == 0) nfg = 0;
   unsigned b = xcd_tile(blockIdx.x, gridDim.x);
   const int bx = b % nbx; b /= nbx;
   const int by = b % nby; b /= nby;
   const int x0 = bx * CX, y0 = by * CY, z0 = (int)b * CZ;
-  const i64 WH = (i64)W * H;
+  const uint32_t WH = (uint32_t)W * (uint32_t)H;
+  const int lx = threadIdx.x & (CX - 1), ly = threadIdx.x / CX;
+  const int gx = x0 + lx, gy = y0 + ly;
+  const bool col = gx < W && gy < H;
+  const uint32_t g0 = (uint32_t)z0 * WH + (uint32_t)gy * (uint32_t)W + (uint32_t)gx;
+  const int lane = threadIdx.x & 63;
   uint8_t fg[CPT];
 #pragma unroll
   for (int k = 0; k < CPT; ++k) {
-    const int v = threadIdx.x + k * NTC;
-    const int gx = x0 + (v & (CX - 1)), gy = y0 + ((v / CX) & (CY - 1)), gz = z0 + v / (CX * CY);
-    const i64 g = (i64)gz * WH + (i64)gy * W + gx;
-    fg[k] = (gx < W && gy < H && gz < D && g > 0) ? pd[g] : 0;
+    const uint32_t g = g0 + (uint32_t)k * WH;
+    fg[k] = (col && z0 + k < D && g > 0) ? pd[g] : (uint8_t)0;
   }
+  __syncthreads();  // nfg initialised
 #pragma unroll
   for (int k = 0; k < CPT; ++k) {
     const int v = threadIdx.x + k * NTC;
     lab[v] = fg[k] ? (uint32_t)v : NONE;
+    const unsigned long long bal = __ballot(fg[k] != 0);
+    if (bal) {
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(&nfg, (uint32_t)__popcll(bal));
+      base = __shfl(base, 0, 64);
+      if (fg[k]) list[base + __popcll(bal & ((1ull << lane) - 1ull))] = (uint16_t)v;
+    }
   }
   __syncthreads();
+  const int n = (int)nfg;
 #pragma unroll 1
-  for (int k = 0; k < CPT; ++k) {
-    if (!fg[k]) continue;
-    const int v = threadIdx.x + k * NTC;
-    const int lx = v & (CX - 1), ly = (v / CX) & (CY - 1), lz = v / (CX * CY);
+  for (int i = threadIdx.x; i < n; i += NTC) {
+    const int v = list[i];
+    const int vx = v & (CX - 1), vy = (v / CX) & (CY - 1), vz = v / (CX * CY);
     uint32_t root = (uint32_t)v;  // a representative of v's set, refreshed by each link
 #pragma unroll
     for (int q = 0; q < 13; ++q) {
       int dx, dy, dz;
       fwd_offset(q, dx, dy, dz);
-      const int nx = lx + dx, ny = ly + dy, nz = lz + dz;
+      const int nx = vx + dx, ny = vy + dy, nz = vz + dz;
       if (nx < 0 || nx >= CX || ny < 0 || ny >= CY || nz >= CZ) continue;
-      const int u = nx + CX * (ny + CY * nz);
+      const int u = v + dx + CX * (dy + CY * dz);
       const uint32_t lu = lab[u];
       // outside-volume and background positions hold NONE; a parent equal to our
       // representative means u is already in our set
@@ -188,20 +201,18 @@ __global__ __launch_bounds__(NTC) void k_ccl_local(const uint8_t* __restrict__ p
     }
   }
   __syncthreads();
+  if (!col) return;
 #pragma unroll
   for (int k = 0; k < CPT; ++k) {
-    const int v = threadIdx.x + k * NTC;
-    const int gx = x0 + (v & (CX - 1)), gy = y0 + ((v / CX) & (CY - 1)), gz = z0 + v / (CX * CY);
-    if (gx >= W || gy >= H || gz >= D) continue;
-    const i64 g = (i64)gz * WH + (i64)gy * W + gx;
+    if (z0 + k >= D) break;
     uint32_t out = 0u;
     if (fg[k]) {
-      const uint32_t r = lds_find(lab, (uint32_t)v);
-      const int rx = x0 + (int)(r & (CX - 1)), ry = y0 + (int)((r / CX) & (CY - 1)),
-                rz = z0 + (int)(r / (CX * CY));
-      out = (uint32_t)((i64)rz * WH + (i64)ry * W + rx);
+      const uint32_t r = lds_find(lab, (uint32_t)(threadIdx.x + k * NTC));
+      out = (uint32_t)(z0 + (int)(r / (CX * CY))) * WH +
+            (uint32_t)(y0 + (int)((r / CX) & (CY - 1))) * (uint32_t)W + (uint32_t)x0 +
+            (r & (CX - 1));
     }
-    L[g] = out;
+    L[g0 + (uint32_t)k * WH] = out;
   }
 }
 
@@ -249,45 +260,44 @@ constexpr int KR = 4;  // rim voxels per thread per pass: a 32 x 8 x 8 tile has 
 // voxel of a tile-local component carries the same label, so few links remain).  Labels
 // read here may already have been linked by other work-groups; any value read is an
 // ancestor, so every link still joins the right sets.
-__global__ __launch_bounds__(NTM) void k_ccl_merge(const uint8_t* __restrict__ pd, uint32_t* L,
-                                                   int W, int H, int D, int nbx, int nby) {
-  (void)pd;
-  unsigned b = xcd_tile(blockIdx.x, gridDim.x);
-  const int bx = b % nbx; b /= nbx;
-  const int by = b % nby; b /= nby;
-  const int x0 = bx * CX, y0 = by * CY, z0 = (int)b * CZ;
-  const int ex = min(CX, W - x0), ey = min(CY, H - y0), ez = min(CZ, D - z0);
-  const i64 WH = (i64)W * H, N = WH * D;
+template <bool FULL>
+__device__ __forceinline__ void merge_tile(uint32_t* L, int W, uint32_t WH, uint32_t N, int x0,
+                                           int y0, int z0, int ex, int ey, int ez) {
+  if (FULL) { ex = CX; ey = CY; ez = CZ; }  // compile-time shape: the rim math folds
   const int F = rim_count(ex, ey, ez);
 #pragma unroll 1
   for (int t0 = 0; t0 < F; t0 += KR * NTM) {
     int lx[KR], ly[KR], lz[KR];
-    i64 g[KR];
-    uint32_t own[KR];
+    uint32_t g[KR], own[KR];
 #pragma unroll
     for (int k = 0; k < KR; ++k) {
       const int t = t0 + threadIdx.x + k * NTM;
       own[k] = 0u;
-      g[k] = 0;
+      g[k] = 0u;
       lx[k] = ly[k] = lz[k] = 0;
       if (t < F) {
         rim_voxel(t, ex, ey, ez, lx[k], ly[k], lz[k]);
-        g[k] = (i64)(z0 + lz[k]) * WH + (i64)(y0 + ly[k]) * W + (x0 + lx[k]);
+        g[k] = (uint32_t)(z0 + lz[k]) * WH + (uint32_t)(y0 + ly[k]) * (uint32_t)W +
+               (uint32_t)(x0 + lx[k]);
         own[k] = L[g[k]];  // 0: background (voxel 0 is always background)
       }
     }
     uint32_t lj[KR][13];
 #pragma unroll
-    for (int k = 0; k < KR; ++k)
+    for (int q = 0; q < 13; ++q) {
+      int dx, dy, dz;
+      fwd_offset(q, dx, dy, dz);
+      // forward linear offset (positive for W, H >= 2); wraps like the reference
+      const i64 off64 = (i64)dz * WH + (i64)dy * W + dx;
+      const uint32_t off = (uint32_t)off64;
+      const bool fits = off64 < (i64)N;
 #pragma unroll
-      for (int q = 0; q < 13; ++q) {
-        int dx, dy, dz;
-        fwd_offset(q, dx, dy, dz);
+      for (int k = 0; k < KR; ++k) {
         const int nx = lx[k] + dx, ny = ly[k] + dy, nz = lz[k] + dz;
         const bool local = nx >= 0 && nx < ex && ny >= 0 && ny < ey && nz < ez;
-        const i64 nj = g[k] + (i64)dz * WH + (i64)dy * W + dx;  // wraps like the reference
-        lj[k][q] = (own[k] && !local && nj < N) ? L[nj] : 0u;
+        lj[k][q] = (own[k] && !local && fits && g[k] < N - off) ? L[g[k] + off] : 0u;
       }
+    }
 #pragma unroll
     for (int k = 0; k < KR; ++k) {
       uint32_t seen1 = 0u, seen2 = 0u;
@@ -301,6 +311,20 @@ __global__ __launch_bounds__(NTM) void k_ccl_merge(const uint8_t* __restrict__ p
       }
     }
   }
+}
+
+__global__ __launch_bounds__(NTM) void k_ccl_merge(uint32_t* L, int W, int H, int D, int nbx,
+                                                   int nby) {
+  unsigned b = xcd_tile(blockIdx.x, gridDim.x);
+  const int bx = b % nbx; b /= nbx;
+  const int by = b % nby; b /= nby;
+  const int x0 = bx * CX, y0 = by * CY, z0 = (int)b * CZ;
+  const int ex = min(CX, W - x0), ey = min(CY, H - y0), ez = min(CZ, D - z0);
+  const uint32_t WH = (uint32_t)W * (uint32_t)H, N = WH * (uint32_t)D;
+  if (ex == CX && ey == CY && ez == CZ)
+    merge_tile<true>(L, W, WH, N, x0, y0, z0, ex, ey, ez);
+  else
+    merge_tile<false>(L, W, WH, N, x0, y0, z0, ex, ey, ez);
 }
 
 }  // namespace
@@ -334,8 +358,7 @@ hipError_t launch_ccl(const uint8_t* pd, uint32_t* labels, int W, int H, int D, 
                        nby);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     mark("ccl/local", s);
-    hipLaunchKernelGGL(k_ccl_merge, dim3((unsigned)nt), dim3(NTM), 0, s, pd, labels, W, H, D, nbx,
-                       nby);
+    hipLaunchKernelGGL(k_ccl_merge, dim3((unsigned)nt), dim3(NTM), 0, s, labels, W, H, D, nbx, nby);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     mark("ccl/merge", s);
   } else {

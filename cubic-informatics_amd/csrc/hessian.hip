@@ -51,13 +51,16 @@ __device__ __forceinline__ uint8_t negdef_band(float xx, float xy, float xz, flo
 }
 
 // z-streaming form.  A work-group owns a TX x TY column of outputs over ZC consecutive
-// slices and walks s = z0-2 .. z1+1 with three-slot LDS rings:
+// slices z0..z1-1 and walks s = z0 .. z1+1 with three-slot LDS rings:
 //   step s: Hx(s-1), Hy(s-1) from dog(s-1) (xy halo 1), Hz(s-1) from dog(s) - dog(s-2)
 //           ->  outputs of slice z = s-2, which need Hx / Hy at z-1..z+1 and Hz at
-//           z-1, z+1  ->  dog(s+1) (xy halo 2) into the ring, fetched one step earlier.
+//           z-1, z+1  ->  dog(s+1) (xy halo 2) into the slot of dog(s-2), and the loads of
+//           dog(s+2) issued into registers.
 // Every dog value is read once per column (plus the xy halo, served by L2) instead of
 // (TZ+4)/TZ times; first derivatives outside the volume are zero as in the reference's
-// second pass.
+// second pass.  Each thread's positions in the slices do not change with z, so their
+// LDS and global offsets and their bounds tests are computed once, before the walk; ring
+// slots rotate instead of being recomputed; global indices are 32-bit (N < 2^32).
 template <typename TD, typename TS, bool BAND>
 __global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* __restrict__ hess,
                                                 uint8_t* __restrict__ pd, int W, int H, int D,
@@ -66,6 +69,9 @@ __global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* 
   constexpr int DX = TX + 4, DY = TY + 4;        // dog slice with halo 2
   constexpr int XX = TX + 2, XY = TY + 2;        // Hx slice with halo 1 in x and y
   constexpr int YX = TX, YY = TY + 2;            // Hy slice with halo 1 in y
+  constexpr int NPF = (DY * DX + NT - 1) / NT;   // dog elements per thread and slice
+  constexpr int NHX = (XY * XX + NT - 1) / NT, NHY = (YY * YX + NT - 1) / NT;
+  constexpr int NHZ = TY * TX / NT, NOUT = TY / (NT / TX);
   __shared__ float sD[3][DY * DX];
   __shared__ float sHx[3][XY * XX];
   __shared__ float sHy[3][YY * YX];
@@ -80,96 +86,118 @@ __global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* 
   const int by = b % nby; b /= nby;
   const int x0 = bx * TX, y0 = by * TY;
   const int z0 = (int)b * ZC, z1 = min(z0 + ZC, D);
-  const i64 WH = (i64)W * H;
-  const i64 N = WH * D;
+  const uint32_t Wu = (uint32_t)W, WH = Wu * (uint32_t)H;
+  auto inplane = [&](int gx, int gy) { return gx >= 0 && gx < W && gy >= 0 && gy < H; };
+
+  // slice-invariant addressing of this thread
+  uint32_t fo[NPF];
+  bool fv[NPF];
+#pragma unroll
+  for (int k = 0; k < NPF; ++k) {
+    const int e = threadIdx.x + k * NT, lx = e % DX, ly = e / DX;
+    const int gx = x0 - 2 + lx, gy = y0 - 2 + ly;
+    fv[k] = e < DY * DX && inplane(gx, gy);
+    fo[k] = fv[k] ? (uint32_t)gy * Wu + (uint32_t)gx : 0u;
+  }
+  int hxi[NHX], hyi[NHY], hzi[NHZ];
+  bool hxe[NHX], hxv[NHX], hye[NHY], hyv[NHY], hzv[NHZ];
+#pragma unroll
+  for (int j = 0; j < NHX; ++j) {
+    const int e = threadIdx.x + j * NT, lx = e % XX, ly = e / XX;
+    hxe[j] = e < XY * XX;
+    hxv[j] = hxe[j] && inplane(x0 - 1 + lx, y0 - 1 + ly);
+    hxi[j] = (ly + 1) * DX + lx;  // Hx = g2(d[hxi + 2], d[hxi])
+  }
+#pragma unroll
+  for (int j = 0; j < NHY; ++j) {
+    const int e = threadIdx.x + j * NT, lx = e % YX, ly = e / YX;
+    hye[j] = e < YY * YX;
+    hyv[j] = hye[j] && inplane(x0 + lx, y0 - 1 + ly);
+    hyi[j] = ly * DX + lx + 2;  // Hy = g2(d[hyi + 2 DX], d[hyi])
+  }
+#pragma unroll
+  for (int j = 0; j < NHZ; ++j) {
+    const int e = threadIdx.x + j * NT, lx = e % TX, ly = e / TX;
+    hzv[j] = inplane(x0 + lx, y0 + ly);
+    hzi[j] = (ly + 2) * DX + lx + 2;  // Hz = g2(dn[hzi], dp[hzi])
+  }
   const int tx = threadIdx.x % TX, ty0 = threadIdx.x / TX;  // outputs (tx, ty0 + k*(NT/TX))
-  // dog slices go global -> registers one step ahead -> LDS ring, so the loads of slice
-  // s+2 are in flight while slice s is processed
-  constexpr int NPF = (DY * DX + NT - 1) / NT;
+  bool ov[NOUT];
+  uint32_t po[NOUT];
+#pragma unroll
+  for (int k = 0; k < NOUT; ++k) {
+    const int gx = x0 + tx, gy = y0 + ty0 + k * (NT / TX);
+    ov[k] = gx < W && gy < H;
+    po[k] = ov[k] ? (uint32_t)gy * Wu + (uint32_t)gx : 0u;
+  }
+
+  // dog slices go global -> registers one step ahead -> LDS ring
   float pf[NPF];
   auto fetch = [&](int sl) {
     const bool zin = sl >= 0 && sl < D;
+    const uint32_t base = zin ? (uint32_t)sl * WH : 0u;
 #pragma unroll
-    for (int k = 0; k < NPF; ++k) {
-      const int e = threadIdx.x + k * NT;
-      const int lx = e % DX, ly = e / DX;
-      const int gx = x0 - 2 + lx, gy = y0 - 2 + ly;
-      pf[k] = (e < DY * DX && zin && gx >= 0 && gx < W && gy >= 0 && gy < H)
-                  ? ld(dog, (i64)sl * WH + (i64)gy * W + gx)
-                  : 0.0f;
-    }
+    for (int k = 0; k < NPF; ++k) pf[k] = (zin && fv[k]) ? ld(dog, (i64)(base + fo[k])) : 0.0f;
   };
-  auto put = [&](int sl) {
-    float* d = sD[(sl + 3) % 3];
+  auto put = [&](float* d) {
 #pragma unroll
     for (int k = 0; k < NPF; ++k) {
       const int e = threadIdx.x + k * NT;
       if (e < DY * DX) d[e] = pf[k];
     }
   };
+  int dp = 0, dc = 1, dn = 2;   // dog slots of slices s-2, s-1, s
+  int h1 = 0, h2 = 1, hw = 2;   // H slots of slices s-3, s-2 and the one written at step s
   fetch(z0 - 2);
-  put(z0 - 2);
+  put(sD[dp]);
   fetch(z0 - 1);
+  put(sD[dc]);
+  fetch(z0);
+  put(sD[dn]);
+  fetch(z0 + 1);
   __syncthreads();
-  for (int s = z0 - 2; s <= z1 + 1; ++s) {
-    // invariant: dog(s-2..s) are in LDS, pf holds dog(s+1)
-    // 1. first derivatives of slice s-1 (Hz from dog(s) - dog(s-2))
+  for (int s = z0; s <= z1 + 1; ++s) {
+    // 1. first derivatives of slice s-1
     {
-      const int zc = s - 1;
-      const bool zin = zc >= 0 && zc < D;
-      const float* dc = sD[(zc + 3) % 3];  // dog(s-1)
-      const float* dn = sD[(s + 3) % 3];   // dog(s)
-      const float* dp = sD[(s + 1) % 3];   // dog(s-2)
-      float* hx = sHx[(zc + 3) % 3];
-      float* hy = sHy[(zc + 3) % 3];
-      float* hz = sHz[(zc + 3) % 3];
-      for (int e = threadIdx.x; e < XY * XX; e += NT) {
-        const int lx = e % XX, ly = e / XX;
-        const int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
-        float v = 0.0f;
-        if (zin && gx >= 0 && gx < W && gy >= 0 && gy < H)
-          v = g2(dc[(ly + 1) * DX + lx + 2], dc[(ly + 1) * DX + lx]);
-        hx[e] = v;
-      }
-      for (int e = threadIdx.x; e < YY * YX; e += NT) {
-        const int lx = e % YX, ly = e / YX;
-        const int gx = x0 + lx, gy = y0 - 1 + ly;
-        float v = 0.0f;
-        if (zin && gx < W && gy >= 0 && gy < H)
-          v = g2(dc[(ly + 2) * DX + lx + 2], dc[ly * DX + lx + 2]);
-        hy[e] = v;
-      }
-      for (int e = threadIdx.x; e < TY * TX; e += NT) {
-        const int lx = e % TX, ly = e / TX;
-        const int gx = x0 + lx, gy = y0 + ly;
-        float v = 0.0f;
-        if (zin && gx < W && gy < H)
-          v = g2(dn[(ly + 2) * DX + lx + 2], dp[(ly + 2) * DX + lx + 2]);
-        hz[e] = v;
-      }
+      const bool zin = s - 1 >= 0 && s - 1 < D;
+      const float* Dc = sD[dc];
+      const float* Dn = sD[dn];
+      const float* Dp = sD[dp];
+      float* hx = sHx[hw];
+      float* hy = sHy[hw];
+      float* hz = sHz[hw];
+#pragma unroll
+      for (int j = 0; j < NHX; ++j)
+        if (hxe[j])
+          hx[threadIdx.x + j * NT] = (zin && hxv[j]) ? g2(Dc[hxi[j] + 2], Dc[hxi[j]]) : 0.0f;
+#pragma unroll
+      for (int j = 0; j < NHY; ++j)
+        if (hye[j])
+          hy[threadIdx.x + j * NT] = (zin && hyv[j]) ? g2(Dc[hyi[j] + 2 * DX], Dc[hyi[j]]) : 0.0f;
+#pragma unroll
+      for (int j = 0; j < NHZ; ++j)
+        hz[threadIdx.x + j * NT] = (zin && hzv[j]) ? g2(Dn[hzi[j]], Dp[hzi[j]]) : 0.0f;
     }
     __syncthreads();
-    // 2. outputs of slice z = s - 2; then dog(s+1) into the slot of dog(s-2) (last read
-    //    above) and start fetching dog(s+2)
+    // 2. outputs of slice z = s - 2
     const int z = s - 2;
     if (z >= z0 && z < z1) {
-      const float* hxm = sHx[(z + 2) % 3];  // Hx(z-1)
-      const float* hx0 = sHx[(z + 3) % 3];  // Hx(z)
-      const float* hxp = sHx[(z + 4) % 3];  // Hx(z+1)
-      const float* hym = sHy[(z + 2) % 3];
-      const float* hy0 = sHy[(z + 3) % 3];
-      const float* hyp = sHy[(z + 4) % 3];
-      const float* hzm = sHz[(z + 2) % 3];
-      const float* hzp = sHz[(z + 4) % 3];
-      const int gx = x0 + tx;
+      const float* hxm = sHx[h1];  // Hx(z-1)
+      const float* hx0 = sHx[h2];  // Hx(z)
+      const float* hxp = sHx[hw];  // Hx(z+1)
+      const float* hym = sHy[h1];
+      const float* hy0 = sHy[h2];
+      const float* hyp = sHy[hw];
+      const float* hzm = sHz[h1];
+      const float* hzp = sHz[hw];
+      const uint32_t zb = (uint32_t)z * WH;
 #pragma unroll
-      for (int k = 0; k < TY / (NT / TX); ++k) {
+      for (int k = 0; k < NOUT; ++k) {
         const int ty = ty0 + k * (NT / TX);
-        const int gy = y0 + ty;
         bool is_cand = false;
         float h6[6];
         uint32_t cidx = 0;
-        if (gx < W && gy < H) {
+        if (ov[k]) {
           const int cx = (ty + 1) * XX + tx + 1;  // Hx centre
           const int cy = (ty + 1) * YX + tx;      // Hy centre
           const int cz = ty * TX + tx;            // Hz centre
@@ -179,8 +207,9 @@ __global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* 
           const float yy = g2(hy0[cy + YX], hy0[cy - YX]);
           const float yz = g2(hyp[cy], hym[cy]);
           const float zz = g2(hzp[cz], hzm[cz]);
-          const i64 idx = (i64)z * WH + (i64)gy * W + gx;
-          if (hess) {  // NULL: mask only (the region pass recomputes the elements)
+          const uint32_t idx = zb + po[k];
+          if (hess) {  // NULL: mask only (the region pass reads the candidate records)
+            const i64 N = (i64)WH * D;
             hess[idx] = cvt<TS>(xx);
             hess[N + idx] = cvt<TS>(xy);
             hess[2 * N + idx] = cvt<TS>(xz);
@@ -195,7 +224,7 @@ __global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* 
           if (is_cand) {
             h6[0] = round_to<TS>(xx); h6[1] = round_to<TS>(xy); h6[2] = round_to<TS>(xz);
             h6[3] = round_to<TS>(yy); h6[4] = round_to<TS>(yz); h6[5] = round_to<TS>(zz);
-            cidx = (uint32_t)idx;
+            cidx = idx;
           }
         }
         // candidate records: one LDS reservation per wave (a wave is one row of 64 x),
@@ -214,9 +243,12 @@ __global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* 
         }
       }
     }
-    put(s + 1);
+    // 3. dog(s+1) into the slot of dog(s-2) (last read above), loads of dog(s+2) issued
+    put(sD[dp]);
     fetch(s + 2);
     __syncthreads();
+    const int t = dp; dp = dc; dc = dn; dn = t;
+    const int u = h1; h1 = h2; h2 = hw; hw = u;
   }
   if (cand_cnt && threadIdx.x == 0) cand_cnt[tile] = s_cnt;
 }
