@@ -251,38 +251,62 @@ __device__ __forceinline__ int rim_count(int ex, int ey, int ez) {
 }
 
 constexpr int NTM = 256;
-constexpr int KR = 4;  // rim voxels per thread per pass: a 32 x 8 x 8 tile has <= 788
+constexpr int KR = 4;  // rim voxels per thread: a 32 x 8 x 8 tile has at most 788 <= KR * NTM
+static_assert(32 * 8 + 7 * (2 * 32 + 6 * 2) <= KR * 256, "rim of a full tile fits one pass");
 
 // Kernel 2.  Background voxels carry label 0 after kernel 1, so labels double as the
-// mask: round 1 loads the labels of up to KR rim voxels per thread, round 2 the labels of
-// their neighbours outside the clipped tile, all in flight together; then each distinct
-// neighbour label is linked with the voxel's own label by the global union-find (every
-// voxel of a tile-local component carries the same label, so few links remain).  Labels
-// read here may already have been linked by other work-groups; any value read is an
-// ancestor, so every link still joins the right sets.
+// mask.  Each distinct neighbour label is linked with the voxel's own label by the global
+// union-find (every voxel of a tile-local component carries the same label, so few links
+// remain).  Labels read here may already have been linked by other work-groups; any value
+// read is an ancestor, so every link still joins the right sets.
+// Round 1: the labels of up to KR rim voxels per thread (in flight together); the
+// foreground ones (label != 0) are compacted into an LDS list.  Round 2: one dense pass
+// over that list loads the labels of each voxel's neighbours outside the clipped tile and
+// links every distinct one with the voxel's own label.
 template <bool FULL>
 __device__ __forceinline__ void merge_tile(uint32_t* L, int W, uint32_t WH, uint32_t N, int x0,
-                                           int y0, int z0, int ex, int ey, int ez) {
+                                           int y0, int z0, int ex, int ey, int ez,
+                                           uint16_t* list_t, uint32_t* list_own,
+                                           uint32_t* n_list) {
   if (FULL) { ex = CX; ey = CY; ez = CZ; }  // compile-time shape: the rim math folds
   const int F = rim_count(ex, ey, ez);
-#pragma unroll 1
-  for (int t0 = 0; t0 < F; t0 += KR * NTM) {
-    int lx[KR], ly[KR], lz[KR];
-    uint32_t g[KR], own[KR];
+  const int lane = threadIdx.x & 63;
+  uint32_t own[KR];
 #pragma unroll
-    for (int k = 0; k < KR; ++k) {
-      const int t = t0 + threadIdx.x + k * NTM;
-      own[k] = 0u;
-      g[k] = 0u;
-      lx[k] = ly[k] = lz[k] = 0;
-      if (t < F) {
-        rim_voxel(t, ex, ey, ez, lx[k], ly[k], lz[k]);
-        g[k] = (uint32_t)(z0 + lz[k]) * WH + (uint32_t)(y0 + ly[k]) * (uint32_t)W +
-               (uint32_t)(x0 + lx[k]);
-        own[k] = L[g[k]];  // 0: background (voxel 0 is always background)
+  for (int k = 0; k < KR; ++k) {
+    const int t = threadIdx.x + k * NTM;
+    own[k] = 0u;
+    if (t < F) {
+      int lx, ly, lz;
+      rim_voxel(t, ex, ey, ez, lx, ly, lz);
+      own[k] = L[(uint32_t)(z0 + lz) * WH + (uint32_t)(y0 + ly) * (uint32_t)W +
+                 (uint32_t)(x0 + lx)];  // 0: background (voxel 0 is always background)
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < KR; ++k) {
+    const unsigned long long bal = __ballot(own[k] != 0u);
+    if (bal) {
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(n_list, (uint32_t)__popcll(bal));
+      base = __shfl(base, 0, 64);
+      if (own[k]) {
+        const uint32_t slot = base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+        list_t[slot] = (uint16_t)(threadIdx.x + k * NTM);
+        list_own[slot] = own[k];
       }
     }
-    uint32_t lj[KR][13];
+  }
+  __syncthreads();
+  const int n = (int)*n_list;
+#pragma unroll 1
+  for (int i = threadIdx.x; i < n; i += NTM) {
+    int lx, ly, lz;
+    rim_voxel(list_t[i], ex, ey, ez, lx, ly, lz);
+    const uint32_t me = list_own[i];
+    const uint32_t g = (uint32_t)(z0 + lz) * WH + (uint32_t)(y0 + ly) * (uint32_t)W +
+                       (uint32_t)(x0 + lx);
+    uint32_t lj[13];
 #pragma unroll
     for (int q = 0; q < 13; ++q) {
       int dx, dy, dz;
@@ -290,31 +314,29 @@ __device__ __forceinline__ void merge_tile(uint32_t* L, int W, uint32_t WH, uint
       // forward linear offset (positive for W, H >= 2); wraps like the reference
       const i64 off64 = (i64)dz * WH + (i64)dy * W + dx;
       const uint32_t off = (uint32_t)off64;
-      const bool fits = off64 < (i64)N;
-#pragma unroll
-      for (int k = 0; k < KR; ++k) {
-        const int nx = lx[k] + dx, ny = ly[k] + dy, nz = lz[k] + dz;
-        const bool local = nx >= 0 && nx < ex && ny >= 0 && ny < ey && nz < ez;
-        lj[k][q] = (own[k] && !local && fits && g[k] < N - off) ? L[g[k] + off] : 0u;
-      }
+      const int nx = lx + dx, ny = ly + dy, nz = lz + dz;
+      const bool local = nx >= 0 && nx < ex && ny >= 0 && ny < ey && nz < ez;
+      lj[q] = (!local && off64 < (i64)N && g < N - off) ? L[g + off] : 0u;
     }
+    uint32_t seen1 = 0u, seen2 = 0u;
 #pragma unroll
-    for (int k = 0; k < KR; ++k) {
-      uint32_t seen1 = 0u, seen2 = 0u;
-#pragma unroll
-      for (int q = 0; q < 13; ++q) {
-        const uint32_t l = lj[k][q];
-        if (!l || l == own[k] || l == seen1 || l == seen2) continue;
-        unite(L, own[k], l);
-        seen2 = seen1;
-        seen1 = l;
-      }
+    for (int q = 0; q < 13; ++q) {
+      const uint32_t l = lj[q];
+      if (!l || l == me || l == seen1 || l == seen2) continue;
+      unite(L, me, l);
+      seen2 = seen1;
+      seen1 = l;
     }
   }
 }
 
 __global__ __launch_bounds__(NTM) void k_ccl_merge(uint32_t* L, int W, int H, int D, int nbx,
                                                    int nby) {
+  __shared__ uint16_t list_t[KR * NTM];
+  __shared__ uint32_t list_own[KR * NTM];
+  __shared__ uint32_t n_list;
+  if (threadIdx.x == 0) n_list = 0;
+  __syncthreads();
   unsigned b = xcd_tile(blockIdx.x, gridDim.x);
   const int bx = b % nbx; b /= nbx;
   const int by = b % nby; b /= nby;
@@ -322,9 +344,9 @@ __global__ __launch_bounds__(NTM) void k_ccl_merge(uint32_t* L, int W, int H, in
   const int ex = min(CX, W - x0), ey = min(CY, H - y0), ez = min(CZ, D - z0);
   const uint32_t WH = (uint32_t)W * (uint32_t)H, N = WH * (uint32_t)D;
   if (ex == CX && ey == CY && ez == CZ)
-    merge_tile<true>(L, W, WH, N, x0, y0, z0, ex, ey, ez);
+    merge_tile<true>(L, W, WH, N, x0, y0, z0, ex, ey, ez, list_t, list_own, &n_list);
   else
-    merge_tile<false>(L, W, WH, N, x0, y0, z0, ex, ey, ez);
+    merge_tile<false>(L, W, WH, N, x0, y0, z0, ex, ey, ez, list_t, list_own, &n_list);
 }
 
 }  // namespace

@@ -135,16 +135,28 @@ __global__ __launch_bounds__(NT) void k_gauss_x_r(const TS* __restrict__ src,
     const float4 q = T4[v];
     win[4 * v] = q.x; win[4 * v + 1] = q.y; win[4 * v + 2] = q.z; win[4 * v + 3] = q.w;
   }
+  float2 out[RUN];
 #pragma unroll
   for (int o = 0; o < RUN; ++o) {
-    if (xs + o >= W) break;
     // window index of x = xs + o + j is o + j + RP
     float s1 = 0.0f, s2 = 0.0f;
 #pragma unroll
     for (int j = -R1; j <= R1; ++j) s1 = fmaf(w1[R1 - j], win[o + j + RP], s1);
 #pragma unroll
     for (int j = -R2; j <= R2; ++j) s2 = fmaf(w2[R2 - j], win[o + j + RP], s2);
-    dst[grow * W + xs + o] = make_float2(s1, s2);
+    out[o] = make_float2(s1, s2);
+  }
+  float2* drow = dst + grow * W;
+  // 16-byte stores (a wave writes 1 KB in two instructions) where the row start allows
+  if (xs + RUN <= W && ((grow * W) & 1) == 0) {
+#pragma unroll
+    for (int o = 0; o < RUN; o += 2)
+      *reinterpret_cast<float4*>(drow + xs + o) =
+          make_float4(out[o].x, out[o].y, out[o + 1].x, out[o + 1].y);
+  } else {
+#pragma unroll
+    for (int o = 0; o < RUN; ++o)
+      if (xs + o < W) drow[xs + o] = out[o];
   }
 }
 

@@ -94,76 +94,140 @@ __global__ __launch_bounds__(NT) void k_minmax_line(const void* __restrict__ src
   }
 }
 
-// x pass of min/max (packed) fused with the x pass of the box filter.  Tile: ROWS rows
-// times C outputs.  The min/max result is needed on [c0-r, c0+C+r) clamped to [0, W)
-// because the box pass clamps to the edge of the *filtered* row.  Each thread then
-// sums RUN consecutive box windows by sliding (integers: exact in any order).
-// Output: float2 (E_x, D_x) = (sum(min) * k, sum(max) * k).
-template <int ROWS, int RUN>
+// x pass of min/max (packed) fused with the x pass of the box filter.  One wave per row
+// (4 rows per work-group), C = 64 * RUN outputs per row tile.  The min/max result is
+// needed on [c0-r, c0+C+r) clamped to [0, W) because the box pass clamps to the edge of
+// the *filtered* row.  Phase 1 stages the clamped packed input of the row in LDS; phase 2
+// gives each lane 4 consecutive window minima (the window middle, shared by all four,
+// plus short prefix / suffix runs: about (2r + 12) / 4 packed mins per output); phase 3
+// gives each lane RUN consecutive box sums by sliding (integers: exact in any order).
+// Output: float2 (E_x, D_x) = (sum(min) * k, sum(max) * k).  R > 0 fixes the radius at
+// compile time (r must equal it); R = 0 takes r at run time.
+template <int R, int RUN>
 __global__ __launch_bounds__(NT) void k_minmax_x_box(const uint32_t* __restrict__ src,
                                                      float2* __restrict__ dst, int W, i64 nrows,
-                                                     int r, int nct, float k) {
-  constexpr int C = NT / ROWS * RUN;
-  extern __shared__ uint32_t smem[];
-  const int spanmax = C + 4 * r;
-  uint32_t* A = smem;
-  uint32_t* P = smem + ROWS * spanmax;
+                                                     int r_rt, int nct, float k) {
+  static_assert(RUN == 4, "four outputs per lane");
+  constexpr int C = 64 * RUN;
+  const int r = R > 0 ? R : r_rt;
+  extern __shared__ __align__(16) uint32_t smem_x[];
+  const int spanmax = C + 4 * r + 4;  // a multiple of 4: both rows stay 16-byte aligned
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint32_t* A = smem_x + wave * 2 * spanmax;  // clamped packed input of the row
+  uint32_t* B = A + spanmax;                  // window minima
   const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
   const int ct = tile % nct;
-  const i64 row0 = (i64)(tile / nct) * ROWS;
+  const i64 row = (i64)(tile / nct) * (NT / 64) + wave;
+  const bool live = row < nrows;  // wave-uniform
   const int c0 = ct * C;
   const int qa = max(0, c0 - r), qb = min(W, c0 + C + r);
-  const int nq = qb - qa;
-  const int span = nq + 2 * r;
-  stage_tile<8, NT, uint32_t>(
-      ROWS * span,
-      [&](int e) -> uint32_t {
-        const int row = e / span, p = e % span;
-        const i64 grow = row0 + row;
-        return grow < nrows ? src[grow * W + clampi(qa - r + p, 0, W - 1)] : 0u;
-      },
-      [&](int e, uint32_t v) { A[(e / span) * spanmax + e % span] = v; });
+  const int nq = qb - qa, span = nq + 2 * r;
+  const uint32_t* srow = src + (live ? row : 0) * W;
+  if (live)
+    for (int p = lane; p < span; p += 64) A[p] = srow[clampi(qa - r + p, 0, W - 1)];
   __syncthreads();
-  const int w = 2 * r + 1;
-  const int nb = (span + w - 1) / w;
-  for (int t = threadIdx.x; t < ROWS * nb; t += NT) {
-    const int row = t / nb, blk = t % nb;
-    const int p0 = blk * w;
-    vh_block(A + row * spanmax, P + row * spanmax, p0, min(p0 + w, span), 1);
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < ROWS * nq; e += NT) {
-    const int row = e / nq, j = e % nq;
-    A[row * spanmax + j] = pkmin(A[row * spanmax + j], P[row * spanmax + j + 2 * r]);
-  }
-  __syncthreads();
-  const int row = threadIdx.x / (NT / ROWS), seg = threadIdx.x % (NT / ROWS);
-  const i64 grow = row0 + row;
-  if (grow >= nrows) return;
-  const uint32_t* m = A + row * spanmax - qa;
-  const int xs = c0 + seg * RUN;
-  if (xs >= W) return;
-  uint32_t se = 0, sdn = 0;  // sum of min, sum of (65535 - max)
-  for (int j = -r; j <= r; ++j) {
-    const uint32_t v = m[clampi(xs + j, 0, W - 1)];
-    se += v & 0xFFFFu;
-    sdn += v >> 16;
-  }
-  const uint32_t full = (uint32_t)(2 * r + 1) * 65535u;
+  // window minima B[j] = min(A[j .. j + 2r]), j in [0, nq), four per lane: the window
+  // middle A[j0+3 .. j0+2r] is shared by all four, plus short prefix / suffix runs
+  for (int j0 = lane * 4; live && j0 < nq; j0 += 256) {
+    uint32_t o[4];
+    if constexpr (R >= 2) {  // compile-time radius: 16-byte LDS reads, conflict-free
+      constexpr int NV = (2 * R + 4) / 4;
+      uint32_t w[4 * NV];
 #pragma unroll
-  for (int q = 0; q < RUN; ++q) {
-    const int x = xs + q;
-    if (x >= W) break;
-    if (q > 0) {
-      const uint32_t vin = m[clampi(x + r, 0, W - 1)], vout = m[clampi(x - r - 1, 0, W - 1)];
-      se += (vin & 0xFFFFu) - (vout & 0xFFFFu);
-      sdn += (vin >> 16) - (vout >> 16);
+      for (int v = 0; v < NV; ++v) {
+        const uint4 q = *reinterpret_cast<const uint4*>(A + j0 + 4 * v);
+        w[4 * v] = q.x; w[4 * v + 1] = q.y; w[4 * v + 2] = q.z; w[4 * v + 3] = q.w;
+      }
+      uint32_t mid = w[3];
+#pragma unroll
+      for (int t = 4; t <= 2 * R; ++t) mid = pkmin(mid, w[t]);
+      const uint32_t l2 = w[2], l1 = pkmin(w[1], l2), l0 = pkmin(w[0], l1);
+      const uint32_t r1 = w[2 * R + 1], r2 = pkmin(r1, w[2 * R + 2]), r3 = pkmin(r2, w[2 * R + 3]);
+      o[0] = pkmin(mid, l0);
+      o[1] = pkmin(pkmin(mid, l1), r1);
+      o[2] = pkmin(pkmin(mid, l2), r2);
+      o[3] = pkmin(mid, r3);
+    } else if (2 * r + 1 >= 4) {
+      uint32_t mid = A[j0 + 3];
+      for (int t = 4; t <= 2 * r; ++t) mid = pkmin(mid, A[j0 + t]);
+      const uint32_t l2 = A[j0 + 2], l1 = pkmin(A[j0 + 1], l2), l0 = pkmin(A[j0], l1);
+      const uint32_t r1 = A[j0 + 2 * r + 1], r2 = pkmin(r1, A[j0 + 2 * r + 2]),
+                     r3 = pkmin(r2, A[j0 + 2 * r + 3]);
+      o[0] = pkmin(mid, l0);
+      o[1] = pkmin(pkmin(mid, l1), r1);
+      o[2] = pkmin(pkmin(mid, l2), r2);
+      o[3] = pkmin(mid, r3);
+    } else {  // r = 1
+#pragma unroll
+      for (int q = 0; q < 4; ++q) o[q] = pkmin(pkmin(A[j0 + q], A[j0 + q + 1]), A[j0 + q + 2]);
     }
-    dst[grow * W + x] = make_float2((float)se * k, (float)(full - sdn) * k);
+    *reinterpret_cast<uint4*>(B + j0) = make_uint4(o[0], o[1], o[2], o[3]);
+  }
+  __syncthreads();
+  const int xs = c0 + lane * RUN;
+  if (!live || xs >= W) return;
+  const bool interior = c0 - r >= 0 && c0 + C + r <= W;  // no clamping in this tile
+  const uint32_t full = (uint32_t)(2 * r + 1) * 65535u;
+  float2 out[RUN];
+  if (R > 0 && interior) {
+    // the window minima of x - r .. x + 3 + r are B[4 lane .. 4 lane + 2r + 3] (qa = c0 - r)
+    constexpr int RR = R > 0 ? R : 1;
+    constexpr int NV = (2 * RR + 4 + 3) / 4;
+    uint32_t w[4 * NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      const uint4 q = *reinterpret_cast<const uint4*>(B + lane * 4 + 4 * v);
+      w[4 * v] = q.x; w[4 * v + 1] = q.y; w[4 * v + 2] = q.z; w[4 * v + 3] = q.w;
+    }
+    uint32_t se = 0, sdn = 0;  // sum of min, sum of (65535 - max)
+#pragma unroll
+    for (int t = 0; t <= 2 * RR; ++t) {
+      se += w[t] & 0xFFFFu;
+      sdn += w[t] >> 16;
+    }
+#pragma unroll
+    for (int q = 0; q < RUN; ++q) {
+      if (q > 0) {
+        const uint32_t vin = w[2 * RR + q], vout = w[q - 1];
+        se += (vin & 0xFFFFu) - (vout & 0xFFFFu);
+        sdn += (vin >> 16) - (vout >> 16);
+      }
+      out[q] = make_float2((float)se * k, (float)(full - sdn) * k);
+    }
+  } else {
+    const uint32_t* m = B - qa;  // m[x]: window minimum at x, x in [qa, qb)
+    auto at = [&](int x) { return m[clampi(x, 0, W - 1)]; };
+    uint32_t se = 0, sdn = 0;
+    for (int j = -r; j <= r; ++j) {
+      const uint32_t v = at(xs + j);
+      se += v & 0xFFFFu;
+      sdn += v >> 16;
+    }
+#pragma unroll
+    for (int q = 0; q < RUN; ++q) {
+      if (q > 0) {
+        const int x = xs + q;
+        const uint32_t vin = at(x + r), vout = at(x - r - 1);
+        se += (vin & 0xFFFFu) - (vout & 0xFFFFu);
+        sdn += (vin >> 16) - (vout >> 16);
+      }
+      out[q] = make_float2((float)se * k, (float)(full - sdn) * k);
+    }
+  }
+  float2* drow = dst + row * W;
+  // 16-byte stores (a wave writes 1 KB in two instructions) where the row start allows
+  if (xs + RUN <= W && ((row * W) & 1) == 0) {
+#pragma unroll
+    for (int q = 0; q < RUN; q += 2)
+      *reinterpret_cast<float4*>(drow + xs + q) =
+          make_float4(out[q].x, out[q].y, out[q + 1].x, out[q + 1].y);
+  } else {
+#pragma unroll
+    for (int q = 0; q < RUN; ++q)
+      if (xs + q < W) drow[xs + q] = out[q];
   }
 }
 
-// Generic box pass on float2 (E, D) along y or z: clamp-to-edge, sequential FP32 sums.
 // With NORM the z pass also normalizes against img.
 template <int XW, bool NORM, typename TS>
 __global__ __launch_bounds__(NT) void k_box_line(const float2* __restrict__ src,
@@ -430,15 +494,18 @@ hipError_t launch_normalize(const uint16_t* img, TS* norm, void* tmp1, void* tmp
   mark("normalize/minmax_y", s);
   // min/max along x fused with the box x pass -> float2 (E, D) in tmp1
   {
-    constexpr int ROWS = 4, RUN = 4;
-    constexpr int C = NT / ROWS * RUN;
+    constexpr int RUN = 4, C = 64 * RUN, ROWS = NT / 64;
     const int nct = (W + C - 1) / C;
     const i64 nrows = (i64)H * D;
     const i64 nblk = ((nrows + ROWS - 1) / ROWS) * nct;
-    const size_t lds = 2 * (size_t)ROWS * (C + 4 * rxy) * 4;
+    const size_t lds = (size_t)ROWS * 2 * (C + 4 * rxy + 4) * 4;
     const float k = 1.0f / (float)(2 * rxy + 1);
-    hipLaunchKernelGGL((k_minmax_x_box<ROWS, RUN>), dim3((unsigned)nblk), dim3(NT), lds, s, p2,
-                       (float2*)tmp1, W, nrows, rxy, nct, k);
+    if (rxy == 10)
+      hipLaunchKernelGGL((k_minmax_x_box<10, RUN>), dim3((unsigned)nblk), dim3(NT), lds, s, p2,
+                         (float2*)tmp1, W, nrows, rxy, nct, k);
+    else
+      hipLaunchKernelGGL((k_minmax_x_box<0, RUN>), dim3((unsigned)nblk), dim3(NT), lds, s, p2,
+                         (float2*)tmp1, W, nrows, rxy, nct, k);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     mark("normalize/minmax_x_box_x", s);
   }
