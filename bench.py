@@ -91,6 +91,22 @@ C1 = dict(name="c1", rxy=10, rz=10, min_intensity=0, sigma_xy1=2.0, sigma_z1=1.6
           sigma_xy2=3.2, sigma_z2=2.56, truncate=4.0, gamma=1.0)
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hdog_c1_fp32_traffic.json")
+
+
+def measured_traffic(stage, form, N):
+    """HBM bytes per launch of `stage` (FETCH_SIZE + WRITE_SIZE, scaled by the copy
+    calibration) from the committed rocprofv3 summary of this workload, or None."""
+    if form != "fp32" or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        t = json.load(f)
+    k = t.get("kernels", {}).get(stage)
+    if k is None or t.get("voxels") != N:
+        return None
+    return k["fetch_bytes"] + k["write_bytes"]
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -330,7 +346,8 @@ def main():
                    "sigma2_zyx": [C1["sigma_z2"], C1["sigma_xy2"], C1["sigma_xy2"]],
                    "truncate": C1["truncate"], "storage": main_f},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(dom, main_f, N),
                      "alg_bytes_per_voxel": kb,
                      "avg_ms": kernels[dom]},
         "pipeline_roofline": {"bytes_per_voxel": pipeline_bytes(main_f), "achieved": pipe_gbs,
