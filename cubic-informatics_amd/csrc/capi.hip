@@ -21,7 +21,7 @@ struct Buf {
 struct BatchState {
   std::vector<hdog_tile> tiles;
   hdog_params p{};
-  Buf tmp1, tmp2, norm, dog, pd, labels, rid, bcnt, dR, acc, weights, tout, running, cand,
+  Buf tmp1, tmp2, norm, dog, pd, labels, rid, bcnt, rmask, dR, acc, weights, tout, running, cand,
       cand_cnt;
   Buf a_label, a_size, a_vox, a_cen, a_eig, a_nrm, a_hess, a_isum, a_imax;
   i64 acc_cap = 0, arena_cap = 0;
@@ -36,7 +36,7 @@ struct BatchState {
 struct hdog_ctx {
   int device = 0;
   std::string err;
-  Buf tmp1, tmp2, rid, bcnt, dR, acc, weights, blobs, sw, sums, ddog, cand, cand_cnt;
+  Buf tmp1, tmp2, rid, bcnt, rmask, dR, acc, weights, blobs, sw, sums, ddog, cand, cand_cnt;
   Buf o_label, o_size, o_vox, o_cen, o_eig, o_nrm, o_hess, o_isum, o_imax;
   i64 out_cap = 0;
   Buf img, norm, dog, hess, pd, labels;  // substack buffers of hdog_process_substack
@@ -154,11 +154,13 @@ int regional_features_t(hdog_ctx* ctx, hipStream_t s, uint32_t* labels, const TH
                         bool flatten = false) {
   const i64 N = (i64)W * H * D;
   HIPCHK(ensure(ctx->rid, (size_t)N * 4));
-  const i64 nb = (N + 4095) / 4096;
+  const i64 nb = (N + kRootBlock - 1) / kRootBlock;
   HIPCHK(ensure(ctx->bcnt, (size_t)nb * 4));
+  HIPCHK(ensure(ctx->rmask, (size_t)nb * (kRootBlock / 8)));
   HIPCHK(ensure(ctx->dR, 64));
   i64 nbo = 0;
-  HIPCHK(launch_region_ids(labels, (uint32_t*)ctx->rid.p, (uint32_t*)ctx->bcnt.p,
+  unsigned long long* masks = (unsigned long long*)ctx->rmask.p;
+  HIPCHK(launch_region_ids(labels, (uint32_t*)ctx->rid.p, (uint32_t*)ctx->bcnt.p, masks,
                            (uint32_t*)ctx->dR.p, N, s, &nbo, flatten));
   HIPCHK(hipMemcpyAsync(ctx->h_R, ctx->dR.p, 4, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
@@ -166,7 +168,7 @@ int regional_features_t(hdog_ctx* ctx, hipStream_t s, uint32_t* labels, const TH
   int rc = ensure_regions(ctx, R);
   if (rc) return rc;
   RegionAcc* acc = (RegionAcc*)ctx->acc.p;
-  HIPCHK(launch_region_assign(labels, (const uint32_t*)ctx->bcnt.p, (uint32_t*)ctx->rid.p, acc, R, R,
+  HIPCHK(launch_region_assign(masks, (const uint32_t*)ctx->bcnt.p, (uint32_t*)ctx->rid.p, acc, R, R,
                               nullptr, N, s));
   if (cand.rec)
     HIPCHK(launch_region_accum_cand<TN>(cand.rec, cand.cnt, cand.ntiles, labels,
@@ -298,7 +300,8 @@ void batch_free(BatchState* B) {
   if (B->graph) (void)hipGraphDestroy(B->graph);
   if (B->cs) (void)hipStreamDestroy(B->cs);
   if (B->h_tout) (void)hipHostFree(B->h_tout);
-  for (Buf* b : {&B->tmp1, &B->tmp2, &B->norm, &B->dog, &B->pd, &B->labels, &B->rid, &B->cand,
+  for (Buf* b : {&B->tmp1, &B->tmp2, &B->norm, &B->dog, &B->pd, &B->labels, &B->rid, &B->rmask,
+                 &B->cand,
                  &B->cand_cnt,
                  &B->bcnt, &B->dR, &B->acc, &B->weights, &B->tout, &B->running, &B->a_label,
                  &B->a_size, &B->a_vox, &B->a_cen, &B->a_eig, &B->a_nrm, &B->a_hess, &B->a_isum,
@@ -335,8 +338,9 @@ hipError_t batch_tile(hdog_ctx* ctx, BatchState* B, int t, hipStream_t s) {
   if ((e = launch_ccl(pd, labels, W, H, D, s, false)) != hipSuccess) return e;
   i64 nbo = 0;
   uint32_t* dR = (uint32_t*)B->dR.p;
-  if ((e = launch_region_ids(labels, (uint32_t*)B->rid.p, (uint32_t*)B->bcnt.p, dR, N, s, &nbo,
-                             true)) !=
+  unsigned long long* masks = (unsigned long long*)B->rmask.p;
+  if ((e = launch_region_ids(labels, (uint32_t*)B->rid.p, (uint32_t*)B->bcnt.p, masks, dR, N, s,
+                             &nbo, true)) !=
       hipSuccess)
     return e;
   TileOut* tout = (TileOut*)B->tout.p + t;
@@ -344,7 +348,7 @@ hipError_t batch_tile(hdog_ctx* ctx, BatchState* B, int t, hipStream_t s) {
       hipSuccess)
     return e;
   RegionAcc* acc = (RegionAcc*)B->acc.p;
-  if ((e = launch_region_assign(labels, (const uint32_t*)B->bcnt.p, (uint32_t*)B->rid.p, acc,
+  if ((e = launch_region_assign(masks, (const uint32_t*)B->bcnt.p, (uint32_t*)B->rid.p, acc,
                                 B->acc_cap, B->acc_cap, dR, N, s)) != hipSuccess)
     return e;
   if ((e = launch_region_accum_cand<TN>((const float*)B->cand.p, (const uint32_t*)B->cand_cnt.p,
@@ -412,7 +416,8 @@ int hdog_create(hdog_ctx** out, int device) {
 int hdog_release_scratch(hdog_ctx* ctx) {
   if (!ctx) return HDOG_ERR_ARG;
   (void)hipSetDevice(ctx->device);
-  for (Buf* b : {&ctx->tmp1, &ctx->tmp2, &ctx->rid, &ctx->bcnt, &ctx->dR, &ctx->acc, &ctx->blobs,
+  for (Buf* b : {&ctx->tmp1, &ctx->tmp2, &ctx->rid, &ctx->bcnt, &ctx->rmask, &ctx->dR, &ctx->acc,
+                 &ctx->blobs,
                  &ctx->sw, &ctx->sums, &ctx->ddog,
                  &ctx->o_label, &ctx->o_size, &ctx->o_vox, &ctx->o_cen, &ctx->o_eig, &ctx->o_nrm,
                  &ctx->o_hess, &ctx->o_isum, &ctx->o_imax, &ctx->img, &ctx->norm, &ctx->dog,
@@ -853,7 +858,9 @@ int hdog_batch_plan(hdog_ctx* ctx, const hdog_params* p, const hdog_tile* tiles,
   } need[] = {{&B->tmp1, (size_t)maxN * 8}, {&B->tmp2, (size_t)maxN * 8},
               {&B->norm, maxN * es}, {&B->dog, maxN * ed},
               {&B->pd, (size_t)maxN}, {&B->labels, (size_t)maxN * 4}, {&B->rid, (size_t)maxN * 4},
-              {&B->bcnt, (size_t)((maxN + 4095) / 4096) * 4}, {&B->dR, 64},
+              {&B->bcnt, (size_t)((maxN + kRootBlock - 1) / kRootBlock) * 4},
+              {&B->rmask, (size_t)((maxN + kRootBlock - 1) / kRootBlock) * (kRootBlock / 8)},
+              {&B->dR, 64},
               {&B->acc, (size_t)B->acc_cap * sizeof(RegionAcc)}, {&B->weights, ctx->weights.bytes},
               {&B->tout, (size_t)n_tiles * sizeof(TileOut)}, {&B->running, 64},
               {&B->a_label, (size_t)cap * 4}, {&B->a_size, (size_t)cap * 2},

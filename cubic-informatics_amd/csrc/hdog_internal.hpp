@@ -15,6 +15,7 @@ namespace hdog {
 constexpr int kMaxGaussRadius = 40;            // 2r+1 <= 81 taps per axis
 constexpr int kMaxTaps = 2 * kMaxGaussRadius + 1;
 constexpr int kMaxNormRadius = 200;
+constexpr int kRootBlock = 4096;  // voxels per root-numbering block (64 mask words)
 // Hessian tiles: 64 x 8 outputs per work-group over 16 slices.  Candidate records are kept
 // per tile in a block of kCandTile slots x 7 words (voxel index, then the six elements).
 constexpr int kHessTX = 64, kHessTY = 8, kHessTZ = 16;
@@ -157,14 +158,16 @@ hipError_t launch_hessian(const TD* dog, TH* hess, uint8_t* pd, int W, int H, in
 hipError_t launch_ccl(const uint8_t* pd, uint32_t* labels, int W, int H, int D, hipStream_t s,
                       bool flatten = true);
 // region numbering: fills rid[root] for every root, block counts, and *d_R = R
+// Root counts per block of 4096 voxels and their positions (masks: 64 words per block),
+// then *d_R = 1 + number of roots.
 hipError_t launch_region_ids(uint32_t* labels, uint32_t* rid, uint32_t* block_cnt,
-                             uint32_t* d_R, i64 N, hipStream_t s, i64* nblocks_out,
-                             bool flatten = false);
+                             unsigned long long* masks, uint32_t* d_R, i64 N, hipStream_t s,
+                             i64* nblocks_out, bool flatten = false);
 // R is the host's region count, or, when d_R is non-null, read on the device as
 // min(*d_R, cap) (graph-captured batches, where the host never sees R); ids >= cap are skipped.
-hipError_t launch_region_assign(const uint32_t* labels, const uint32_t* block_off, uint32_t* rid,
-                                RegionAcc* acc, i64 R, i64 cap, const uint32_t* d_R, i64 N,
-                                hipStream_t s);
+hipError_t launch_region_assign(const unsigned long long* masks, const uint32_t* block_off,
+                                uint32_t* rid, RegionAcc* acc, i64 R, i64 cap,
+                                const uint32_t* d_R, i64 N, hipStream_t s);
 template <typename TH, typename TN>
 hipError_t launch_region_accum(const uint32_t* labels, const uint32_t* rid, const TH* hess,
                                const TN* norm, const uint16_t* img, RegionAcc* acc, i64 cap,

@@ -22,6 +22,7 @@ namespace {
 constexpr int NT = 256;
 constexpr int ITEMS = 16;
 constexpr i64 BLK = (i64)NT * ITEMS;  // voxels per numbering block
+static_assert(BLK == kRootBlock, "numbering block");
 
 __device__ __forceinline__ int block_sum_int(int v, int* sh) {
   // wave64 reduction, then across the 4 waves
@@ -34,14 +35,17 @@ __device__ __forceinline__ int block_sum_int(int v, int* sh) {
   return t;
 }
 
-// Per block of BLK voxels: the number of roots (L[i] == i, i > 0).  FLATTEN: also points
-// every other foreground voxel at its root (labels leave the union-find with parents that
-// are only ancestors).  A root is its own parent before and after, so counting and
-// flattening can share the pass.
+// Per block of BLK voxels: the number of roots (L[i] == i, i > 0) and their positions as
+// a bitmask (masks[64 b + 4 k + wave] bit lane <-> voxel b*BLK + k*NT + 64 wave + lane,
+// so bit order is linear order).  FLATTEN: also points every other foreground voxel at its
+// root (labels leave the union-find with parents that are only ancestors).  A root is its
+// own parent before and after, so counting and flattening can share the pass.
 template <bool FLATTEN>
-__global__ __launch_bounds__(NT) void k_count_roots(uint32_t* L, i64 N, uint32_t* __restrict__ cnt) {
+__global__ __launch_bounds__(NT) void k_count_roots(uint32_t* L, i64 N, uint32_t* __restrict__ cnt,
+                                                    unsigned long long* __restrict__ masks) {
   __shared__ int sh[4];
   const i64 base = (i64)blockIdx.x * BLK;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   uint32_t v[ITEMS];
 #pragma unroll
   for (int k = 0; k < ITEMS; ++k) {
@@ -52,10 +56,11 @@ __global__ __launch_bounds__(NT) void k_count_roots(uint32_t* L, i64 N, uint32_t
 #pragma unroll
   for (int k = 0; k < ITEMS; ++k) {
     const i64 i = base + (i64)k * NT + threadIdx.x;
-    if (i <= 0 || i >= N || v[k] == 0u) continue;
-    if (v[k] == (uint32_t)i) {
-      ++c;
-    } else if (FLATTEN) {
+    const bool root = i > 0 && i < N && v[k] == (uint32_t)i;
+    const unsigned long long m = __ballot(root);
+    if (lane == 0) masks[(size_t)blockIdx.x * (BLK / 64) + k * (NT / 64) + wave] = m;
+    c += root ? 1 : 0;
+    if (FLATTEN && !root && i > 0 && i < N && v[k] != 0u) {
       const uint32_t r = find_root_readonly(L, v[k]);
       if (r != v[k]) L[i] = r;
     }
@@ -117,34 +122,32 @@ __global__ __launch_bounds__(1024) void k_scan_counts(uint32_t* cnt, i64 nb, uin
   if (threadIdx.x == 0) *d_R = 1u + carry;
 }
 
-// rid[root] = 1 + rank of the root in linear order; acc[rid].root = root
-__global__ __launch_bounds__(NT) void k_assign_ids(const uint32_t* __restrict__ L, i64 N,
+// rid[root] = 1 + rank of the root in linear order; acc[rid].root = root.  One wave per
+// block of BLK voxels: lane m takes root mask m of the block, a wave scan of the mask
+// popcounts gives each mask's rank offset, and only the (few) roots are visited.
+__global__ __launch_bounds__(64) void k_assign_ids(const unsigned long long* __restrict__ masks,
                                                    const uint32_t* __restrict__ off,
                                                    uint32_t* __restrict__ rid,
                                                    RegionAcc* __restrict__ acc, i64 cap) {
-  __shared__ uint32_t wsum[4];
-  const i64 base = (i64)blockIdx.x * BLK;
-  uint32_t run = off[blockIdx.x];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int k = 0; k < ITEMS; ++k) {
-    const i64 i = base + (i64)k * NT + threadIdx.x;
-    const bool isroot = i > 0 && i < N && L[i] == (uint32_t)i;
-    const unsigned long long m = __ballot(isroot);
-    const uint32_t pre = __popcll(m & ((1ull << lane) - 1ull));
-    if (lane == 0) wsum[wave] = __popcll(m);
-    __syncthreads();
-    uint32_t woff = 0, tot = 0;
-    for (int q = 0; q < 4; ++q) {
-      woff += q < wave ? wsum[q] : 0;
-      tot += wsum[q];
-    }
-    if (isroot) {
-      const uint32_t r = 1 + run + woff + pre;
-      rid[i] = r;
-      if ((i64)r < cap) acc[r].root = (uint32_t)i;
-    }
-    run += tot;
-    __syncthreads();
+  static_assert(BLK / 64 == 64, "one mask per lane");
+  const int lane = threadIdx.x;
+  unsigned long long m = masks[(size_t)blockIdx.x * 64 + lane];
+  const uint32_t n = (uint32_t)__popcll(m);
+  uint32_t incl = n;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  uint32_t r = 1 + off[blockIdx.x] + (incl - n);
+  const i64 base = (i64)blockIdx.x * BLK + 64 * lane;
+  while (m) {
+    const int bit = __builtin_ctzll(m);
+    m &= m - 1;
+    const i64 i = base + bit;
+    rid[i] = r;
+    if ((i64)r < cap) acc[r].root = (uint32_t)i;
+    ++r;
   }
 }
 
@@ -445,17 +448,17 @@ __global__ __launch_bounds__(NT) void k_fill_u64(u64* p, i64 n, u64 v) {
 }  // namespace
 
 hipError_t launch_region_ids(uint32_t* labels, uint32_t* rid, uint32_t* block_cnt,
-                             uint32_t* d_R, i64 N, hipStream_t s, i64* nblocks_out,
-                             bool flatten) {
+                             unsigned long long* masks, uint32_t* d_R, i64 N, hipStream_t s,
+                             i64* nblocks_out, bool flatten) {
   (void)rid;
   const i64 nb = (N + BLK - 1) / BLK;
   *nblocks_out = nb;
   if (flatten) {
     hipLaunchKernelGGL(k_count_roots<true>, dim3((unsigned)nb), dim3(NT), 0, s, labels, N,
-                       block_cnt);
+                       block_cnt, masks);
   } else {
     hipLaunchKernelGGL(k_count_roots<false>, dim3((unsigned)nb), dim3(NT), 0, s, labels, N,
-                       block_cnt);
+                       block_cnt, masks);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -465,9 +468,9 @@ hipError_t launch_region_ids(uint32_t* labels, uint32_t* rid, uint32_t* block_cn
   return hipGetLastError();
 }
 
-hipError_t launch_region_assign(const uint32_t* labels, const uint32_t* block_off, uint32_t* rid,
-                                RegionAcc* acc, i64 R, i64 cap, const uint32_t* d_R, i64 N,
-                                hipStream_t s) {
+hipError_t launch_region_assign(const unsigned long long* masks, const uint32_t* block_off,
+                                uint32_t* rid, RegionAcc* acc, i64 R, i64 cap,
+                                const uint32_t* d_R, i64 N, hipStream_t s) {
   const i64 words = (d_R ? cap : std::min(R, cap)) * (i64)(sizeof(RegionAcc) / 8);
   const i64 zb = std::min<i64>((words + NT - 1) / NT, 1 << 16);
   mark(nullptr, s);  // the host waited for the region count: start a new interval
@@ -477,8 +480,8 @@ hipError_t launch_region_assign(const uint32_t* labels, const uint32_t* block_of
   if (e != hipSuccess) return e;
   mark("regions/zero", s);
   const i64 nb = (N + BLK - 1) / BLK;
-  hipLaunchKernelGGL(k_assign_ids, dim3((unsigned)nb), dim3(NT), 0, s, labels, N, block_off, rid,
-                     acc, cap);
+  hipLaunchKernelGGL(k_assign_ids, dim3((unsigned)nb), dim3(64), 0, s, masks, block_off, rid, acc,
+                     cap);
   mark("regions/assign_ids", s);
   return hipGetLastError();
 }
