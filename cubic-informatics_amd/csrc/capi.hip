@@ -50,6 +50,12 @@ struct hdog_ctx {
   Prof pool;
   bool ev_valid = false;
   BatchState* batch = nullptr;
+  // split substack form: two device image slots filled on a copy stream
+  Buf up_img[2];
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t up_done[2] = {nullptr, nullptr};   // upload into the slot complete
+  hipEvent_t run_done[2] = {nullptr, nullptr};  // last run reading the slot complete
+  int up_depth[2] = {0, 0};
 };
 
 thread_local hdog::Prof* hdog::tl_prof = nullptr;
@@ -431,6 +437,13 @@ int hdog_release_scratch(hdog_ctx* ctx) {
 void hdog_destroy(hdog_ctx* ctx) {
   if (!ctx) return;
   hdog_batch_release(ctx);
+  (void)hipSetDevice(ctx->device);
+  for (int k = 0; k < 2; ++k) {
+    if (ctx->up_done[k]) (void)hipEventDestroy(ctx->up_done[k]);
+    if (ctx->run_done[k]) (void)hipEventDestroy(ctx->run_done[k]);
+    release(ctx->up_img[k]);
+  }
+  if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
   hdog_release_scratch(ctx);
   if (ctx->prof)
     for (hipEvent_t& e : ctx->pool.ev) (void)hipEventDestroy(e);
@@ -970,6 +983,68 @@ int hdog_batch_download(hdog_ctx* ctx, void* stream, int tile, hdog_regions* out
   HIPCHK(cp(out->intensity_max, B->a_imax.p, 2));
   HIPCHK(hipStreamSynchronize(s));
   return HDOG_OK;
+}
+
+int hdog_host_alloc(hdog_ctx* ctx, size_t bytes, void** p) {
+  if (!ctx || !p) return HDOG_ERR_ARG;
+  *p = nullptr;
+  HIPCHK(hipSetDevice(ctx->device));
+  HIPCHK(hipHostMalloc(p, bytes ? bytes : 1, hipHostMallocDefault));
+  return HDOG_OK;
+}
+
+int hdog_host_free(hdog_ctx* ctx, void* p) {
+  if (!ctx) return HDOG_ERR_ARG;
+  if (p) HIPCHK(hipHostFree(p));
+  return HDOG_OK;
+}
+
+int hdog_substack_upload(hdog_ctx* ctx, const hdog_params* p, const uint16_t* h_img,
+                         int depth_valid, int slot) {
+  if (!ctx) return HDOG_ERR_ARG;
+  int rc = check_params(ctx, p);
+  if (rc) return rc;
+  if (!h_img || depth_valid < 1 || depth_valid > p->depth || slot < 0 || slot > 1)
+    return fail(ctx, HDOG_ERR_ARG, "depth_valid %d outside [1, %d] or slot %d not 0 / 1",
+                depth_valid, p->depth, slot);
+  HIPCHK(hipSetDevice(ctx->device));
+  if (!ctx->copy_stream) HIPCHK(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
+  for (hipEvent_t* ev : {&ctx->up_done[slot], &ctx->run_done[slot]})
+    if (!*ev) HIPCHK(hipEventCreateWithFlags(ev, hipEventDisableTiming));
+  const i64 plane = (i64)p->width * p->height;
+  const i64 N = plane * p->depth;
+  // the slot is rewritten only after its previous upload and the last run reading it ended
+  HIPCHK(hipEventSynchronize(ctx->up_done[slot]));
+  HIPCHK(hipEventSynchronize(ctx->run_done[slot]));
+  HIPCHK(ensure(ctx->up_img[slot], (size_t)N * 2));
+  uint16_t* d = (uint16_t*)ctx->up_img[slot].p;
+  HIPCHK(hipMemcpyAsync(d, h_img, (size_t)(plane * depth_valid) * 2, hipMemcpyHostToDevice,
+                        ctx->copy_stream));
+  if (depth_valid < p->depth)  // zero-fill, src/3D/HDoG3D_main.cpp:166-169
+    HIPCHK(hipMemsetAsync(d + plane * depth_valid, 0, (size_t)(N - plane * depth_valid) * 2,
+                          ctx->copy_stream));
+  HIPCHK(hipEventRecord(ctx->up_done[slot], ctx->copy_stream));
+  ctx->up_depth[slot] = p->depth;
+  return HDOG_OK;
+}
+
+int hdog_substack_run(hdog_ctx* ctx, void* stream, const hdog_params* p, int slot,
+                      hdog_regions* out) {
+  if (!ctx) return HDOG_ERR_ARG;
+  int rc = check_params(ctx, p);
+  if (rc) return rc;
+  if (slot < 0 || slot > 1 || !ctx->up_done[slot] || ctx->up_depth[slot] != p->depth)
+    return fail(ctx, HDOG_ERR_STATE, "slot %d holds no upload of this shape", slot);
+  HIPCHK(hipSetDevice(ctx->device));
+  hipStream_t s = (hipStream_t)stream;
+  HIPCHK(hipStreamWaitEvent(s, ctx->up_done[slot], 0));
+  int64_t R = 0;
+  rc = hdog_run_device(ctx, stream, p, (const uint16_t*)ctx->up_img[slot].p, nullptr, nullptr,
+                       nullptr, nullptr, nullptr, &R);
+  HIPCHK(hipEventRecord(ctx->run_done[slot], s));
+  if (rc) return rc;
+  if (!out) return HDOG_OK;
+  return hdog_regions_download(ctx, stream, out);
 }
 
 }  // extern "C"

@@ -18,7 +18,10 @@ constexpr int kMaxNormRadius = 200;
 constexpr int kRootBlock = 4096;  // voxels per root-numbering block (64 mask words)
 // Hessian tiles: 64 x 8 outputs per work-group over 16 slices.  Candidate records are kept
 // per tile in a block of kCandTile slots x 7 words (voxel index, then the six elements).
-constexpr int kHessTX = 64, kHessTY = 8, kHessTZ = 16;
+#ifndef HDOG_HESS_TY
+#define HDOG_HESS_TY 8
+#endif
+constexpr int kHessTX = 64, kHessTY = HDOG_HESS_TY, kHessTZ = 16;
 constexpr int kCandTile = kHessTX * kHessTY * kHessTZ;
 inline i64 hessian_tiles(int W, int H, int D) {
   return (i64)((W + kHessTX - 1) / kHessTX) * ((H + kHessTY - 1) / kHessTY) *

@@ -23,7 +23,7 @@ namespace hdog {
 namespace {
 
 constexpr int TX = kHessTX, TY = kHessTY, ZC = kHessTZ;
-constexpr int NT = 256;
+constexpr int NT = TX * TY / 2;  // two output rows per thread
 
 __device__ __forceinline__ float g2(float hi, float lo) { return cdiff_half(hi, lo); }
 

@@ -320,6 +320,11 @@ __global__ __launch_bounds__(NT) void k_accum(const uint32_t* __restrict__ L,
 // Hessian tile folds that tile's records (voxel index + six elements, SoA) into the table,
 // gathering the label, normalized and raw intensity of each candidate voxel; no pass
 // over the non-candidate voxels.
+#ifndef HDOG_ACC_TPW
+#define HDOG_ACC_TPW 2
+#endif
+constexpr int TPW = HDOG_ACC_TPW;  // z-consecutive Hessian tiles folded per work-group
+
 template <typename TN>
 __global__ __launch_bounds__(NT) void k_accum_cand(const float* __restrict__ cand,
                                                    const uint32_t* __restrict__ cand_cnt,
@@ -328,25 +333,41 @@ __global__ __launch_bounds__(NT) void k_accum_cand(const float* __restrict__ can
                                                    const TN* __restrict__ norm,
                                                    const uint16_t* __restrict__ img,
                                                    RegionAcc* acc, uint32_t cap, uint32_t W,
-                                                   uint32_t WH, int is_ave) {
-  const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
-  const uint32_t n = cand_cnt[tile];
-  if (n == 0) return;  // uniform over the work-group
+                                                   uint32_t WH, int is_ave, unsigned plane,
+                                                   unsigned nbz) {
+  // work-group p takes the tiles (x, y, TPW z + q), q < TPW, of the Hessian tiling with
+  // `plane` tiles per z level; regions spanning those tiles pay one set of global atomics
+  const unsigned p = xcd_tile(blockIdx.x, gridDim.x);
+  const unsigned xy = p % plane, zq = p / plane;
+  uint32_t n[TPW];
+  unsigned tiles[TPW];
+  uint32_t any = 0;
+#pragma unroll
+  for (int q = 0; q < TPW; ++q) {
+    const unsigned z = zq * TPW + q;
+    tiles[q] = xy + z * plane;
+    n[q] = z < nbz ? cand_cnt[tiles[q]] : 0u;
+    any |= n[q];
+  }
+  if (any == 0) return;  // uniform over the work-group
   __shared__ LdsTable<256> tab;
   tab.init();
   const bool has_img = img != nullptr;
-  const float* t = cand + (size_t)tile * kCandTile * 7;
   __syncthreads();
-  for (uint32_t i = threadIdx.x; i < n; i += NT) {
-    const uint32_t idx = __float_as_uint(t[i]);
-    float hv[6];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) hv[k] = t[(size_t)kCandTile * (1 + k) + i];
-    const uint32_t r = rid[L[idx]];
-    const float nv = ld(norm, (i64)idx);
-    const uint32_t iv = has_img ? (uint32_t)img[idx] : 0u;
-    const uint32_t z = idx / WH, rem = idx - z * WH, y = rem / W, x = rem - y * W;
-    if (r < cap) tab.add(acc, r, x, y, z, hv, nv, iv, is_ave, has_img);
+  for (int q = 0; q < TPW; ++q) {
+    const float* t = cand + (size_t)tiles[q] * kCandTile * 7;
+    for (uint32_t i = threadIdx.x; i < n[q]; i += NT) {
+      const uint32_t idx = __float_as_uint(t[i]);
+      float hv[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) hv[k] = t[(size_t)kCandTile * (1 + k) + i];
+      const uint32_t r = rid[L[idx]];
+      const float nv = ld(norm, (i64)idx);
+      const uint32_t iv = has_img ? (uint32_t)img[idx] : 0u;
+      const uint32_t z = idx / WH, rem = idx - z * WH, y = rem / W, x = rem - y * W;
+      if (r < cap) tab.add(acc, r, x, y, z, hv, nv, iv, is_ave, has_img);
+    }
   }
   __syncthreads();
   tab.flush(acc, is_ave, has_img);
@@ -505,9 +526,12 @@ hipError_t launch_region_accum_cand(const float* cand, const uint32_t* cand_cnt,
                                     const uint32_t* labels, const uint32_t* rid, const TN* norm,
                                     const uint16_t* img, RegionAcc* acc, i64 cap, int W, int H,
                                     int is_ave, hipStream_t s) {
-  hipLaunchKernelGGL((k_accum_cand<TN>), dim3((unsigned)ntiles), dim3(NT), 0, s, cand, cand_cnt,
+  const unsigned plane = (unsigned)(((W + kHessTX - 1) / kHessTX) * ((H + kHessTY - 1) / kHessTY));
+  const unsigned nbz = (unsigned)(ntiles / plane);
+  const i64 ngroups = (i64)plane * ((nbz + TPW - 1) / TPW);
+  hipLaunchKernelGGL((k_accum_cand<TN>), dim3((unsigned)ngroups), dim3(NT), 0, s, cand, cand_cnt,
                      labels, rid, norm, img, acc, (uint32_t)std::min<i64>(cap, 0xFFFFFFFFll),
-                     (uint32_t)W, (uint32_t)((i64)W * H), is_ave);
+                     (uint32_t)W, (uint32_t)((i64)W * H), is_ave, plane, nbz);
   mark("regions/accum_candidates", s);
   return hipGetLastError();
 }

@@ -77,6 +77,11 @@ hdog_process_substack = _sig("hdog_process_substack", [_vp, _vp, ctypes.POINTER(
                                                        ctypes.POINTER(Regions)])
 hdog_run_device = _sig("hdog_run_device", [_vp, _vp, ctypes.POINTER(Params), _vp, _vp, _vp, _vp,
                                            _vp, _vp, ctypes.POINTER(_i64)])
+hdog_host_alloc = _sig("hdog_host_alloc", [_vp, ctypes.c_size_t, ctypes.POINTER(_vp)])
+hdog_host_free = _sig("hdog_host_free", [_vp, _vp])
+hdog_substack_upload = _sig("hdog_substack_upload", [_vp, ctypes.POINTER(Params), _vp, _i, _i])
+hdog_substack_run = _sig("hdog_substack_run", [_vp, _vp, ctypes.POINTER(Params), _i,
+                                               ctypes.POINTER(Regions)])
 hdog_batch_plan = _sig("hdog_batch_plan", [_vp, ctypes.POINTER(Params), ctypes.POINTER(Tile), _i,
                                            _i64, _i64])
 hdog_batch_run = _sig("hdog_batch_run", [_vp, _vp])
@@ -105,4 +110,5 @@ EXPORTED = ["hdog_create", "hdog_destroy", "hdog_last_error", "hdog_release_scra
             "hdog_process_substack", "hdog_run_device", "hdog_synth_volume",
             "hdog_set_profiling", "hdog_stage_times", "hdog_soft_forward", "hdog_soft_backward",
             "hdog_batch_plan", "hdog_batch_run", "hdog_batch_tile_info", "hdog_batch_download",
-            "hdog_batch_release"]
+            "hdog_batch_release", "hdog_host_alloc", "hdog_host_free", "hdog_substack_upload",
+            "hdog_substack_run"]

@@ -8,12 +8,15 @@
 // margin = r_z(sigma2) + extra_depth_margin; the last one is zero-filled to `depth`.
 //
 // Differences from the reference, all additive:
-//   * the next substack's slices are read on a loader thread while the GPU works on the
-//     current one (the reference reads serially, src/3D/HDoG3D_main.cpp:155-157);
+//   * slices are read into page-locked buffers by several threads, one substack ahead,
+//     and each substack's host-to-device copy runs on a copy stream while the previous
+//     substack's kernels run (the reference reads, copies and computes serially,
+//     src/3D/HDoG3D_main.cpp:155-170);
 //   * a substack step depth - 2*margin <= 0 is rejected (the reference loops forever);
 //   * a missing parameter key exits with code 2 and a message (the reference aborts);
 //   * optional HDoG_param.truncate / HDoG_param.precision keys.
 // Exit codes: 0 ok, 2 usage / parameter / file error, 1 device error.
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -116,8 +119,17 @@ int main(int argc, char** argv) {
   hp.precision = precision_code(p.precision);
 
   const size_t plane = (size_t)p.width * p.height;
-  std::vector<uint16_t> buf[2] = {std::vector<uint16_t>(plane * p.depth),
-                                  std::vector<uint16_t>(plane * p.depth)};
+  // three page-locked substack buffers: one being read from disk, one being copied to the
+  // device, one whose copy the running substack consumed
+  uint16_t* buf[3] = {nullptr, nullptr, nullptr};
+  for (auto& b : buf)
+    if (hdog_host_alloc(ctx, plane * p.depth * sizeof(uint16_t), (void**)&b) != HDOG_OK) {
+      std::cout << "Unable to allocate host buffers: " << hdog_last_error(ctx) << std::endl;
+      for (auto& q : buf) hdog_host_free(ctx, q);
+      hdog_destroy(ctx);
+      return 1;
+    }
+  const unsigned nthreads = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
   RegionHost reg;
   int rc = 0;
 
@@ -132,23 +144,44 @@ int main(int argc, char** argv) {
 
     std::vector<int> z0s;
     for (int z0 = 0; z0 < L - p.depth_margin; z0 += step) z0s.push_back(z0);
+    const size_t nz = z0s.size();
 
-    // loader: fills buf[slot] with the slices of substack k; reports failure in `err`
-    auto load = [&](size_t k, int slot, std::string* err, double* took) {
+    // loader: reads the slices of substack k into buf[k % 3] on `nthreads` threads
+    struct Load {
+      std::string err;
+      double took = 0;
+      std::thread th;
+    };
+    auto load = [&](size_t k, Load* ld) {
       auto t = Clock::now();
       const int z0 = z0s[k];
       const int d = std::min(p.depth, L - z0);
-      for (int i = 0; i < d; ++i)
-        if (!load_slice(src[z0 + i], buf[slot].data() + plane * i, plane, *err)) return;
-      *took = ms_since(t);
+      uint16_t* dst = buf[k % 3];
+      std::vector<std::string> errs(nthreads);
+      std::vector<std::thread> pool;
+      for (unsigned w = 0; w < nthreads; ++w)
+        pool.emplace_back([&, w] {
+          for (int i = (int)w; i < d; i += (int)nthreads)
+            if (!load_slice(src[z0 + i], dst + plane * i, plane, errs[w])) return;
+        });
+      for (auto& th : pool) th.join();
+      for (auto& e : errs)
+        if (!e.empty() && ld->err.empty()) ld->err = e;
+      ld->took = ms_since(t);
     };
-    std::string lerr;
-    double ltook = 0;
-    if (!z0s.empty()) load(0, 0, &lerr, &ltook);
-    for (size_t k = 0; k < z0s.size(); ++k) {
-      const int slot = (int)(k & 1);
-      if (!lerr.empty()) {
-        std::cout << lerr << std::endl;
+    auto upload = [&](size_t k) {
+      const int d = std::min(p.depth, L - z0s[k]);
+      return hdog_substack_upload(ctx, &hp, buf[k % 3], d, (int)(k & 1));
+    };
+    Load cur, next;
+    if (nz > 0) {
+      load(0, &cur);
+      if (cur.err.empty() && upload(0) != HDOG_OK) cur.err = hdog_last_error(ctx);
+    }
+    if (nz > 1) next.th = std::thread(load, 1, &next);
+    for (size_t k = 0; k < nz; ++k) {
+      if (!cur.err.empty()) {
+        std::cout << cur.err << std::endl;
         rc = 2;
         break;
       }
@@ -158,29 +191,27 @@ int main(int argc, char** argv) {
       std::cout << "processSubstack(z0=" << z0 << ", depth=" << d << ")" << std::endl;
       std::cout << "\tfirst_src_path: " << src[z0] << std::endl;
       std::cout << "\tlast_src_path: " << src[z0 + d - 1] << std::endl;
-      std::cout << "loadImage took " << ltook << "msec" << std::endl;
-      // prefetch the next substack while this one runs on the GPU
-      std::string nerr;
-      double ntook = 0;
-      std::thread pre;
-      if (k + 1 < z0s.size()) pre = std::thread(load, k + 1, 1 - slot, &nerr, &ntook);
-
+      std::cout << "loadImage took " << cur.took << "msec" << std::endl;
+      // substack k+1: its slices are read (joined here) and its copy starts before k runs,
+      // so the copy overlaps k's kernels; substack k+2 is read meanwhile
+      Load after;
+      if (k + 1 < nz) {
+        next.th.join();
+        if (next.err.empty() && upload(k + 1) != HDOG_OK) next.err = hdog_last_error(ctx);
+        if (k + 2 < nz) after.th = std::thread(load, k + 2, &after);
+      }
       auto t = Clock::now();
-      // one extra capacity probe: process, then size the host arrays to the region count
+      // the first download is a capacity probe that returns the region count
       hdog_regions probe{};
-      int e = hdog_process_substack(ctx, nullptr, &hp, buf[slot].data(), d, &probe);
-      if (e == HDOG_ERR_CAPACITY) {
-        hdog_regions v = reg.view(probe.count);
-        e = hdog_regions_download(ctx, nullptr, &v);
-        probe.count = v.count;
-      } else if (e == HDOG_OK) {
+      int e = hdog_substack_run(ctx, nullptr, &hp, (int)(k & 1), &probe);
+      if (e == HDOG_ERR_CAPACITY || e == HDOG_OK) {
         hdog_regions v = reg.view(probe.count);
         e = hdog_regions_download(ctx, nullptr, &v);
       }
       if (e != HDOG_OK) {
         std::cout << "HDoG failed: " << hdog_last_error(ctx) << std::endl;
         rc = e == HDOG_ERR_ARG ? 2 : 1;
-        if (pre.joinable()) pre.join();
+        if (after.th.joinable()) after.th.join();
         break;
       }
       const int64_t R = probe.count;
@@ -200,11 +231,16 @@ int main(int argc, char** argv) {
         std::cout << "[save] end of the stack : " << z0 + p.depth_margin << " - " << z0 + d - 1
                   << " (" << d - p.depth_margin << ")" << std::endl;
       std::cout << "saveImage took " << ms_since(t) << "msec" << std::endl;
-      if (pre.joinable()) pre.join();
-      lerr = nerr;
-      ltook = ntook;
+      cur.err = std::move(next.err);
+      cur.took = next.took;
+      next.err = std::move(after.err);
+      next.took = after.took;
+      next.th = std::move(after.th);
+      if (rc) break;
     }
+    if (next.th.joinable()) next.th.join();
   }
+  for (auto& b : buf) hdog_host_free(ctx, b);
   hdog_destroy(ctx);
   return rc;
 }

@@ -156,6 +156,26 @@ int hdog_regions_download(hdog_ctx* ctx, void* stream, hdog_regions* out);
 int hdog_process_substack(hdog_ctx* ctx, void* stream, const hdog_params* p,
                           const uint16_t* h_img, int depth_valid, hdog_regions* out);
 
+/* Split form of hdog_process_substack, to overlap the upload of the next substack with
+ * the compute of this one (the reference reads, copies and computes serially,
+ * src/3D/HDoG3D_main.cpp:155-170).  hdog_substack_upload copies depth_valid host slices
+ * (best from hdog_host_alloc memory: then the copy is asynchronous) into device slot 0 or 1
+ * on the context's own copy stream, zero-fills up to p->depth and returns at once; the
+ * host buffer must stay unchanged until the next upload into the same slot returns.
+ * hdog_substack_run makes `stream` wait for that slot's upload, runs every stage on it
+ * and downloads the regions into *out (NULL: keep them on the device), synchronizing
+ * `stream`.  Alternate the slots: upload k+1 into one while k runs from the other. */
+int hdog_substack_upload(hdog_ctx* ctx, const hdog_params* p, const uint16_t* h_img,
+                         int depth_valid, int slot);
+int hdog_substack_run(hdog_ctx* ctx, void* stream, const hdog_params* p, int slot,
+                      hdog_regions* out);
+
+/* Page-locked host memory (hipHostMalloc): host-to-device copies from it run at the full
+ * link rate and asynchronously.  No counterpart in the reference, which reads into
+ * cudaHostAlloc buffers it allocates itself (src/3D/HDoG3D_main.cpp:58-109). */
+int hdog_host_alloc(hdog_ctx* ctx, size_t bytes, void** p);
+int hdog_host_free(hdog_ctx* ctx, void* p);
+
 /* Same pipeline on a volume already resident on the device.  The intermediates are
  * written to caller buffers when the pointers are non-NULL (else context scratch is
  * used); storage types follow p->precision (enum hdog_precision).  With d_hessian NULL

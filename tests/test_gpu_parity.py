@@ -291,6 +291,58 @@ def test_process_substack_zero_fills_short_tail(hd):
     np.testing.assert_array_equal(arr["voxels"][1:], o["regions"]["count"][1:])
 
 
+def test_split_upload_and_run_match_the_oracle(hd):
+    # substacks uploaded from page-locked memory into alternating device slots while the
+    # previous one runs; the last one is short (zero-filled)
+    import ctypes
+    from hdog import _capi as C
+    p = dict(C0P, truncate=2.0)
+    shape = (24, 48, 64)
+    vols = [synth.make_volume("c0", shape=shape, n_blobs=30, seed=40 + k) for k in range(3)]
+    valid = [24, 24, 17]
+    params = hd.make_params(shape, p["rxy"], p["rz"], p["min_intensity"], p["sigma_xy1"],
+                            p["sigma_z1"], p["sigma_xy2"], p["sigma_z2"], 2.0)
+    n = int(np.prod(shape))
+    bufs = []
+    for k in range(3):
+        ptr = ctypes.c_void_p()
+        assert C.hdog_host_alloc(hd._h, n * 2, ctypes.byref(ptr)) == 0
+        h = np.ctypeslib.as_array((ctypes.c_uint16 * n).from_address(ptr.value)).reshape(shape)
+        h[:valid[k]] = vols[k][:valid[k]]
+        bufs.append((ptr, h))
+    try:
+        def up(k):
+            rc = C.hdog_substack_upload(hd._h, ctypes.byref(params), bufs[k][0], valid[k], k & 1)
+            assert rc == 0, C.hdog_last_error(hd._h)
+        up(0)
+        for k in range(3):
+            if k + 1 < 3:
+                up(k + 1)
+            padded = vols[k].copy()
+            padded[valid[k]:] = 0
+            o = oracle.substack(padded, p)["regions"]
+            R = len(o["label"])
+            lab = np.zeros(R, np.uint32)
+            cnt = np.zeros(R, np.uint64)
+            reg = C.Regions()
+            reg.capacity = R
+            reg.label, reg.voxels = lab.ctypes.data, cnt.ctypes.data
+            rc = C.hdog_substack_run(hd._h, hd._stream(), ctypes.byref(params), k & 1,
+                                     ctypes.byref(reg))
+            assert rc == 0, C.hdog_last_error(hd._h)
+            assert reg.count == R
+            np.testing.assert_array_equal(lab[1:], o["label"][1:])
+            np.testing.assert_array_equal(cnt[1:], o["count"][1:])
+        # a slot without an upload of this shape is refused
+        other = hd.make_params((8, 48, 64), p["rxy"], p["rz"], p["min_intensity"],
+                               p["sigma_xy1"], p["sigma_z1"], p["sigma_xy2"], p["sigma_z2"], 2.0)
+        assert C.hdog_substack_run(hd._h, hd._stream(), ctypes.byref(other), 0, None) == \
+            C.HDOG_ERR_STATE
+    finally:
+        for ptr, _ in bufs:
+            C.hdog_host_free(hd._h, ptr)
+
+
 def test_errors_raise(hd):
     from hdog import HDoG, HDoGError
     with pytest.raises(HDoGError):
