@@ -146,26 +146,30 @@ __global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* 
       if (e < DY * DX) d[e] = pf[k];
     }
   };
-  int dp = 0, dc = 1, dn = 2;   // dog slots of slices s-2, s-1, s
-  int h1 = 0, h2 = 1, hw = 2;   // H slots of slices s-3, s-2 and the one written at step s
+  // Both rings rotate with period 3, so the walk is unrolled by three steps whose slots
+  // are compile-time constants: every LDS access is then a per-thread offset plus an
+  // immediate.  Slots at a step: dog(s-2), dog(s-1), dog(s) in DP, DC, DN; H(s-3), H(s-2)
+  // in DP, DC and H(s-1) written into DN.
   fetch(z0 - 2);
-  put(sD[dp]);
+  put(sD[0]);
   fetch(z0 - 1);
-  put(sD[dc]);
+  put(sD[1]);
   fetch(z0);
-  put(sD[dn]);
+  put(sD[2]);
   fetch(z0 + 1);
   __syncthreads();
-  for (int s = z0; s <= z1 + 1; ++s) {
+  auto step = [&](int s, auto dp_c, auto dc_c, auto dn_c) {
+    constexpr int dp = decltype(dp_c)::value, dc = decltype(dc_c)::value,
+                  dn = decltype(dn_c)::value;
     // 1. first derivatives of slice s-1
     {
       const bool zin = s - 1 >= 0 && s - 1 < D;
       const float* Dc = sD[dc];
       const float* Dn = sD[dn];
       const float* Dp = sD[dp];
-      float* hx = sHx[hw];
-      float* hy = sHy[hw];
-      float* hz = sHz[hw];
+      float* hx = sHx[dn];
+      float* hy = sHy[dn];
+      float* hz = sHz[dn];
 #pragma unroll
       for (int j = 0; j < NHX; ++j)
         if (hxe[j])
@@ -182,14 +186,14 @@ __global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* 
     // 2. outputs of slice z = s - 2
     const int z = s - 2;
     if (z >= z0 && z < z1) {
-      const float* hxm = sHx[h1];  // Hx(z-1)
-      const float* hx0 = sHx[h2];  // Hx(z)
-      const float* hxp = sHx[hw];  // Hx(z+1)
-      const float* hym = sHy[h1];
-      const float* hy0 = sHy[h2];
-      const float* hyp = sHy[hw];
-      const float* hzm = sHz[h1];
-      const float* hzp = sHz[hw];
+      const float* hxm = sHx[dp];  // Hx(z-1)
+      const float* hx0 = sHx[dc];  // Hx(z)
+      const float* hxp = sHx[dn];  // Hx(z+1)
+      const float* hym = sHy[dp];
+      const float* hy0 = sHy[dc];
+      const float* hyp = sHy[dn];
+      const float* hzm = sHz[dp];
+      const float* hzp = sHz[dn];
       const uint32_t zb = (uint32_t)z * WH;
 #pragma unroll
       for (int k = 0; k < NOUT; ++k) {
@@ -247,8 +251,16 @@ __global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* 
     put(sD[dp]);
     fetch(s + 2);
     __syncthreads();
-    const int t = dp; dp = dc; dc = dn; dn = t;
-    const int u = h1; h1 = h2; h2 = hw; hw = u;
+  };
+  using I0 = std::integral_constant<int, 0>;
+  using I1 = std::integral_constant<int, 1>;
+  using I2 = std::integral_constant<int, 2>;
+  for (int s = z0; s <= z1 + 1; s += 3) {  // the step count is uniform over the work-group
+    step(s, I0(), I1(), I2());
+    if (s + 1 > z1 + 1) break;
+    step(s + 1, I1(), I2(), I0());
+    if (s + 2 > z1 + 1) break;
+    step(s + 2, I2(), I0(), I1());
   }
   if (cand_cnt && threadIdx.x == 0) cand_cnt[tile] = s_cnt;
 }
