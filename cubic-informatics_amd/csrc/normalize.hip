@@ -334,6 +334,18 @@ __global__ __launch_bounds__(NT) void k_box_walk(const float2* __restrict__ src,
 #pragma unroll
   for (int q = 0; q < WB; ++q) ring[q] = src[base + (i64)clampi(q - R, 0, L - 1) * sa];
   for (int c0 = 0; c0 < L; c0 += WB) {
+    // with the normalization fused, the raw intensities of this block's WB outputs are
+    // loaded up front: loading each one at its output made every output wait for all
+    // outstanding loads, the ring refills included
+    float iv[WB];
+    if (NORM) {
+      const uint16_t* ip = img + base + (i64)c0 * sa;
+#pragma unroll
+      for (int q = 0; q < WB; ++q) {
+        iv[q] = (c0 + q < L) ? (float)*ip : 0.0f;
+        ip += sa;
+      }
+    }
 #pragma unroll
     for (int q = 0; q < WB; ++q) {
       const int c = c0 + q;
@@ -348,7 +360,7 @@ __global__ __launch_bounds__(NT) void k_box_walk(const float2* __restrict__ src,
         vd *= k;
         const i64 idx = base + (i64)c * sa;
         if (NORM) {
-          const float I = (float)img[idx];
+          const float I = iv[q];
           norm[idx] = cvt<TS>((I >= min_int) ? (I - ve) / (vd - ve) : 0.0f);
         } else {
           dst[idx] = make_float2(ve, vd);
