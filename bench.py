@@ -51,6 +51,7 @@ KERNEL_BYTES = {
     "normalize/copy": lambda sn, sd, sh, cf: 2 + sn,
     "dog/gauss_x": lambda sn, sd, sh, cf: sn + 8,
     "dog/gauss_y": lambda sn, sd, sh, cf: 8 + 8,
+    "dog/gauss_xy": lambda sn, sd, sh, cf: sn + 8,
     "dog/gauss_z_dog": lambda sn, sd, sh, cf: 8 + sd,
     # mask, plus a 28-byte record (index + six elements) per candidate voxel
     "hessian/hessian_pd": lambda sn, sd, sh, cf: sd + 1 + 28 * cf,

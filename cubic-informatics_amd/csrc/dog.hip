@@ -201,6 +201,93 @@ __global__ __launch_bounds__(NT) void k_gauss_walk(const float2* __restrict__ sr
   }
 }
 
+// x and y passes fused: a one-wave work-group walks the rows of one plane for a strip of
+// 64 columns.  Each row's window x0-16 .. x0+79 goes through an LDS line (two lines,
+// alternating, one barrier per row); lane l forms the x outputs (G1_x, G2_x) of column
+// x0+l, which enter the register ring of the last 2R+1 rows exactly as in the column
+// walker, and the y outputs leave as float2.  The (G1_x, G2_x) volume of the unfused
+// passes is never stored.  Rows are fetched PF rows ahead of their use.  The Gaussian
+// weights are symmetric (set pairwise from one value, w[r - i] == w[r + i]), so tap j
+// reads w[r + |j|]: r + 1 distinct values per Gaussian stay in scalar registers.
+template <int R1, int R2, typename TS>
+__global__ __launch_bounds__(64) void k_gauss_xy_walk(const TS* __restrict__ src,
+                                                      float2* __restrict__ dst,
+                                                      const float* __restrict__ w1,
+                                                      const float* __restrict__ w2, int W, int H,
+                                                      int nxs) {
+  constexpr int R = R1 > R2 ? R1 : R2;
+  constexpr int WB = 2 * R + 1;
+  // rows fetched ahead: a divisor of WB, so the queue slot of every row is static in
+  // the loop unrolled by WB
+  constexpr int PF = WB % 3 == 0 ? 3 : WB % 5 == 0 ? 5 : WB <= 13 ? WB : 1;
+  constexpr int HX = 16, LW = 128;  // line positions 96 .. 127 stay unused (zero)
+  static_assert(R <= HX, "x window exceeds the LDS line");
+  __shared__ float line[2][LW];
+  const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int l = threadIdx.x;
+  const int x0 = (int)(tile % nxs) * 64;
+  const i64 pbase = (i64)(tile / nxs) * H * W;
+  const int xa = x0 - HX + l, xb = x0 + 64 - HX + l;  // line positions l and 64 + l
+  const bool va = xa >= 0 && xa < W, vb = l < 2 * HX && xb < W;
+  const int xac = min(max(xa, 0), W - 1), xbc = min(xb, W - 1);
+  // branch-free: every lane loads an in-bounds element (row and column clamped); zero is
+  // selected outside the plane when the row is consumed, so no load sits under an exec
+  // mask and none is waited for before its use
+  float qa[PF], qb[PF];  // row y is held in slot y % PF
+  auto fetch = [&](int y, int k) {
+    const i64 rb = pbase + (i64)min(y, H - 1) * W;
+    qa[k] = ld(src, rb + xac);
+    qb[k] = ld(src, rb + xbc);
+  };
+#pragma unroll
+  for (int k = 0; k < PF; ++k) fetch(k, k);
+  // x outputs of row y, whose slot is k; fetches row y + PF into the freed slot.  Past
+  // the plane the line holds zeros, so the outputs are +0 as the y pass's zero padding
+  // requires.
+  auto xrow = [&](int y, int k) -> float2 {
+    const float a = (va && y < H) ? qa[k] : 0.0f, b = (vb && y < H) ? qb[k] : 0.0f;
+    fetch(y + PF, k);
+    float* ln = line[y & 1];
+    ln[l] = a;
+    ln[64 + l] = b;
+    __syncthreads();
+    const float* t = ln + l + HX;
+    float s1 = 0.0f, s2 = 0.0f;
+#pragma unroll
+    for (int j = -R1; j <= R1; ++j) s1 = fmaf(w1[R1 + (j < 0 ? -j : j)], t[j], s1);
+#pragma unroll
+    for (int j = -R2; j <= R2; ++j) s2 = fmaf(w2[R2 + (j < 0 ? -j : j)], t[j], s2);
+    // pin the row's outputs here: otherwise the compiler sinks the FMA chains towards
+    // their uses and keeps many rows' LDS windows live at once
+    asm volatile("" : "+v"(s1), "+v"(s2));
+    return make_float2(s1, s2);
+  };
+  const bool out = x0 + l < W;
+  float2* dcol = dst + pbase + x0 + l;
+  float2 ring[WB];  // row p lives in ring[(p + R) % WB]; rows < 0 are zero
+#pragma unroll
+  for (int q = 0; q < R; ++q) ring[q] = make_float2(0.0f, 0.0f);
+#pragma unroll
+  for (int q = R; q < WB; ++q) ring[q] = xrow(q - R, (q - R) % PF);
+  // The plane's end leaves the kernel (a wave-uniform return) instead of skipping the
+  // remaining unrolled rows: no path that skips a row's loads merges back into the loop,
+  // so the compiler's memory-wait counts keep the PF rows in flight.
+  for (int c0 = 0;; c0 += WB) {
+#pragma unroll
+    for (int q = 0; q < WB; ++q) {
+      const int c = c0 + q;
+      if (c >= H) return;
+      float s1 = 0.0f, s2 = 0.0f;
+#pragma unroll
+      for (int j = -R1; j <= R1; ++j) s1 = fmaf(w1[R1 + (j < 0 ? -j : j)], ring[(q + j + R) % WB].x, s1);
+#pragma unroll
+      for (int j = -R2; j <= R2; ++j) s2 = fmaf(w2[R2 + (j < 0 ? -j : j)], ring[(q + j + R) % WB].y, s2);
+      if (out) dcol[(i64)c * W] = make_float2(s1, s2);
+      ring[q] = xrow(c + R + 1, (q + R + 1) % PF);  // row c + R + 1 replaces row c - R
+    }
+  }
+}
+
 // (r1, r2) pairs with compile-time kernels: truncation 4 and 2 of the benchmark sigmas
 // (2.0 / 3.2, 1.6 / 2.56) and of the reference's example sigmas (4.0 / 6.0, 1.8 / 2.7).
 #define HDOG_GAUSS_PAIRS(X) X(8, 13) X(6, 10) X(4, 6) X(3, 5) X(8, 12) X(4, 5) X(7, 11)
@@ -227,6 +314,24 @@ hipError_t gauss_x(const TS* src, float2* dst, const float* w1, const float* w2,
   hipLaunchKernelGGL((k_gauss_x<ROWS, TS>), dim3((unsigned)nblk), dim3(NT), lds, s, src, dst, w1,
                      w2, r1, r2, W, nrows, C, nct);
   return hipGetLastError();
+}
+
+// Fused x + y passes for the compile-time radius pairs; false when (r1, r2) has none.
+template <typename TS>
+bool gauss_xy(const TS* src, float2* dst, const float* w1, const float* w2, int r1, int r2,
+              int W, int H, int D, hipStream_t s, hipError_t* e) {
+#define X(A, B)                                                                              \
+  if (r1 == A && r2 == B) {                                                                  \
+    const int nxs = (W + 63) / 64;                                                           \
+    const i64 nblk = (i64)nxs * D;                                                           \
+    hipLaunchKernelGGL((k_gauss_xy_walk<A, B, TS>), dim3((unsigned)nblk), dim3(64), 0, s,    \
+                       src, dst, w1, w2, W, H, nxs);                                         \
+    *e = hipGetLastError();                                                                  \
+    return true;                                                                             \
+  }
+  HDOG_GAUSS_PAIRS(X)
+#undef X
+  return false;
 }
 
 template <bool LAST, typename TS>
@@ -269,17 +374,24 @@ hipError_t launch_dog(const TN* norm, TD* dog, void* tmp1, void* tmp2, const flo
   const float* wz1 = weights + kMaxTaps;
   const float* wxy2 = weights + 2 * kMaxTaps;
   const float* wz2 = weights + 3 * kMaxTaps;
-  hipError_t e;
-  if ((e = gauss_x<TN>(norm, (float2*)tmp1, wxy1, wxy2, r[0], r[2], W, (i64)H * D, s)) !=
-      hipSuccess)
-    return e;
-  mark("dog/gauss_x", s);
-  if ((e = gauss_line<false, TD>((const float2*)tmp1, (float2*)tmp2, nullptr, wxy1, wxy2, r[0],
-                                 r[2], 0.0f, W, H, W, WH, D, s)) != hipSuccess)
-    return e;
-  mark("dog/gauss_y", s);
-  e = gauss_line<true, TD>((const float2*)tmp2, nullptr, dog, wz1, wz2, r[1], r[3], normalizer, W,
-                           D, WH, W, H, s);
+  hipError_t e = hipSuccess;
+  const float2* gxy = (const float2*)tmp2;
+  if (gauss_xy<TN>(norm, (float2*)tmp1, wxy1, wxy2, r[0], r[2], W, H, D, s, &e)) {
+    if (e != hipSuccess) return e;
+    mark("dog/gauss_xy", s);
+    gxy = (const float2*)tmp1;
+  } else {
+    if ((e = gauss_x<TN>(norm, (float2*)tmp1, wxy1, wxy2, r[0], r[2], W, (i64)H * D, s)) !=
+        hipSuccess)
+      return e;
+    mark("dog/gauss_x", s);
+    if ((e = gauss_line<false, TD>((const float2*)tmp1, (float2*)tmp2, nullptr, wxy1, wxy2, r[0],
+                                   r[2], 0.0f, W, H, W, WH, D, s)) != hipSuccess)
+      return e;
+    mark("dog/gauss_y", s);
+  }
+  e = gauss_line<true, TD>(gxy, nullptr, dog, wz1, wz2, r[1], r[3], normalizer, W, D, WH, W, H,
+                           s);
   mark("dog/gauss_z_dog", s);
   return e;
 }
