@@ -47,6 +47,7 @@ KERNEL_BYTES = {
     "normalize/minmax_y": lambda sn, sd, sh, cf: 4 + 4,
     "normalize/minmax_x_box_x": lambda sn, sd, sh, cf: 4 + 8,
     "normalize/box_y": lambda sn, sd, sh, cf: 8 + 8,
+    "normalize/minmax_x_box_xy": lambda sn, sd, sh, cf: 4 + 8,
     "normalize/box_z_norm": lambda sn, sd, sh, cf: 8 + 2 + sn,
     "normalize/copy": lambda sn, sd, sh, cf: 2 + sn,
     "dog/gauss_x": lambda sn, sd, sh, cf: sn + 8,

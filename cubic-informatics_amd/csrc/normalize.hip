@@ -372,6 +372,150 @@ __global__ __launch_bounds__(NT) void k_box_walk(const float2* __restrict__ src,
   }
 }
 
+// min/max along x, box along x and box along y fused.  A one-wave work-group walks the
+// rows of one plane of the y-min/maxed (packed) volume for a strip of CW = 128 columns,
+// two per lane.  Per row:
+//   1. the clamped packed row segment x0-2R .. x0+CW-1+2R enters an LDS line A;
+//   2. window minima B[b] = min(A[b .. b+2R]) (the min/max at x = x0-R+b) are formed four
+//      per task (a shared window middle plus short prefix / suffix runs); on a strip at
+//      the plane's edge, positions outside [0, W) then take the value at the edge, since
+//      the box pass clamps to the edge of the filtered row;
+//   3. each lane sums B over its two windows (integers, exact in any order; v_dot2 with
+//      (1, 0) / (0, 1) adds one 16-bit half per instruction) -> (E_x, D_x) of its columns;
+//   4. these enter a register ring of the last 2R+1 rows (rows clamped), and each output
+//      row sums its window in the reference order, as the y pass walker does.
+// The (E_x, D_x) volume of the unfused passes is never stored.  Rows are fetched PF rows
+// ahead of their use; the plane's end leaves the kernel (see the DoG row walker).
+template <int R>
+__global__ __launch_bounds__(64) void k_norm_xy_walk(const uint32_t* __restrict__ src,
+                                                     float4* __restrict__ dst, int W, int H,
+                                                     int nxs, float k) {
+  constexpr int WB = 2 * R + 1;
+  constexpr int CW = 128;                    // output columns per wave
+  constexpr int HX = (2 * R + 3) / 4 * 4;    // line halo: 16-byte aligned task reads
+  constexpr int NB = CW + 2 * R;             // window minima per row
+  constexpr int NTASK = (NB + 3) / 4;        // four minima per task
+  constexpr int LW = 64 * ((4 * NTASK + 2 * R + 4 + 63) / 64);  // line, whole lane loads
+  constexpr int NLD = LW / 64;               // loads per lane per row
+  constexpr int PF = WB % 3 == 0 ? 3 : WB % 5 == 0 ? 5 : WB % 7 == 0 ? 7 : 1;
+  static_assert(HX == 2 * R, "task windows assume a halo of exactly 2R");
+  static_assert(4 * NTASK + 2 * R + 4 <= LW, "task reads stay inside the line");
+  __shared__ __align__(16) uint32_t A[LW];
+  __shared__ __align__(16) uint32_t B[4 * NTASK];
+  const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int l = threadIdx.x;
+  const int x0 = (int)(tile % nxs) * CW;
+  const uint32_t* plane = src + (i64)(tile / nxs) * H * W;
+  int xl[NLD];  // clamped column of each line position this lane loads
+#pragma unroll
+  for (int j = 0; j < NLD; ++j) xl[j] = clampi(x0 - HX + 64 * j + l, 0, W - 1);
+  uint32_t qv[PF][NLD];  // row y is held in slot y % PF
+  auto fetch = [&](int y, int s) {
+    const uint32_t* row = plane + (i64)min(y, H - 1) * W;
+#pragma unroll
+    for (int j = 0; j < NLD; ++j) qv[s][j] = row[xl[j]];
+  };
+#pragma unroll
+  for (int s = 0; s < PF; ++s) fetch(s, s);
+  const bool edge = x0 - R < 0 || x0 + CW + R > W;  // wave-uniform
+  const int blo = R - x0, bhi = W - 1 - x0 + R;     // B index of x = 0 and x = W - 1
+  const uint32_t full = (uint32_t)WB * 65535u;
+  const us2 lo1 = {1, 0}, hi1 = {0, 1};
+  // (E_x, D_x) of the lane's two columns for row y (slot s); fetches row y + PF into s
+  auto xrow = [&](int y, int s) -> float4 {
+#pragma unroll
+    for (int j = 0; j < NLD; ++j) A[64 * j + l] = qv[s][j];
+    fetch(y + PF, s);
+    __syncthreads();
+    if (l < NTASK) {
+      constexpr int NV = (2 * R + 4) / 4;
+      uint32_t w[4 * NV];
+#pragma unroll
+      for (int v = 0; v < NV; ++v) {
+        const uint4 q = *reinterpret_cast<const uint4*>(A + 4 * l + 4 * v);
+        w[4 * v] = q.x; w[4 * v + 1] = q.y; w[4 * v + 2] = q.z; w[4 * v + 3] = q.w;
+      }
+      uint32_t mid = w[3];
+#pragma unroll
+      for (int t = 4; t <= 2 * R; ++t) mid = pkmin(mid, w[t]);
+      const uint32_t l2 = w[2], l1 = pkmin(w[1], l2), l0 = pkmin(w[0], l1);
+      const uint32_t r1 = w[2 * R + 1], r2 = pkmin(r1, w[2 * R + 2]), r3 = pkmin(r2, w[2 * R + 3]);
+      *reinterpret_cast<uint4*>(B + 4 * l) =
+          make_uint4(pkmin(mid, l0), pkmin(pkmin(mid, l1), r1), pkmin(pkmin(mid, l2), r2),
+                     pkmin(mid, r3));
+    }
+    __syncthreads();
+    if (edge) {  // positions outside the plane take the edge value
+      uint32_t fix[(4 * NTASK + 63) / 64];
+#pragma unroll
+      for (int j = 0; j < (4 * NTASK + 63) / 64; ++j) {
+        const int b = 64 * j + l;
+        fix[j] = b < 4 * NTASK ? B[clampi(b, blo, bhi)] : 0u;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < (4 * NTASK + 63) / 64; ++j)
+        if (64 * j + l < 4 * NTASK) B[64 * j + l] = fix[j];
+      __syncthreads();
+    }
+    // column x0 + 2l + o sums B[2l + o .. 2l + o + 2R]
+    uint32_t v[2 * R + 2];
+#pragma unroll
+    for (int t = 0; t < R + 1; ++t) {
+      const uint2 q = *reinterpret_cast<const uint2*>(B + 2 * l + 2 * t);
+      v[2 * t] = q.x;
+      v[2 * t + 1] = q.y;
+    }
+    uint32_t se = 0, sdn = 0;
+#pragma unroll
+    for (int t = 0; t <= 2 * R; ++t) {
+      se = __builtin_amdgcn_udot2(__builtin_bit_cast(us2, v[t]), lo1, se, false);
+      sdn = __builtin_amdgcn_udot2(__builtin_bit_cast(us2, v[t]), hi1, sdn, false);
+    }
+    const uint32_t se1 = se + (v[2 * R + 1] & 0xFFFFu) - (v[0] & 0xFFFFu);
+    const uint32_t sdn1 = sdn + (v[2 * R + 1] >> 16) - (v[0] >> 16);
+    float4 o = make_float4((float)se * k, (float)(full - sdn) * k, (float)se1 * k,
+                           (float)(full - sdn1) * k);
+    // pin the row's outputs here (see the DoG row walker)
+    asm volatile("" : "+v"(o.x), "+v"(o.y), "+v"(o.z), "+v"(o.w));
+    return o;
+  };
+  const int xc = x0 + 2 * l;
+  const bool st2 = xc + 1 < W && (W & 1) == 0, st1 = xc < W;
+  float4* dq = dst + ((tile / nxs) * (i64)H * W + xc) / 2;  // valid when st2
+  float2* dp = reinterpret_cast<float2*>(dst) + (tile / nxs) * (i64)H * W + xc;
+  float4 ring[WB];  // row p lives in ring[(p + R) % WB]; rows clamp to [0, H)
+  ring[R] = xrow(0, 0);
+#pragma unroll
+  for (int q = 0; q < R; ++q) ring[q] = ring[R];
+#pragma unroll
+  for (int q = R + 1; q < WB; ++q) ring[q] = xrow(min(q - R, H - 1), (q - R) % PF);
+  for (int c0 = 0;; c0 += WB) {
+#pragma unroll
+    for (int q = 0; q < WB; ++q) {
+      const int c = c0 + q;
+      if (c >= H) return;
+      float e0 = 0.0f, d0 = 0.0f, e1 = 0.0f, d1 = 0.0f;
+#pragma unroll
+      for (int j = 0; j <= 2 * R; ++j) {  // row c - R + j -> ring[(q + j) % WB]
+        const float4 t = ring[(q + j) % WB];
+        e0 += t.x;
+        d0 += t.y;
+        e1 += t.z;
+        d1 += t.w;
+      }
+      const float4 o = make_float4(e0 * k, d0 * k, e1 * k, d1 * k);
+      if (st2) {
+        dq[(i64)c * W / 2] = o;
+      } else if (st1) {
+        dp[(i64)c * W] = make_float2(o.x, o.y);
+        if (xc + 1 < W) dp[(i64)c * W + 1] = make_float2(o.z, o.w);
+      }
+      ring[q] = xrow(min(c + R + 1, H - 1), (q + R + 1) % PF);  // replaces row c - R
+    }
+  }
+}
+
 template <typename TS>
 __global__ __launch_bounds__(NT) void k_copy_norm(const uint16_t* __restrict__ img,
                                                   TS* __restrict__ norm, float min_int, i64 N) {
@@ -504,6 +648,21 @@ hipError_t launch_normalize(const uint16_t* img, TS* norm, void* tmp1, void* tmp
   mark("normalize/minmax_z", s);
   if ((e = minmax_line(p1, false, p2, W, H, W, WH, D, rxy, s)) != hipSuccess) return e;
   mark("normalize/minmax_y", s);
+  if (rxy == 10 && !(ero && dil)) {
+    // min/max along x, box along x and box along y in one row walker -> (E, D) in tmp1,
+    // then box along z fused with normalization
+    constexpr int CW = 128;
+    const int nxs = (W + CW - 1) / CW;
+    const i64 nblk = (i64)nxs * D;
+    hipLaunchKernelGGL((k_norm_xy_walk<10>), dim3((unsigned)nblk), dim3(64), 0, s, p2,
+                       (float4*)tmp1, W, H, nxs, 1.0f / 21.0f);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    mark("normalize/minmax_x_box_xy", s);
+    e = box_line<true, TS>((const float2*)tmp1, nullptr, norm, img, min_intensity, W, D, WH, W, H,
+                           rz, s);
+    mark("normalize/box_z_norm", s);
+    return e;
+  }
   // min/max along x fused with the box x pass -> float2 (E, D) in tmp1
   {
     constexpr int RUN = 4, C = 64 * RUN, ROWS = NT / 64;

@@ -57,11 +57,24 @@ def assert_regions_match(g, o, hess):
 @pytest.mark.parametrize("shape,rxy,rz,m", [((24, 48, 40), 4, 3, 0), ((17, 33, 70), 10, 10, 300),
                                             ((5, 7, 300), 10, 2, 0), ((9, 20, 19), 0, 3, 250),
                                             ((6, 40, 40), 100, 5, 0), ((40, 3, 2), 2, 25, 0),
-                                            ((1, 1, 1), 3, 3, 0)])
+                                            ((1, 1, 1), 3, 3, 0), ((6, 45, 301), 10, 4, 0),
+                                            ((3, 130, 259), 10, 10, 100), ((4, 2, 129), 10, 1, 0),
+                                            ((2, 23, 128), 10, 2, 0), ((3, 11, 9), 10, 3, 0)])
 def test_normalize_bitexact(hd, shape, rxy, rz, m):
     img = RNG.integers(0, 4000, size=shape, dtype=np.uint16)
     got = to_np(hd.Normalize3DFilter(img, m, rxy, rz))
     np.testing.assert_array_equal(got, oracle.normalize3d(img, m, rxy, rz))
+
+
+@pytest.mark.parametrize("shape,rxy,rz", [((5, 40, 270), 10, 3), ((7, 30, 66), 3, 2)])
+def test_normalize_full_u16_range_bitexact(hd, shape, rxy, rz):
+    # values at both ends of the u16 range: the packed (v, 65535 - v) min/max and the
+    # integer box sums of 21 such values
+    img = RNG.integers(0, 65536, size=shape, dtype=np.uint16)
+    img[:, ::7, ::5] = 65535
+    img[:, 3::11, 1::4] = 0
+    got = to_np(hd.Normalize3DFilter(img, 0, rxy, rz))
+    np.testing.assert_array_equal(got, oracle.normalize3d(img, 0, rxy, rz))
 
 
 @pytest.mark.parametrize("shape,s,trunc", [((20, 36, 44), (2.0, 2.0, 3.2, 3.2), 4.0),
