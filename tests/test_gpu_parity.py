@@ -102,7 +102,8 @@ def test_hessian_bitexact(hd, shape):
 @pytest.mark.parametrize("shape,p", [((10, 20, 30), 0.3), ((4, 5, 6), 0.5), ((1, 1, 64), 0.6),
                                      ((33, 1, 7), 0.5), ((9, 64, 65), 0.45), ((2, 3, 1), 0.9),
                                      ((9, 17, 40), 0.35), ((17, 9, 33), 0.4), ((3, 2, 2), 0.7),
-                                     ((20, 3, 70), 0.4), ((8, 16, 64), 0.3), ((11, 2, 5), 0.6)])
+                                     ((20, 3, 70), 0.4), ((8, 16, 64), 0.3), ((11, 2, 5), 0.6),
+                                     ((70, 24, 96), 0.3), ((41, 19, 132), 0.2)])
 def test_ccl_exact(hd, shape, p):
     for seed in range(3):
         mask = (np.random.default_rng(seed).random(shape) < p).astype(np.uint8)
