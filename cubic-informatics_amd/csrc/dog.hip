@@ -162,7 +162,10 @@ __global__ __launch_bounds__(NT) void k_gauss_x_r(const TS* __restrict__ src,
 
 // y / z pass as a column walker: one thread per line, a register ring of the last 2R+1
 // (G1, G2) pairs (zero outside the line), unrolled by the ring period so every index is
-// static; every input is read once and no LDS is used.
+// static; every input is read once and no LDS is used.  Ring refills are loaded PF steps
+// before they enter the ring (PF divides the ring period: static queue slots), and the
+// walk leaves the kernel at the line's end so no merged control path makes the compiler
+// wait for the whole queue.
 template <int R1, int R2, bool LAST, typename TS>
 __global__ __launch_bounds__(NT) void k_gauss_walk(const float2* __restrict__ src,
                                                    float2* __restrict__ dst, TS* __restrict__ dog,
@@ -171,32 +174,48 @@ __global__ __launch_bounds__(NT) void k_gauss_walk(const float2* __restrict__ sr
                                                    int L, i64 sa, i64 so, int nxb) {
   constexpr int R = R1 > R2 ? R1 : R2;
   constexpr int WB = 2 * R + 1;
+  constexpr int PF = WB % 3 == 0 ? 3 : WB % 5 == 0 ? 5 : WB % 7 == 0 ? 7 : 1;
   const int x = (blockIdx.x % nxb) * NT + threadIdx.x;
   if (x >= W) return;
   const i64 base = (i64)(blockIdx.x / nxb) * so + x;
-  auto load = [&](int p) -> float2 {
-    return (p >= 0 && p < L) ? src[base + (i64)p * sa] : make_float2(0.0f, 0.0f);
+  // zero padding in the load itself: position p is read through a buffer descriptor of
+  // the wave's 64 columns at p (a uniform base), whose record count is 0 outside the
+  // line, so those loads return zeros and no value is tested (or waited for) when it
+  // enters the ring
+  const int lane = (int)(threadIdx.x & 63);
+  const int xw = __builtin_amdgcn_readfirstlane(x - lane);  // the wave's first column
+  const i64 bw = (i64)(blockIdx.x / nxb) * so + xw;
+  const int nrec = min(64, W - xw) * (int)sizeof(float2);
+  auto at = [&](int p) -> float2 {
+    const float2* line = src + bw + (i64)p * sa;
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float2*>(line), 0, (p >= 0 && p < L) ? nrec : 0, kBufferWord3);
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(
+                                          r, lane * (int)sizeof(float2), 0, 0));
   };
   float2 ring[WB];  // position p lives in ring[(p + R) % WB]
 #pragma unroll
-  for (int q = 0; q < WB; ++q) ring[q] = load(q - R);
-  for (int c0 = 0; c0 < L; c0 += WB) {
+  for (int q = 0; q < WB; ++q) ring[q] = at(q - R);
+  float2 pf[PF];  // the refill of step c (position c + R + 1) waits in slot c % PF
+#pragma unroll
+  for (int j = 0; j < PF; ++j) pf[j] = at(R + 1 + j);
+  for (int c0 = 0;; c0 += WB) {
 #pragma unroll
     for (int q = 0; q < WB; ++q) {
       const int c = c0 + q;
-      if (c < L) {
-        float s1 = 0.0f, s2 = 0.0f;
+      if (c >= L) return;
+      float s1 = 0.0f, s2 = 0.0f;
 #pragma unroll
-        for (int j = -R1; j <= R1; ++j) s1 = fmaf(w1[R1 - j], ring[(q + j + R) % WB].x, s1);
+      for (int j = -R1; j <= R1; ++j) s1 = fmaf(w1[R1 - j], ring[(q + j + R) % WB].x, s1);
 #pragma unroll
-        for (int j = -R2; j <= R2; ++j) s2 = fmaf(w2[R2 - j], ring[(q + j + R) % WB].y, s2);
-        const i64 idx = base + (i64)c * sa;
-        if (LAST)
-          dog[idx] = cvt<TS>((s1 - s2) * nz);
-        else
-          dst[idx] = make_float2(s1, s2);
-      }
-      ring[q] = load(c + R + 1);  // replaces position c - R
+      for (int j = -R2; j <= R2; ++j) s2 = fmaf(w2[R2 - j], ring[(q + j + R) % WB].y, s2);
+      const i64 idx = base + (i64)c * sa;
+      if (LAST)
+        dog[idx] = cvt<TS>((s1 - s2) * nz);
+      else
+        dst[idx] = make_float2(s1, s2);
+      ring[q] = pf[q % PF];  // position c + R + 1 replaces position c - R
+      pf[q % PF] = at(c + R + 1 + PF);
     }
   }
 }

@@ -16,6 +16,8 @@ constexpr int kMaxGaussRadius = 40;            // 2r+1 <= 81 taps per axis
 constexpr int kMaxTaps = 2 * kMaxGaussRadius + 1;
 constexpr int kMaxNormRadius = 200;
 constexpr int kRootBlock = 4096;  // voxels per root-numbering block (64 mask words)
+// word 3 of a raw buffer descriptor on gfx9-family GPUs (32-bit data format; no swizzle)
+constexpr int kBufferWord3 = 0x00020000;
 // Hessian tiles: 64 x 8 outputs per work-group over 16 slices.  Candidate records are kept
 // per tile in a block of kCandTile slots x 7 words (voxel index, then the six elements).
 #ifndef HDOG_HESS_TY

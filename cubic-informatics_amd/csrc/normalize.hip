@@ -320,54 +320,60 @@ __global__ __launch_bounds__(NT) void k_minmax_walk(const void* __restrict__ src
 }
 
 // Box mean: a register ring of the last 2R+1 inputs (clamp), unrolled by the ring period
-// so every index is static; each output sums its window in the reference order.
+// so every index is static; each output sums its window in the reference order.  Ring
+// refills are loaded PF steps before they enter the ring (PF divides the ring period, so
+// every queue slot is static too), and the walk leaves the kernel at the line's end
+// instead of skipping the remaining unrolled steps, so no merged control path makes the
+// compiler wait for the whole queue.
 template <int R, bool NORM, typename TS>
 __global__ __launch_bounds__(NT) void k_box_walk(const float2* __restrict__ src,
                                                  float2* __restrict__ dst, TS* __restrict__ norm,
                                                  const uint16_t* __restrict__ img, float min_int,
                                                  int W, int L, i64 sa, i64 so, int nxb, float k) {
   constexpr int WB = 2 * R + 1;
+  constexpr int PF = WB % 3 == 0 ? 3 : WB % 5 == 0 ? 5 : WB % 7 == 0 ? 7 : 1;
   const int x = (blockIdx.x % nxb) * NT + threadIdx.x;
   if (x >= W) return;
   const i64 base = (i64)(blockIdx.x / nxb) * so + x;
+  auto at = [&](int p) { return src[base + (i64)clampi(p, 0, L - 1) * sa]; };
   float2 ring[WB];  // position p lives in ring[(p + R) % WB]
 #pragma unroll
-  for (int q = 0; q < WB; ++q) ring[q] = src[base + (i64)clampi(q - R, 0, L - 1) * sa];
-  for (int c0 = 0; c0 < L; c0 += WB) {
+  for (int q = 0; q < WB; ++q) ring[q] = at(q - R);
+  float2 pf[PF];  // the refill of step c (position c + R + 1) waits in slot c % PF
+#pragma unroll
+  for (int j = 0; j < PF; ++j) pf[j] = at(R + 1 + j);
+  for (int c0 = 0;; c0 += WB) {
     // with the normalization fused, the raw intensities of this block's WB outputs are
     // loaded up front: loading each one at its output made every output wait for all
     // outstanding loads, the ring refills included
     float iv[WB];
     if (NORM) {
-      const uint16_t* ip = img + base + (i64)c0 * sa;
 #pragma unroll
-      for (int q = 0; q < WB; ++q) {
-        iv[q] = (c0 + q < L) ? (float)*ip : 0.0f;
-        ip += sa;
-      }
+      for (int q = 0; q < WB; ++q)
+        iv[q] = (float)img[base + (i64)min(c0 + q, L - 1) * sa];
     }
 #pragma unroll
     for (int q = 0; q < WB; ++q) {
       const int c = c0 + q;
-      if (c < L) {
-        float ve = 0.0f, vd = 0.0f;
+      if (c >= L) return;
+      float ve = 0.0f, vd = 0.0f;
 #pragma unroll
-        for (int j = 0; j <= 2 * R; ++j) {  // position c - R + j -> ring[(q + j) % WB]
-          ve += ring[(q + j) % WB].x;
-          vd += ring[(q + j) % WB].y;
-        }
-        ve *= k;
-        vd *= k;
-        const i64 idx = base + (i64)c * sa;
-        if (NORM) {
-          const float I = iv[q];
-          norm[idx] = cvt<TS>((I >= min_int) ? (I - ve) / (vd - ve) : 0.0f);
-        } else {
-          dst[idx] = make_float2(ve, vd);
-        }
+      for (int j = 0; j <= 2 * R; ++j) {  // position c - R + j -> ring[(q + j) % WB]
+        ve += ring[(q + j) % WB].x;
+        vd += ring[(q + j) % WB].y;
+      }
+      ve *= k;
+      vd *= k;
+      const i64 idx = base + (i64)c * sa;
+      if (NORM) {
+        const float I = iv[q];
+        norm[idx] = cvt<TS>((I >= min_int) ? (I - ve) / (vd - ve) : 0.0f);
+      } else {
+        dst[idx] = make_float2(ve, vd);
       }
       // slot of position c - R is reused for position c + R + 1
-      ring[q] = src[base + (i64)clampi(c + R + 1, 0, L - 1) * sa];
+      ring[q] = pf[q % PF];
+      pf[q % PF] = at(c + R + 1 + PF);
     }
   }
 }
