@@ -46,6 +46,32 @@ template <> __device__ __forceinline__ __hip_bfloat16 cvt<__hip_bfloat16>(float 
   return __float2bfloat16(v);
 }
 
+// Raw buffer loads of one stored element (the bits, converted by from_bits): a load whose
+// byte offset lies at or beyond the descriptor's record count returns zero.
+template <typename T> struct StoredBits { typedef uint16_t type; };
+template <> struct StoredBits<float> { typedef uint32_t type; };
+template <typename T>
+__device__ __forceinline__ typename StoredBits<T>::type buf_load(__amdgpu_buffer_rsrc_t r,
+                                                                 uint32_t off) {
+  if constexpr (sizeof(T) == 4)
+    return __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0);
+  else
+    return __builtin_amdgcn_raw_buffer_load_b16(r, (int)off, 0, 0);
+}
+template <typename T> __device__ __forceinline__ float from_bits(typename StoredBits<T>::type b);
+template <> __device__ __forceinline__ float from_bits<float>(uint32_t b) {
+  return __uint_as_float(b);
+}
+template <> __device__ __forceinline__ float from_bits<__half>(uint16_t b) {
+  return __half2float(__builtin_bit_cast(__half, b));
+}
+template <> __device__ __forceinline__ float from_bits<__hip_bfloat16>(uint16_t b) {
+  return __bfloat162float(__builtin_bit_cast(__hip_bfloat16, b));
+}
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, kBufferWord3);
+}
+
 __host__ __device__ __forceinline__ int clampi(int v, int lo, int hi) {
   return v < lo ? lo : (v > hi ? hi : v);
 }

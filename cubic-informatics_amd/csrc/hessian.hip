@@ -14,9 +14,7 @@
 // pitch N) and the mask with coalesced stores.
 //
 // Reduced-precision storage (FP16 / BF16): the elements are computed in FP32 from the
-// stored DoG and rounded only on the store; the candidate test uses the FP32 values, and
-// when a leading minor lies within 1e-3 of the sum of the magnitudes of its terms, the
-// minors are re-evaluated in FP64 from the same FP32 elements.
+// stored DoG and rounded only on the store; the candidate test uses the FP32 values.
 #include "hdog_internal.hpp"
 
 namespace hdog {
@@ -34,22 +32,6 @@ __device__ __forceinline__ uint8_t negdef32(float xx, float xy, float xz, float 
   return (xx < 0 && m2 > 0 && m3 < 0) ? 1 : 0;
 }
 
-__device__ __forceinline__ uint8_t negdef_band(float xx, float xy, float xz, float yy, float yz,
-                                               float zz) {
-  const float m2 = xx * yy - xy * xy;
-  const float m3 = xx * yy * zz + 2 * xy * yz * xz - xx * yz * yz - yy * xz * xz - zz * xy * xy;
-  const float s2 = fabsf(xx * yy) + fabsf(xy * xy);
-  const float s3 = fabsf(xx * yy * zz) + fabsf(2 * xy * yz * xz) + fabsf(xx * yz * yz) +
-                   fabsf(yy * xz * xz) + fabsf(zz * xy * xy);
-  if (fabsf(m2) < 1e-3f * s2 || fabsf(m3) < 1e-3f * s3) {
-    const double a = xx, b = xy, c = xz, d = yy, e = yz, f = zz;
-    const double n2 = a * d - b * b;
-    const double n3 = a * d * f + 2 * b * e * c - a * e * e - d * c * c - f * b * b;
-    return (a < 0 && n2 > 0 && n3 < 0) ? 1 : 0;
-  }
-  return (xx < 0 && m2 > 0 && m3 < 0) ? 1 : 0;
-}
-
 // z-streaming form.  A work-group owns a TX x TY column of outputs over ZC consecutive
 // slices z0..z1-1 and walks s = z0 .. z1+1 with three-slot LDS rings:
 //   step s: Hx(s-1), Hy(s-1) from dog(s-1) (xy halo 1), Hz(s-1) from dog(s) - dog(s-2)
@@ -61,7 +43,7 @@ __device__ __forceinline__ uint8_t negdef_band(float xx, float xy, float xz, flo
 // second pass.  Each thread's positions in the slices do not change with z, so their
 // LDS and global offsets and their bounds tests are computed once, before the walk; ring
 // slots rotate instead of being recomputed; global indices are 32-bit (N < 2^32).
-template <typename TD, typename TS, bool BAND>
+template <typename TD, typename TS>
 __global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* __restrict__ hess,
                                                 uint8_t* __restrict__ pd, int W, int H, int D,
                                                 int nbx, int nby, float* __restrict__ cand,
@@ -90,6 +72,7 @@ __global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* 
   auto inplane = [&](int gx, int gy) { return gx >= 0 && gx < W && gy >= 0 && gy < H; };
 
   // slice-invariant addressing of this thread
+  constexpr bool kRaw = !std::is_same<TD, float>::value;  // 16-bit DoG: buffer loads
   uint32_t fo[NPF];
   bool fv[NPF];
 #pragma unroll
@@ -97,7 +80,10 @@ __global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* 
     const int e = threadIdx.x + k * NT, lx = e % DX, ly = e / DX;
     const int gx = x0 - 2 + lx, gy = y0 - 2 + ly;
     fv[k] = e < DY * DX && inplane(gx, gy);
-    fo[k] = fv[k] ? (uint32_t)gy * Wu + (uint32_t)gx : 0u;
+    // FP32: element index in the slice; 16-bit: byte offset, and positions outside the
+    // plane read past the descriptor's record count
+    const uint32_t i = (uint32_t)gy * Wu + (uint32_t)gx;
+    fo[k] = kRaw ? (fv[k] ? i * (uint32_t)sizeof(TD) : 0xFFFFFFF0u) : (fv[k] ? i : 0u);
   }
   int hxi[NHX], hyi[NHY], hzi[NHZ];
   bool hxe[NHX], hxv[NHX], hye[NHY], hyv[NHY], hzv[NHZ];
@@ -131,19 +117,33 @@ __global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* 
     po[k] = ov[k] ? (uint32_t)gy * Wu + (uint32_t)gx : 0u;
   }
 
-  // dog slices go global -> registers one step ahead -> LDS ring
-  float pf[NPF];
+  // dog slices go global -> registers one step ahead -> LDS ring.  FP32: loads under
+  // the position's mask.  16-bit storage: raw buffer loads through a descriptor of the
+  // slice (a position outside the plane has an offset past the record count, a slice
+  // outside the volume has no records, so both read zero), converted when the slice is
+  // written to LDS; converting at the load made every load wait where it was issued.
+  // (Measured at 2160 x 2560 x 512: each form is fastest with its own fetch.)
+  typename StoredBits<TD>::type pf[NPF];
+  const uint32_t slice_bytes = WH * (uint32_t)sizeof(TD);
   auto fetch = [&](int sl) {
     const bool zin = sl >= 0 && sl < D;
-    const uint32_t base = zin ? (uint32_t)sl * WH : 0u;
+    if constexpr (kRaw) {
+      const __amdgpu_buffer_rsrc_t r =
+          buf_rsrc(dog + (size_t)(zin ? sl : 0) * WH, zin ? slice_bytes : 0u);
 #pragma unroll
-    for (int k = 0; k < NPF; ++k) pf[k] = (zin && fv[k]) ? ld(dog, (i64)(base + fo[k])) : 0.0f;
+      for (int k = 0; k < NPF; ++k) pf[k] = buf_load<TD>(r, fo[k]);
+    } else {
+      const uint32_t base = zin ? (uint32_t)sl * WH : 0u;
+#pragma unroll
+      for (int k = 0; k < NPF; ++k)
+        pf[k] = (zin && fv[k]) ? __float_as_uint(ld(dog, (i64)(base + fo[k]))) : 0u;
+    }
   };
   auto put = [&](float* d) {
 #pragma unroll
     for (int k = 0; k < NPF; ++k) {
       const int e = threadIdx.x + k * NT;
-      if (e < DY * DX) d[e] = pf[k];
+      if (e < DY * DX) d[e] = from_bits<TD>(pf[k]);
     }
   };
   // Both rings rotate with period 3, so the walk is unrolled by three steps whose slots
@@ -221,8 +221,7 @@ __global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* 
             hess[4 * N + idx] = cvt<TS>(yz);
             hess[5 * N + idx] = cvt<TS>(zz);
           }
-          const uint8_t m =
-              BAND ? negdef_band(xx, xy, xz, yy, yz, zz) : negdef32(xx, xy, xz, yy, yz, zz);
+          const uint8_t m = negdef32(xx, xy, xz, yy, yz, zz);
           pd[idx] = m;
           is_cand = m != 0 && rec != nullptr;
           if (is_cand) {
@@ -272,9 +271,9 @@ hipError_t launch_hessian(const TD* dog, TH* hess, uint8_t* pd, int W, int H, in
                           hipStream_t s, float* cand, uint32_t* cand_cnt) {
   const int nbx = (W + TX - 1) / TX, nby = (H + TY - 1) / TY, nbz = (D + ZC - 1) / ZC;
   const i64 nblk = (i64)nbx * nby * nbz;
-  // reduced-precision storage anywhere on the path: FP64 re-evaluation band for the minors
-  constexpr bool band = !std::is_same<TD, float>::value || !std::is_same<TH, float>::value;
-  hipLaunchKernelGGL((k_hessian<TD, TH, band>), dim3((unsigned)nblk), dim3(NT), 0, s, dog, hess,
+  // slices are read through buffer descriptors with 32-bit byte offsets
+  if ((i64)W * H * (i64)sizeof(TD) >= 0xFFFFFFF0ll) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_hessian<TD, TH>), dim3((unsigned)nblk), dim3(NT), 0, s, dog, hess,
                      pd, W, H, D, nbx, nby, cand, cand_cnt);
   mark("hessian/hessian_pd", s);
   return hipGetLastError();
