@@ -14,7 +14,9 @@
 // pitch N) and the mask with coalesced stores.
 //
 // Reduced-precision storage (FP16 / BF16): the elements are computed in FP32 from the
-// stored DoG and rounded only on the store; the candidate test uses the FP32 values.
+// stored DoG and rounded only on the store; the candidate test uses the FP32 values, and
+// when a leading minor lies within 1e-5 of the sum of the magnitudes of its terms, the
+// minors are re-evaluated in FP64 from the same FP32 elements.
 #include "hdog_internal.hpp"
 
 namespace hdog {
@@ -32,6 +34,32 @@ __device__ __forceinline__ uint8_t negdef32(float xx, float xy, float xz, float 
   return (xx < 0 && m2 > 0 && m3 < 0) ? 1 : 0;
 }
 
+// Reduced-precision forms: a leading minor within kBand of the sum of its terms'
+// magnitudes has its sign re-decided in FP64 from the same FP32 elements.  The FP32
+// evaluation error of these minors is at most about 6 * 2^-24 (3.6e-7) of that sum (two
+// or three rounded products per term, four rounded additions), so a band of 1e-5 covers
+// every sign the FP32 rounding can flip, with a margin of about 28.  The FP32 minors and
+// the term magnitudes share their products with negdef32's evaluation (same order, same
+// roundings).
+constexpr float kBand = 1e-5f;
+__device__ __forceinline__ uint8_t negdef_band(float xx, float xy, float xz, float yy, float yz,
+                                               float zz) {
+  const float p1 = xx * yy, p2 = xy * xy;
+  const float m2 = p1 - p2;
+  const float t1 = p1 * zz, t2 = 2 * xy * yz * xz, t3 = xx * yz * yz, t4 = yy * xz * xz,
+              t5 = zz * xy * xy;
+  const float m3 = t1 + t2 - t3 - t4 - t5;
+  const float s2 = fabsf(p1) + fabsf(p2);
+  const float s3 = fabsf(t1) + fabsf(t2) + fabsf(t3) + fabsf(t4) + fabsf(t5);
+  if (fabsf(m2) < kBand * s2 || fabsf(m3) < kBand * s3) {
+    const double a = xx, b = xy, c = xz, d = yy, e = yz, f = zz;
+    const double n2 = a * d - b * b;
+    const double n3 = a * d * f + 2 * b * e * c - a * e * e - d * c * c - f * b * b;
+    return (a < 0 && n2 > 0 && n3 < 0) ? 1 : 0;
+  }
+  return (xx < 0 && m2 > 0 && m3 < 0) ? 1 : 0;
+}
+
 // z-streaming form.  A work-group owns a TX x TY column of outputs over ZC consecutive
 // slices z0..z1-1 and walks s = z0 .. z1+1 with three-slot LDS rings:
 //   step s: Hx(s-1), Hy(s-1) from dog(s-1) (xy halo 1), Hz(s-1) from dog(s) - dog(s-2)
@@ -43,7 +71,7 @@ __device__ __forceinline__ uint8_t negdef32(float xx, float xy, float xz, float 
 // second pass.  Each thread's positions in the slices do not change with z, so their
 // LDS and global offsets and their bounds tests are computed once, before the walk; ring
 // slots rotate instead of being recomputed; global indices are 32-bit (N < 2^32).
-template <typename TD, typename TS>
+template <typename TD, typename TS, bool BAND>
 __global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* __restrict__ hess,
                                                 uint8_t* __restrict__ pd, int W, int H, int D,
                                                 int nbx, int nby, float* __restrict__ cand,
@@ -72,7 +100,7 @@ __global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* 
   auto inplane = [&](int gx, int gy) { return gx >= 0 && gx < W && gy >= 0 && gy < H; };
 
   // slice-invariant addressing of this thread
-  constexpr bool kRaw = !std::is_same<TD, float>::value;  // 16-bit DoG: buffer loads
+  constexpr bool kRaw = !std::is_same<TD, float>::value || BAND;  // buffer-load fetch
   uint32_t fo[NPF];
   bool fv[NPF];
 #pragma unroll
@@ -221,7 +249,8 @@ __global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* 
             hess[4 * N + idx] = cvt<TS>(yz);
             hess[5 * N + idx] = cvt<TS>(zz);
           }
-          const uint8_t m = negdef32(xx, xy, xz, yy, yz, zz);
+          const uint8_t m =
+              BAND ? negdef_band(xx, xy, xz, yy, yz, zz) : negdef32(xx, xy, xz, yy, yz, zz);
           pd[idx] = m;
           is_cand = m != 0 && rec != nullptr;
           if (is_cand) {
@@ -273,7 +302,9 @@ hipError_t launch_hessian(const TD* dog, TH* hess, uint8_t* pd, int W, int H, in
   const i64 nblk = (i64)nbx * nby * nbz;
   // slices are read through buffer descriptors with 32-bit byte offsets
   if ((i64)W * H * (i64)sizeof(TD) >= 0xFFFFFFF0ll) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((k_hessian<TD, TH>), dim3((unsigned)nblk), dim3(NT), 0, s, dog, hess,
+  // reduced-precision storage anywhere on the path: FP64 re-evaluation band for the minors
+  constexpr bool band = !std::is_same<TD, float>::value || !std::is_same<TH, float>::value;
+  hipLaunchKernelGGL((k_hessian<TD, TH, band>), dim3((unsigned)nblk), dim3(NT), 0, s, dog, hess,
                      pd, W, H, D, nbx, nby, cand, cand_cnt);
   mark("hessian/hessian_pd", s);
   return hipGetLastError();
