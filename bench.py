@@ -54,8 +54,15 @@ KERNEL_BYTES = {
     "dog/gauss_y": lambda sn, sd, sh, cf: 8 + 8,
     "dog/gauss_xy": lambda sn, sd, sh, cf: sn + 8,
     "dog/gauss_z_dog": lambda sn, sd, sh, cf: 8 + sd,
-    # mask, plus a 28-byte record (index + six elements) per candidate voxel
-    "hessian/hessian_pd": lambda sn, sd, sh, cf: sd + 1 + 28 * cf,
+    # candidate bit mask, plus a 28-byte record (index + six elements) per candidate voxel
+    "hessian/hessian_pd": lambda sn, sd, sh, cf: sd + 0.125 + 28 * cf,
+    # per candidate: record index, parent entry, record label (+ the tile's mask rows)
+    "ccl/cand_local": lambda sn, sd, sh, cf: 0.125 + 12 * cf,
+    # per candidate: record index and label (rim neighbours are L2 reads of the same data)
+    "ccl/cand_merge": lambda sn, sd, sh, cf: 8 * cf,
+    # root bits cleared; per candidate: index, label read and written, parent read
+    "ccl/cand_flatten": lambda sn, sd, sh, cf: 0.125 + 16 * cf,
+    "regions/count_root_bits": lambda sn, sd, sh, cf: 0.125,
     "ccl/local": lambda sn, sd, sh, cf: 1 + 4, "ccl/merge": lambda sn, sd, sh, cf: 1,
     "ccl/init": lambda sn, sd, sh, cf: 1 + 4, "ccl/union": lambda sn, sd, sh, cf: 1,
     "ccl/flatten": lambda sn, sd, sh, cf: 4 + 4,
@@ -82,11 +89,13 @@ def pipeline_bytes(form):
 
 def record_bytes(form, cf):
     """The same boundaries as the step runs them, with the Hessian passed on as 28-byte
-    records of the candidate voxels (fraction cf) instead of six volumes: img 2 + norm
-    (write, read by DoG) 2 sn + dog (write, read by the Hessian pass) 2 sd + mask 2 +
-    labels 8 + per candidate: record write and read 56, label 4, norm sn, img 2."""
+    records of the candidate voxels (fraction cf) instead of six volumes and labeling done
+    on the candidates only: img 2 + norm (write, read by DoG) 2 sn + dog (write, read by
+    the Hessian pass) 2 sd + candidate bit mask (write, read) and root bits 0.375 + per
+    candidate: record write and read 56, parent entry and record label (written, read,
+    rewritten, read) 20, region id 4, norm sn, img 2."""
     sn, sd, sh = FORM_SIZES[form]
-    return 2 + 2 * sn + 2 * sd + 2 + 8 + cf * (56 + 4 + sn + 2)
+    return 2 + 2 * sn + 2 * sd + 0.375 + cf * (56 + 20 + 4 + sn + 2)
 
 
 C1 = dict(name="c1", rxy=10, rz=10, min_intensity=0, sigma_xy1=2.0, sigma_z1=1.6,
@@ -119,9 +128,13 @@ def run_form(hd, img, shape, precision, steps, warmup, keep_pd=False):
                             C1["sigma_z1"], C1["sigma_xy2"], C1["sigma_z2"], C1["truncate"],
                             C1["gamma"], False, precision)
     bufs = hd.alloc_intermediates(shape, precision)
-    # the step's product is the region arrays: the Hessian volumes are not requested, so the
-    # library does not materialize them (the region pass reads candidate records)
-    del bufs["hess"]
+    # the step's product is the region arrays: the Hessian volumes, the label volume and
+    # (unless the forms' candidate sets are compared) the byte mask are not requested, so
+    # the library does not materialize them (labeling and the region pass work from the
+    # candidate records and the candidate bit mask)
+    del bufs["hess"], bufs["labels"]
+    if not keep_pd:
+        del bufs["pd"]
     hd.set_profiling(False)
     R = 0
     for i in range(warmup):
@@ -147,6 +160,9 @@ def run_form(hd, img, shape, precision, steps, warmup, keep_pd=False):
         for name, ms in hd.stage_times():
             kms[name].append(ms)
     hd.set_profiling(False)
+    if "pd" not in bufs:  # one more, untimed pass for the candidate count
+        bufs["pd"] = torch.empty(shape, dtype=torch.uint8, device=img.device)
+        hd.run_device(img, params, bufs)
     pd = bufs["pd"].clone() if keep_pd else None
     cand = int(bufs["pd"].sum(dtype=torch.int64).item())
     del bufs

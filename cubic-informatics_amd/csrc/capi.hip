@@ -22,7 +22,7 @@ struct BatchState {
   std::vector<hdog_tile> tiles;
   hdog_params p{};
   Buf tmp1, tmp2, norm, dog, pd, labels, rid, bcnt, rmask, dR, acc, weights, tout, running, cand,
-      cand_cnt;
+      cand_cnt, fgbits, cand_lab;
   Buf a_label, a_size, a_vox, a_cen, a_eig, a_nrm, a_hess, a_isum, a_imax;
   i64 acc_cap = 0, arena_cap = 0;
   hipStream_t cs = nullptr;
@@ -36,7 +36,8 @@ struct BatchState {
 struct hdog_ctx {
   int device = 0;
   std::string err;
-  Buf tmp1, tmp2, rid, bcnt, rmask, dR, acc, weights, blobs, sw, sums, ddog, cand, cand_cnt;
+  Buf tmp1, tmp2, rid, bcnt, rmask, dR, acc, weights, blobs, sw, sums, ddog, cand, cand_cnt,
+      fgbits, cand_lab;
   Buf o_label, o_size, o_vox, o_cen, o_eig, o_nrm, o_hess, o_isum, o_imax;
   i64 out_cap = 0;
   Buf img, norm, dog, hess, pd, labels;  // substack buffers of hdog_process_substack
@@ -147,12 +148,15 @@ int ensure_regions(hdog_ctx* ctx, i64 R) {
 struct CandRecords {
   const float* rec = nullptr;
   const uint32_t* cnt = nullptr;
+  const uint32_t* rlab = nullptr;  // each record's label (launch_ccl_candidates)
   i64 ntiles = 0;
 };
 
-// With cand.rec the region sums come from the candidate records (hess unused), else from
-// the materialized Hessian.  flatten: `labels` come from launch_ccl(flatten = false) and
-// are finished here (written); otherwise they are final and only read.
+// With cand.rec the components come from launch_ccl_candidates (their root bits already
+// in ctx->rmask, each record's label in cand.rlab) and the region sums from the records
+// (labels and hess unused); else from the label volume and the materialized Hessian.
+// flatten: `labels` come from launch_ccl(flatten = false) and are finished here (written);
+// otherwise they are final and only read.
 template <typename TH, typename TN>
 int regional_features_t(hdog_ctx* ctx, hipStream_t s, uint32_t* labels, const TH* hess,
                         const TN* norm, const uint16_t* img, int W, int H, int D, int is_ave,
@@ -166,8 +170,12 @@ int regional_features_t(hdog_ctx* ctx, hipStream_t s, uint32_t* labels, const TH
   HIPCHK(ensure(ctx->dR, 64));
   i64 nbo = 0;
   unsigned long long* masks = (unsigned long long*)ctx->rmask.p;
-  HIPCHK(launch_region_ids(labels, (uint32_t*)ctx->rid.p, (uint32_t*)ctx->bcnt.p, masks,
-                           (uint32_t*)ctx->dR.p, N, s, &nbo, flatten));
+  if (cand.rec)
+    HIPCHK(launch_region_count_masks(masks, (uint32_t*)ctx->bcnt.p, (uint32_t*)ctx->dR.p, N, s,
+                                     &nbo));
+  else
+    HIPCHK(launch_region_ids(labels, (uint32_t*)ctx->rid.p, (uint32_t*)ctx->bcnt.p, masks,
+                             (uint32_t*)ctx->dR.p, N, s, &nbo, flatten));
   HIPCHK(hipMemcpyAsync(ctx->h_R, ctx->dR.p, 4, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   const i64 R = *ctx->h_R;
@@ -177,7 +185,7 @@ int regional_features_t(hdog_ctx* ctx, hipStream_t s, uint32_t* labels, const TH
   HIPCHK(launch_region_assign(masks, (const uint32_t*)ctx->bcnt.p, (uint32_t*)ctx->rid.p, acc, R, R,
                               nullptr, N, s));
   if (cand.rec)
-    HIPCHK(launch_region_accum_cand<TN>(cand.rec, cand.cnt, cand.ntiles, labels,
+    HIPCHK(launch_region_accum_cand<TN>(cand.rec, cand.cnt, cand.ntiles, cand.rlab,
                                         (const uint32_t*)ctx->rid.p, norm, img, acc, R, W, H,
                                         is_ave, s));
   else
@@ -203,9 +211,12 @@ int regional_features_t(hdog_ctx* ctx, hipStream_t s, uint32_t* labels, const TH
   return HDOG_OK;
 }
 
+// pd and hess may be NULL (not materialized); labels is the union-find's parent array and
+// becomes the full label volume only with `dense`.
 template <typename TN, typename TD, typename TH>
 int run_stages(hdog_ctx* ctx, hipStream_t s, const hdog_params* p, const uint16_t* img, TN* norm,
-               TD* dog, TH* hess, uint8_t* pd, uint32_t* labels, int64_t* num_regions) {
+               TD* dog, TH* hess, uint8_t* pd, uint32_t* labels, bool dense,
+               int64_t* num_regions) {
   const int W = p->width, H = p->height, D = p->depth;
   const i64 N = (i64)W * H * D;
   HIPCHK(ensure(ctx->tmp1, (size_t)N * 8));
@@ -222,19 +233,28 @@ int run_stages(hdog_ctx* ctx, hipStream_t s, const hdog_params* p, const uint16_
   const float nz = powf(ctx->sigma[0], p->gamma * 2) * powf(ctx->sigma[1], p->gamma);
   HIPCHK(launch_dog<TN, TD>(norm, dog, ctx->tmp1.p, ctx->tmp2.p, (const float*)ctx->weights.p,
                             ctx->r, nz, W, H, D, s));
-  // the Hessian pass also leaves compact candidate records, which the region pass reads
-  // instead of the six element volumes (those are written only when hess is non-NULL)
+  // the Hessian pass also leaves compact candidate records and the candidate bit mask;
+  // labeling and the region pass work from those, not from the mask and element volumes
+  // (written only when pd / hess are non-NULL)
   CandRecords cand;
   cand.ntiles = hessian_tiles(W, H, D);
+  const i64 nb = (N + kRootBlock - 1) / kRootBlock;
   HIPCHK(ensure(ctx->cand, (size_t)cand.ntiles * kCandTile * 7 * 4));
   HIPCHK(ensure(ctx->cand_cnt, (size_t)cand.ntiles * 4));
+  HIPCHK(ensure(ctx->cand_lab, (size_t)cand.ntiles * kCandTile * 4));
+  HIPCHK(ensure(ctx->fgbits, (size_t)fg_words(W, H, D) * 8));
+  HIPCHK(ensure(ctx->rmask, (size_t)nb * (kRootBlock / 8)));
   cand.rec = (const float*)ctx->cand.p;
   cand.cnt = (const uint32_t*)ctx->cand_cnt.p;
+  cand.rlab = (const uint32_t*)ctx->cand_lab.p;
   HIPCHK(launch_hessian<TD, TH>(dog, hess, pd, W, H, D, s, (float*)ctx->cand.p,
                                 (uint32_t*)ctx->cand_cnt.p));
-  HIPCHK(launch_ccl(pd, labels, W, H, D, s, false));  // flattened by the region numbering
+  HIPCHK(launch_ccl_candidates(cand.rec, cand.cnt, cand.ntiles,
+                               (unsigned long long*)ctx->fgbits.p, labels,
+                               (uint32_t*)ctx->cand_lab.p, (unsigned long long*)ctx->rmask.p,
+                               nb * (kRootBlock / 64), dense, W, H, D, s));
   int rc = regional_features_t<TH, TN>(ctx, s, labels, hess, norm, img, W, H, D, p->is_ave_mode,
-                                       num_regions, cand, true);
+                                       num_regions, cand);
   if (rc) return rc;
   ctx->ev_valid = ctx->prof;
   return HDOG_OK;
@@ -307,8 +327,7 @@ void batch_free(BatchState* B) {
   if (B->cs) (void)hipStreamDestroy(B->cs);
   if (B->h_tout) (void)hipHostFree(B->h_tout);
   for (Buf* b : {&B->tmp1, &B->tmp2, &B->norm, &B->dog, &B->pd, &B->labels, &B->rid, &B->rmask,
-                 &B->cand,
-                 &B->cand_cnt,
+                 &B->cand, &B->cand_cnt, &B->cand_lab, &B->fgbits,
                  &B->bcnt, &B->dR, &B->acc, &B->weights, &B->tout, &B->running, &B->a_label,
                  &B->a_size, &B->a_vox, &B->a_cen, &B->a_eig, &B->a_nrm, &B->a_hess, &B->a_isum,
                  &B->a_imax})
@@ -327,8 +346,7 @@ hipError_t batch_tile(hdog_ctx* ctx, BatchState* B, int t, hipStream_t s) {
   TN* norm = (TN*)B->norm.p;
   TD* dog = (TD*)B->dog.p;
   TH* hess = nullptr;  // not materialized: the region pass reads the candidate records
-  uint8_t* pd = (uint8_t*)B->pd.p;
-  uint32_t* labels = (uint32_t*)B->labels.p;
+  uint32_t* labels = (uint32_t*)B->labels.p;  // parent array, candidate voxels only
   hipError_t e;
   if ((e = launch_normalize<TN>(T.image, norm, B->tmp1.p, B->tmp2.p, p->min_intensity, W, H, D,
                                 p->radius_xy, p->radius_z, s)) != hipSuccess)
@@ -338,16 +356,19 @@ hipError_t batch_tile(hdog_ctx* ctx, BatchState* B, int t, hipStream_t s) {
                               nz, W, H, D, s)) != hipSuccess)
     return e;
   const i64 ntiles = hessian_tiles(W, H, D);
-  if ((e = launch_hessian<TD, TH>(dog, hess, pd, W, H, D, s, (float*)B->cand.p,
+  if ((e = launch_hessian<TD, TH>(dog, hess, nullptr, W, H, D, s, (float*)B->cand.p,
                                   (uint32_t*)B->cand_cnt.p)) != hipSuccess)
     return e;
-  if ((e = launch_ccl(pd, labels, W, H, D, s, false)) != hipSuccess) return e;
   i64 nbo = 0;
   uint32_t* dR = (uint32_t*)B->dR.p;
   unsigned long long* masks = (unsigned long long*)B->rmask.p;
-  if ((e = launch_region_ids(labels, (uint32_t*)B->rid.p, (uint32_t*)B->bcnt.p, masks, dR, N, s,
-                             &nbo, true)) !=
-      hipSuccess)
+  const i64 nb = (N + kRootBlock - 1) / kRootBlock;
+  if ((e = launch_ccl_candidates((const float*)B->cand.p, (const uint32_t*)B->cand_cnt.p, ntiles,
+                                 (unsigned long long*)B->fgbits.p, labels,
+                                 (uint32_t*)B->cand_lab.p, masks, nb * (kRootBlock / 64), false, W,
+                                 H, D, s)) != hipSuccess)
+    return e;
+  if ((e = launch_region_count_masks(masks, (uint32_t*)B->bcnt.p, dR, N, s, &nbo)) != hipSuccess)
     return e;
   TileOut* tout = (TileOut*)B->tout.p + t;
   if ((e = launch_tile_reserve(dR, B->acc_cap, B->arena_cap, (u64*)B->running.p, tout, s)) !=
@@ -358,7 +379,8 @@ hipError_t batch_tile(hdog_ctx* ctx, BatchState* B, int t, hipStream_t s) {
                                 B->acc_cap, B->acc_cap, dR, N, s)) != hipSuccess)
     return e;
   if ((e = launch_region_accum_cand<TN>((const float*)B->cand.p, (const uint32_t*)B->cand_cnt.p,
-                                        ntiles, labels, (const uint32_t*)B->rid.p, norm, T.image,
+                                        ntiles, (const uint32_t*)B->cand_lab.p,
+                                        (const uint32_t*)B->rid.p, norm, T.image,
                                         acc, B->acc_cap, W, H, p->is_ave_mode, s)) != hipSuccess)
     return e;
   const i64 cap = B->arena_cap;
@@ -423,7 +445,7 @@ int hdog_release_scratch(hdog_ctx* ctx) {
   if (!ctx) return HDOG_ERR_ARG;
   (void)hipSetDevice(ctx->device);
   for (Buf* b : {&ctx->tmp1, &ctx->tmp2, &ctx->rid, &ctx->bcnt, &ctx->rmask, &ctx->dR, &ctx->acc,
-                 &ctx->blobs,
+                 &ctx->blobs, &ctx->cand_lab, &ctx->fgbits,
                  &ctx->sw, &ctx->sums, &ctx->ddog,
                  &ctx->o_label, &ctx->o_size, &ctx->o_vox, &ctx->o_cen, &ctx->o_eig, &ctx->o_nrm,
                  &ctx->o_hess, &ctx->o_isum, &ctx->o_imax, &ctx->img, &ctx->norm, &ctx->dog,
@@ -664,8 +686,10 @@ int hdog_run_device(hdog_ctx* ctx, void* stream, const hdog_params* p, const uin
   const size_t ed = (p->precision == HDOG_FP16 || p->precision == HDOG_BF16) ? 2 : 4;  // dog
   if (!d_norm) { HIPCHK(ensure(ctx->norm, N * es)); d_norm = ctx->norm.p; }
   if (!d_dog) { HIPCHK(ensure(ctx->dog, N * ed)); d_dog = ctx->dog.p; }
-  // no Hessian buffer requested: it is not materialized (the region pass recomputes it)
-  if (!d_pd) { HIPCHK(ensure(ctx->pd, N)); d_pd = (uint8_t*)ctx->pd.p; }
+  // no Hessian or mask buffer requested: not materialized (the region pass works from the
+  // candidate records); no label buffer: the union-find's parent array is context scratch,
+  // written and read at candidate voxels only
+  const bool dense = d_labels != nullptr;
   if (!d_labels) { HIPCHK(ensure(ctx->labels, N * 4)); d_labels = (uint32_t*)ctx->labels.p; }
   hipStream_t s = (hipStream_t)stream;
   typedef __half h16;
@@ -673,19 +697,19 @@ int hdog_run_device(hdog_ctx* ctx, void* stream, const hdog_params* p, const uin
   switch (p->precision) {
     case HDOG_FP16:
       return run_stages<h16, h16, h16>(ctx, s, p, d_img, (h16*)d_norm, (h16*)d_dog, (h16*)d_hessian,
-                                       d_pd, d_labels, num_regions);
+                                       d_pd, d_labels, dense, num_regions);
     case HDOG_BF16:
       return run_stages<b16, b16, b16>(ctx, s, p, d_img, (b16*)d_norm, (b16*)d_dog, (b16*)d_hessian,
-                                       d_pd, d_labels, num_regions);
+                                       d_pd, d_labels, dense, num_regions);
     case HDOG_FP16_DOG32:
       return run_stages<h16, float, h16>(ctx, s, p, d_img, (h16*)d_norm, (float*)d_dog,
-                                         (h16*)d_hessian, d_pd, d_labels, num_regions);
+                                         (h16*)d_hessian, d_pd, d_labels, dense, num_regions);
     case HDOG_BF16_DOG32:
       return run_stages<b16, float, b16>(ctx, s, p, d_img, (b16*)d_norm, (float*)d_dog,
-                                         (b16*)d_hessian, d_pd, d_labels, num_regions);
+                                         (b16*)d_hessian, d_pd, d_labels, dense, num_regions);
     default:
       return run_stages<float, float, float>(ctx, s, p, d_img, (float*)d_norm, (float*)d_dog,
-                                             (float*)d_hessian, d_pd, d_labels, num_regions);
+                                             (float*)d_hessian, d_pd, d_labels, dense, num_regions);
   }
 }
 
@@ -840,7 +864,7 @@ int hdog_batch_plan(hdog_ctx* ctx, const hdog_params* p, const hdog_tile* tiles,
     return fail(ctx, HDOG_ERR_ARG, "negative capacity");
   HIPCHK(hipSetDevice(ctx->device));
   hdog_batch_release(ctx);
-  i64 maxN = 0, totN = 0, maxT = 0;
+  i64 maxN = 0, totN = 0, maxT = 0, maxF = 0;
   for (int t = 0; t < n_tiles; ++t) {
     hdog_params q = *p;
     q.width = tiles[t].width;
@@ -852,6 +876,7 @@ int hdog_batch_plan(hdog_ctx* ctx, const hdog_params* p, const hdog_tile* tiles,
     const i64 N = (i64)q.width * q.height * q.depth;
     maxN = std::max(maxN, N);
     maxT = std::max(maxT, hessian_tiles(q.width, q.height, q.depth));
+    maxF = std::max(maxF, fg_words(q.width, q.height, q.depth));
     totN += N;
   }
   int rc = ensure_gauss(ctx, p);
@@ -870,7 +895,7 @@ int hdog_batch_plan(hdog_ctx* ctx, const hdog_params* p, const hdog_tile* tiles,
     size_t bytes;
   } need[] = {{&B->tmp1, (size_t)maxN * 8}, {&B->tmp2, (size_t)maxN * 8},
               {&B->norm, maxN * es}, {&B->dog, maxN * ed},
-              {&B->pd, (size_t)maxN}, {&B->labels, (size_t)maxN * 4}, {&B->rid, (size_t)maxN * 4},
+              {&B->labels, (size_t)maxN * 4}, {&B->rid, (size_t)maxN * 4},
               {&B->bcnt, (size_t)((maxN + kRootBlock - 1) / kRootBlock) * 4},
               {&B->rmask, (size_t)((maxN + kRootBlock - 1) / kRootBlock) * (kRootBlock / 8)},
               {&B->dR, 64},
@@ -881,7 +906,8 @@ int hdog_batch_plan(hdog_ctx* ctx, const hdog_params* p, const hdog_tile* tiles,
               {&B->a_eig, (size_t)cap * 8}, {&B->a_nrm, (size_t)cap * 4},
               {&B->a_hess, (size_t)cap * 24}, {&B->a_isum, (size_t)cap * 8},
               {&B->a_imax, (size_t)cap * 2},
-              {&B->cand, (size_t)maxT * kCandTile * 7 * 4}, {&B->cand_cnt, (size_t)maxT * 4}};
+              {&B->cand, (size_t)maxT * kCandTile * 7 * 4}, {&B->cand_cnt, (size_t)maxT * 4},
+              {&B->cand_lab, (size_t)maxT * kCandTile * 4}, {&B->fgbits, (size_t)maxF * 8}};
   auto bad = [&](hipError_t e, const char* what) {
     batch_free(B);
     return fail(ctx, HDOG_ERR_HIP, "batch plan: %s failed: %s", what, hipGetErrorString(e));

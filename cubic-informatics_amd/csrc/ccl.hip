@@ -349,28 +349,222 @@ __global__ __launch_bounds__(NTM) void k_ccl_merge(uint32_t* L, int W, int H, in
     merge_tile<false>(L, W, WH, N, x0, y0, z0, ex, ey, ez, list_t, list_own, &n_list);
 }
 
+// ---- candidate-record form (the whole-path call) ------------------------------------
+// The Hessian pass leaves, per 64 x 8 x 16 tile, the positions of its candidate voxels
+// (the foreground of the CCL) in the tile.  Labeling then touches only candidates: P (a
+// volume-sized parent array) is written and read at candidate voxels only, and the
+// per-record labels go to rlab (slot-aligned with the records).
+//   kernel A: per tile, the tile's candidate bit mask (one word per row, stored tile-major
+//             for kernel B) and a union-find in LDS over the tile's candidates (links
+//             inside the clipped tile, no wrap); P[v] = rlab = the global index of v's
+//             tile-local root, the tile-local minimum;
+//   kernel B: per tile, the candidates on the tile's rim link each candidate neighbour
+//             outside the clipped tile (by linear offset, wrapping like the reference) with
+//             the global union-find on P;
+//   kernel C: every record's root (the component minimum), root bits for the region
+//             numbering, and with `dense` the final label written back into P.
+constexpr int RTX = kHessTX, RTY = kHessTY, RTZ = kHessTZ, RTV = RTX * RTY * RTZ, NTR = 256;
+constexpr int RROWS = RTY * RTZ;  // mask words per tile
+static_assert(RTX == 64 && RTV == kCandTile, "a tile row is one mask word");
+
+struct CandTile {
+  int x0, y0, z0, ex, ey, ez;
+};
+__device__ __forceinline__ CandTile cand_tile(unsigned tile, int W, int H, int D, int nbx,
+                                              int nby) {
+  CandTile t;
+  const unsigned r = tile / nbx;
+  t.x0 = (int)(tile % nbx) * RTX;
+  t.y0 = (int)(r % nby) * RTY;
+  t.z0 = (int)(r / nby) * RTZ;
+  t.ex = min(RTX, W - t.x0);
+  t.ey = min(RTY, H - t.y0);
+  t.ez = min(RTZ, D - t.z0);
+  return t;
+}
+// mask word of voxel (x, y, z) in the tile-major layout; its bit is x % 64
+__device__ __forceinline__ size_t mask_word(uint32_t x, uint32_t y, uint32_t z, int nbx,
+                                            int nby) {
+  const size_t tile = ((size_t)(z / RTZ) * nby + y / RTY) * nbx + x / RTX;
+  return tile * RROWS + (z % RTZ) * RTY + y % RTY;
+}
+
+__global__ __launch_bounds__(NTR) void k_cand_local(const float* __restrict__ cand,
+                                                   const uint32_t* __restrict__ cand_cnt,
+                                                   unsigned long long* __restrict__ fgbits,
+                                                   uint32_t* __restrict__ P,
+                                                   uint32_t* __restrict__ rlab, int W, int H,
+                                                   int D, int nbx, int nby, int links) {
+  const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t n = cand_cnt[tile];
+  __shared__ unsigned long long bits[RROWS];  // the tile's mask rows (word = ly + RTY lz)
+  __shared__ uint32_t lab[RTV];              // parents, valid at candidate positions
+  if (threadIdx.x < RROWS) bits[threadIdx.x] = 0ull;
+  __syncthreads();
+  const float* rec = cand + (size_t)tile * kCandTile * 7;
+  for (uint32_t i = threadIdx.x; i < n; i += NTR) {
+    const uint32_t v = __float_as_uint(rec[i]);
+    lab[v] = v;
+    atomicOr(bits + (v >> 6), 1ull << (v & 63));
+  }
+  __syncthreads();
+  if (threadIdx.x < RROWS) fgbits[(size_t)tile * RROWS + threadIdx.x] = bits[threadIdx.x];
+  if (n == 0) return;  // uniform over the work-group
+  const CandTile T = cand_tile(tile, W, H, D, nbx, nby);
+  const uint32_t Wu = (uint32_t)W, WH = Wu * (uint32_t)H;
+#pragma unroll 1
+  for (uint32_t i = threadIdx.x; links && i < n; i += NTR) {
+    const int v = (int)__float_as_uint(rec[i]);
+    const int vx = v & (RTX - 1), vy = (v / RTX) & (RTY - 1), vz = v / (RTX * RTY);
+    uint32_t root = (uint32_t)v;  // a representative of v's set, refreshed by each link
+#pragma unroll
+    for (int q = 0; q < 13; ++q) {
+      int dx, dy, dz;
+      fwd_offset(q, dx, dy, dz);
+      const int nx = vx + dx, ny = vy + dy, nz = vz + dz;
+      if (nx < 0 || nx >= T.ex || ny < 0 || ny >= T.ey || nz >= T.ez) continue;
+      if (!((bits[ny + RTY * nz] >> nx) & 1ull)) continue;
+      const int u = nx + RTX * (ny + RTY * nz);
+      if (lab[u] != root) root = lds_unite(lab, root, (uint32_t)u);
+    }
+  }
+  __syncthreads();
+  auto global = [&](uint32_t v) {  // linear index of tile position v
+    return (uint32_t)(T.z0 + (int)(v / (RTX * RTY))) * WH +
+           (uint32_t)(T.y0 + (int)((v / RTX) & (RTY - 1))) * Wu + (uint32_t)T.x0 +
+           (v & (RTX - 1));
+  };
+  for (uint32_t i = threadIdx.x; i < n; i += NTR) {
+    const uint32_t v = __float_as_uint(rec[i]);
+    const uint32_t label = global(lds_find(lab, v));
+    P[global(v)] = label;
+    rlab[(size_t)tile * kCandTile + i] = label;
+  }
+}
+
+__global__ __launch_bounds__(NTR) void k_cand_merge(const float* __restrict__ cand,
+                                                   const uint32_t* __restrict__ cand_cnt,
+                                                   const unsigned long long* __restrict__ fgbits,
+                                                   uint32_t* P, const uint32_t* __restrict__ rlab,
+                                                   int W, int H, int D, int nbx, int nby,
+                                                   int generic, Offsets offs) {
+  const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+  const uint32_t n = cand_cnt[tile];
+  if (n == 0) return;
+  const CandTile T = cand_tile(tile, W, H, D, nbx, nby);
+  const uint32_t Wu = (uint32_t)W, WH = Wu * (uint32_t)H;
+  const float* rec = cand + (size_t)tile * kCandTile * 7;
+  if (generic) {  // width or height 1: every candidate, the distinct positive offsets
+    const i64 N = (i64)WH * D;
+#pragma unroll 1
+    for (uint32_t i = threadIdx.x; i < n; i += NTR) {
+      const uint32_t v = __float_as_uint(rec[i]);
+      const uint32_t idx = (uint32_t)(T.z0 + (int)(v / (RTX * RTY))) * WH +
+                           (uint32_t)(T.y0 + (int)((v / RTX) & (RTY - 1))) * Wu +
+                           (uint32_t)T.x0 + (v & (RTX - 1));
+      for (int k = 0; k < offs.n; ++k) {
+        const i64 u = (i64)idx + offs.o[k];
+        if (u >= N) break;  // ascending offsets
+        const uint32_t uu = (uint32_t)u, uz = uu / WH, rem = uu - uz * WH, uy = rem / Wu,
+                       ux = rem - uy * Wu;
+        if ((fgbits[mask_word(ux, uy, uz, nbx, nby)] >> (ux & 63)) & 1ull) unite(P, idx, uu);
+      }
+    }
+    return;
+  }
+#pragma unroll 1
+  for (uint32_t i = threadIdx.x; i < n; i += NTR) {
+    const int v = (int)__float_as_uint(rec[i]);
+    const int lx = v & (RTX - 1), ly = (v / RTX) & (RTY - 1), lz = v / (RTX * RTY);
+    // every forward neighbour of an interior voxel lies in the clipped tile
+    if (!(lz == T.ez - 1 || lx == 0 || lx == T.ex - 1 || ly == 0 || ly == T.ey - 1)) continue;
+    const int x = T.x0 + lx, y = T.y0 + ly, z = T.z0 + lz;
+    const uint32_t me = rlab[(size_t)tile * kCandTile + i];
+    uint32_t lj[13];
+#pragma unroll
+    for (int q = 0; q < 13; ++q) {
+      int dx, dy, dz;
+      fwd_offset(q, dx, dy, dz);
+      lj[q] = 0u;
+      const int nx = lx + dx, ny = ly + dy, nz = lz + dz;
+      if (nx >= 0 && nx < T.ex && ny >= 0 && ny < T.ey && nz < T.ez) continue;  // kernel A
+      // the neighbour at linear offset dz WH + dy W + dx (wrapping across x and y like the
+      // reference; one carry per axis for width, height >= 2); it exists below N
+      int ux = x + dx, uy = y + dy, uz = z + dz;
+      if (ux < 0) { ux += W; --uy; } else if (ux >= W) { ux -= W; ++uy; }
+      if (uy < 0) { uy += H; --uz; } else if (uy >= H) { uy -= H; ++uz; }
+      if (uz >= D) continue;
+      if ((fgbits[mask_word((uint32_t)ux, (uint32_t)uy, (uint32_t)uz, nbx, nby)] >> (ux & 63)) &
+          1ull)
+        lj[q] = P[(uint32_t)uz * WH + (uint32_t)uy * Wu + (uint32_t)ux];
+    }
+    uint32_t seen1 = 0u, seen2 = 0u;
+#pragma unroll
+    for (int q = 0; q < 13; ++q) {
+      const uint32_t l = lj[q];
+      if (!l || l == me || l == seen1 || l == seen2) continue;
+      unite(P, me, l);
+      seen2 = seen1;
+      seen1 = l;
+    }
+  }
+}
+
+__global__ __launch_bounds__(NTR) void k_cand_flatten(const float* __restrict__ cand,
+                                                     const uint32_t* __restrict__ cand_cnt,
+                                                     uint32_t* P, uint32_t* __restrict__ rlab,
+                                                     unsigned long long* __restrict__ roots,
+                                                     int dense, int W, int H, int D, int nbx,
+                                                     int nby) {
+  const unsigned tile = blockIdx.x;
+  const uint32_t n = cand_cnt[tile];
+  if (n == 0) return;
+  const CandTile T = cand_tile(tile, W, H, D, nbx, nby);
+  const uint32_t Wu = (uint32_t)W, WH = Wu * (uint32_t)H;
+  const float* rec = cand + (size_t)tile * kCandTile * 7;
+  uint32_t* rl = rlab + (size_t)tile * kCandTile;
+  for (uint32_t i = threadIdx.x; i < n; i += NTR) {
+    const uint32_t v = __float_as_uint(rec[i]);
+    const uint32_t idx = (uint32_t)(T.z0 + (int)(v / (RTX * RTY))) * WH +
+                         (uint32_t)(T.y0 + (int)((v / RTX) & (RTY - 1))) * Wu + (uint32_t)T.x0 +
+                         (v & (RTX - 1));
+    // any value of P met on the way is an ancestor, also where another thread has already
+    // written a root
+    const uint32_t r = find_root_readonly(P, rl[i]);
+    rl[i] = r;
+    if (dense && P[idx] != r) P[idx] = r;
+    if (r == idx) atomicOr(roots + (r >> 6), 1ull << (r & 63));
+  }
+}
+
+// The distinct positive linear offsets dz W H + dy W + dx of the 27-neighbourhood
+// (src/3D/CCL3D.cu:43-48 keeps every offset whose target lies in [0, N)).
+Offsets positive_offsets(int W, int H) {
+  const i64 WH = (i64)W * H;
+  Offsets offs{};
+  i64 v[27];
+  int n = 0;
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const i64 o = dz * WH + dy * (i64)W + dx;
+        if (o <= 0) continue;
+        bool dup = false;
+        for (int k = 0; k < n; ++k) dup |= (v[k] == o);
+        if (!dup) v[n++] = o;
+      }
+  std::sort(v, v + n);
+  for (int k = 0; k < n; ++k) offs.o[k] = v[k];
+  offs.n = n;
+  return offs;
+}
+
 }  // namespace
 
 hipError_t launch_ccl(const uint8_t* pd, uint32_t* labels, int W, int H, int D, hipStream_t s,
                       bool flatten) {
   const i64 WH = (i64)W * H, N = WH * D;
-  Offsets offs{};
-  {
-    i64 v[27];
-    int n = 0;
-    for (int dz = -1; dz <= 1; ++dz)
-      for (int dy = -1; dy <= 1; ++dy)
-        for (int dx = -1; dx <= 1; ++dx) {
-          const i64 o = dz * WH + dy * (i64)W + dx;
-          if (o <= 0) continue;
-          bool dup = false;
-          for (int k = 0; k < n; ++k) dup |= (v[k] == o);
-          if (!dup) v[n++] = o;
-        }
-    std::sort(v, v + n);
-    for (int k = 0; k < n; ++k) offs.o[k] = v[k];
-    offs.n = n;
-  }
+  const Offsets offs = positive_offsets(W, H);
   const i64 nb = std::min<i64>((N + NT - 1) / NT, 1 << 20);
   hipError_t e;
   if (W >= 2 && H >= 2) {
@@ -394,6 +588,34 @@ hipError_t launch_ccl(const uint8_t* pd, uint32_t* labels, int W, int H, int D, 
   if (!flatten) return hipSuccess;
   hipLaunchKernelGGL(k_ccl_flatten, dim3((unsigned)nb), dim3(NT), 0, s, labels, N);
   mark("ccl/flatten", s);
+  return hipGetLastError();
+}
+
+hipError_t launch_ccl_candidates(const float* cand, const uint32_t* cand_cnt, i64 ntiles,
+                                 unsigned long long* fgbits, uint32_t* P, uint32_t* rlab,
+                                 unsigned long long* roots, i64 root_words, bool dense, int W,
+                                 int H, int D, hipStream_t s) {
+  const int nbx = (W + RTX - 1) / RTX, nby = (H + RTY - 1) / RTY;
+  hipError_t e;
+  if (dense) {  // background of the materialized label volume
+    if ((e = launch_fill_u32(P, (i64)W * H * D, 0u, s)) != hipSuccess) return e;
+  }
+  // the 13 forward offsets with one carry per axis are the positive offsets only for
+  // width, height >= 2; otherwise every link goes through the generic merge
+  const int generic = (W < 2 || H < 2) ? 1 : 0;
+  hipLaunchKernelGGL(k_cand_local, dim3((unsigned)ntiles), dim3(NTR), 0, s, cand, cand_cnt, fgbits,
+                     P, rlab, W, H, D, nbx, nby, 1 - generic);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  mark("ccl/cand_local", s);
+  hipLaunchKernelGGL(k_cand_merge, dim3((unsigned)ntiles), dim3(NTR), 0, s, cand, cand_cnt,
+                     (const unsigned long long*)fgbits, P, rlab, W, H, D, nbx, nby, generic,
+                     positive_offsets(W, H));
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  mark("ccl/cand_merge", s);
+  if ((e = launch_fill_u64((u64*)roots, root_words, 0ull, s)) != hipSuccess) return e;
+  hipLaunchKernelGGL(k_cand_flatten, dim3((unsigned)ntiles), dim3(NTR), 0, s, cand, cand_cnt, P,
+                     rlab, roots, dense ? 1 : 0, W, H, D, nbx, nby);
+  mark("ccl/cand_flatten", s);
   return hipGetLastError();
 }
 

@@ -29,6 +29,8 @@ inline i64 hessian_tiles(int W, int H, int D) {
   return (i64)((W + kHessTX - 1) / kHessTX) * ((H + kHessTY - 1) / kHessTY) *
          ((D + kHessTZ - 1) / kHessTZ);
 }
+// words of the candidate bit mask: one per (z, y) row of each Hessian tile, tile-major
+inline i64 fg_words(int W, int H, int D) { return hessian_tiles(W, H, D) * kHessTY * kHessTZ; }
 
 // ---- storage types of float intermediates --------------------------------------
 template <typename T> __device__ __forceinline__ float ld(const T* p, i64 i);
@@ -178,9 +180,11 @@ hipError_t launch_gauss_pair(const float* src, void* tmp1, void* tmp2, const flo
                              const int r[6], float nz, float* dog, float2* pair_out, int W, int H,
                              int D, hipStream_t s);
 template <typename TD, typename TH>
-// hess may be NULL (mask only).  cand / cand_cnt (optional): per Hessian tile, the number
-// of candidate voxels and their records (index, six elements rounded to TH, as float);
-// cand holds hessian_tiles() * kCandTile * 7 words.
+// hess and pd may be NULL (not written).  cand / cand_cnt (optional): per Hessian tile
+// (kHessTX x kHessTY x kHessTZ, tiles x-fastest), the number of candidate voxels and their
+// records: the voxel's position in the tile, lx + kHessTX (ly + kHessTY lz), then the six
+// elements rounded to TH, as 7 SoA words of kCandTile slots; cand holds hessian_tiles() *
+// kCandTile * 7 words.  Voxel 0 never has a record (its label is 0 whatever its mask).
 hipError_t launch_hessian(const TD* dog, TH* hess, uint8_t* pd, int W, int H, int D,
                           hipStream_t s, float* cand = nullptr, uint32_t* cand_cnt = nullptr);
 // CCL: labels out; union-find in place
@@ -206,7 +210,7 @@ hipError_t launch_region_accum(const uint32_t* labels, const uint32_t* rid, cons
 // Region sums from the candidate records of launch_hessian (ntiles = hessian_tiles()).
 template <typename TN>
 hipError_t launch_region_accum_cand(const float* cand, const uint32_t* cand_cnt, i64 ntiles,
-                                    const uint32_t* labels, const uint32_t* rid, const TN* norm,
+                                    const uint32_t* rlab, const uint32_t* rid, const TN* norm,
                                     const uint16_t* img, RegionAcc* acc, i64 cap, int W, int H,
                                     int is_ave, hipStream_t s);
 struct RegionOut {
@@ -228,6 +232,21 @@ hipError_t launch_region_finalize(const RegionAcc* acc, i64 R, int is_ave, Regio
 hipError_t launch_tile_reserve(const uint32_t* d_R, i64 acc_cap, i64 arena_cap, u64* running,
                                TileOut* t, hipStream_t s);
 hipError_t launch_fill_u64(u64* p, i64 n, u64 value, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, i64 n, uint32_t value, hipStream_t s);
+// Labeling of the candidates the Hessian pass recorded (cand / cand_cnt, see
+// launch_hessian): fgbits (fg_words() words) receives the candidate bit mask; P is a
+// volume-sized parent array written and read at candidate voxels only; rlab (slot-aligned
+// with the records) receives each record's label, the component's minimum linear index;
+// roots (root_words 64-bit words, cleared here) receives the root bit of every component,
+// in linear order.  dense: P becomes the full label volume (0 for the background).
+hipError_t launch_ccl_candidates(const float* cand, const uint32_t* cand_cnt, i64 ntiles,
+                                 unsigned long long* fgbits, uint32_t* P, uint32_t* rlab,
+                                 unsigned long long* roots, i64 root_words, bool dense, int W,
+                                 int H, int D, hipStream_t s);
+// Roots per numbering block from a root bit mask in linear order, then the exclusive scan
+// of the counts; *d_R = 1 + number of roots.
+hipError_t launch_region_count_masks(const unsigned long long* masks, uint32_t* block_cnt,
+                                     uint32_t* d_R, i64 N, hipStream_t s, i64* nblocks_out);
 // differentiable blob response (soft.hip)
 hipError_t launch_soft_loss(const float* hess, const uint8_t* mask, const uint8_t* target, i64 N,
                             double inv_tau, double* sums, hipStream_t s);

@@ -251,12 +251,13 @@ __global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* 
           }
           const uint8_t m =
               BAND ? negdef_band(xx, xy, xz, yy, yz, zz) : negdef32(xx, xy, xz, yy, yz, zz);
-          pd[idx] = m;
-          is_cand = m != 0 && rec != nullptr;
+          if (pd) pd[idx] = m;
+          // voxel 0 is background whatever its mask (its label is 0): no record
+          is_cand = m != 0 && idx != 0 && rec != nullptr;
           if (is_cand) {
             h6[0] = round_to<TS>(xx); h6[1] = round_to<TS>(xy); h6[2] = round_to<TS>(xz);
             h6[3] = round_to<TS>(yy); h6[4] = round_to<TS>(yz); h6[5] = round_to<TS>(zz);
-            cidx = idx;
+            cidx = (uint32_t)(tx + TX * (ty + TY * (z - z0)));  // position in the tile
           }
         }
         // candidate records: one LDS reservation per wave (a wave is one row of 64 x),

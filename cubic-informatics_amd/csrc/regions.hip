@@ -24,6 +24,11 @@ constexpr int ITEMS = 16;
 constexpr i64 BLK = (i64)NT * ITEMS;  // voxels per numbering block
 static_assert(BLK == kRootBlock, "numbering block");
 
+__device__ __forceinline__ int wave_sum_int(int v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;  // lane 0 holds the sum
+}
+
 __device__ __forceinline__ int block_sum_int(int v, int* sh) {
   // wave64 reduction, then across the 4 waves
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
@@ -328,13 +333,13 @@ constexpr int TPW = HDOG_ACC_TPW;  // z-consecutive Hessian tiles folded per wor
 template <typename TN>
 __global__ __launch_bounds__(NT) void k_accum_cand(const float* __restrict__ cand,
                                                    const uint32_t* __restrict__ cand_cnt,
-                                                   const uint32_t* __restrict__ L,
+                                                   const uint32_t* __restrict__ rlab,
                                                    const uint32_t* __restrict__ rid,
                                                    const TN* __restrict__ norm,
                                                    const uint16_t* __restrict__ img,
                                                    RegionAcc* acc, uint32_t cap, uint32_t W,
                                                    uint32_t WH, int is_ave, unsigned plane,
-                                                   unsigned nbz) {
+                                                   unsigned nbz, unsigned nbx) {
   // work-group p takes the tiles (x, y, TPW z + q), q < TPW, of the Hessian tiling with
   // `plane` tiles per z level; regions spanning those tiles pay one set of global atomics
   const unsigned p = xcd_tile(blockIdx.x, gridDim.x);
@@ -357,15 +362,21 @@ __global__ __launch_bounds__(NT) void k_accum_cand(const float* __restrict__ can
 #pragma unroll
   for (int q = 0; q < TPW; ++q) {
     const float* t = cand + (size_t)tiles[q] * kCandTile * 7;
+    const uint32_t* rl = rlab + (size_t)tiles[q] * kCandTile;  // each record's root label
+    // tile origin: records hold the voxel's position in the tile
+    const uint32_t tx0 = (xy % nbx) * kHessTX, ty0 = (xy / nbx) * kHessTY,
+                   tz0 = (zq * TPW + q) * kHessTZ;
     for (uint32_t i = threadIdx.x; i < n[q]; i += NT) {
-      const uint32_t idx = __float_as_uint(t[i]);
+      const uint32_t v = __float_as_uint(t[i]);
+      const uint32_t x = tx0 + (v % kHessTX), y = ty0 + (v / kHessTX) % kHessTY,
+                     z = tz0 + v / (kHessTX * kHessTY);
+      const uint32_t idx = z * WH + y * W + x;
       float hv[6];
 #pragma unroll
       for (int k = 0; k < 6; ++k) hv[k] = t[(size_t)kCandTile * (1 + k) + i];
-      const uint32_t r = rid[L[idx]];
+      const uint32_t r = rid[rl[i]];
       const float nv = ld(norm, (i64)idx);
       const uint32_t iv = has_img ? (uint32_t)img[idx] : 0u;
-      const uint32_t z = idx / WH, rem = idx - z * WH, y = rem / W, x = rem - y * W;
       if (r < cap) tab.add(acc, r, x, y, z, hv, nv, iv, is_ave, has_img);
     }
   }
@@ -466,6 +477,20 @@ __global__ __launch_bounds__(NT) void k_fill_u64(u64* p, i64 n, u64 v) {
   for (i64 i = (i64)blockIdx.x * NT + threadIdx.x; i < n; i += (i64)gridDim.x * NT) p[i] = v;
 }
 
+__global__ __launch_bounds__(NT) void k_fill_u32(uint32_t* p, i64 n, uint32_t v) {
+  for (i64 i = (i64)blockIdx.x * NT + threadIdx.x; i < n; i += (i64)gridDim.x * NT) p[i] = v;
+}
+
+// roots per 4096-voxel block from the root bit mask (one wave per block, a word per lane)
+__global__ __launch_bounds__(NT) void k_mask_counts(const unsigned long long* __restrict__ masks,
+                                                    i64 nb, uint32_t* __restrict__ cnt) {
+  const i64 blk = (i64)blockIdx.x * (NT / 64) + (threadIdx.x >> 6);
+  if (blk >= nb) return;  // whole waves
+  const int c = __popcll(masks[blk * (kRootBlock / 64) + (threadIdx.x & 63)]);
+  const int t = wave_sum_int(c);
+  if ((threadIdx.x & 63) == 0) cnt[blk] = (uint32_t)t;
+}
+
 }  // namespace
 
 hipError_t launch_region_ids(uint32_t* labels, uint32_t* rid, uint32_t* block_cnt,
@@ -523,15 +548,16 @@ hipError_t launch_region_accum(const uint32_t* labels, const uint32_t* rid, cons
 
 template <typename TN>
 hipError_t launch_region_accum_cand(const float* cand, const uint32_t* cand_cnt, i64 ntiles,
-                                    const uint32_t* labels, const uint32_t* rid, const TN* norm,
+                                    const uint32_t* rlab, const uint32_t* rid, const TN* norm,
                                     const uint16_t* img, RegionAcc* acc, i64 cap, int W, int H,
                                     int is_ave, hipStream_t s) {
   const unsigned plane = (unsigned)(((W + kHessTX - 1) / kHessTX) * ((H + kHessTY - 1) / kHessTY));
   const unsigned nbz = (unsigned)(ntiles / plane);
   const i64 ngroups = (i64)plane * ((nbz + TPW - 1) / TPW);
   hipLaunchKernelGGL((k_accum_cand<TN>), dim3((unsigned)ngroups), dim3(NT), 0, s, cand, cand_cnt,
-                     labels, rid, norm, img, acc, (uint32_t)std::min<i64>(cap, 0xFFFFFFFFll),
-                     (uint32_t)W, (uint32_t)((i64)W * H), is_ave, plane, nbz);
+                     rlab, rid, norm, img, acc, (uint32_t)std::min<i64>(cap, 0xFFFFFFFFll),
+                     (uint32_t)W, (uint32_t)((i64)W * H), is_ave, plane, nbz,
+                     (unsigned)((W + kHessTX - 1) / kHessTX));
   mark("regions/accum_candidates", s);
   return hipGetLastError();
 }
@@ -565,6 +591,26 @@ hipError_t launch_region_finalize(const RegionAcc* acc, i64 R, int is_ave, Regio
 hipError_t launch_tile_reserve(const uint32_t* d_R, i64 acc_cap, i64 arena_cap, u64* running,
                                TileOut* t, hipStream_t s) {
   hipLaunchKernelGGL(k_tile_reserve, dim3(1), dim3(64), 0, s, d_R, acc_cap, arena_cap, running, t);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t* p, i64 n, uint32_t value, hipStream_t s) {
+  const i64 nb = std::max<i64>(1, std::min<i64>((n + NT - 1) / NT, 1 << 16));
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)nb), dim3(NT), 0, s, p, n, value);
+  return hipGetLastError();
+}
+
+hipError_t launch_region_count_masks(const unsigned long long* masks, uint32_t* block_cnt,
+                                     uint32_t* d_R, i64 N, hipStream_t s, i64* nblocks_out) {
+  const i64 nb = (N + kRootBlock - 1) / kRootBlock;
+  if (nblocks_out) *nblocks_out = nb;
+  hipLaunchKernelGGL(k_mask_counts, dim3((unsigned)((nb + NT / 64 - 1) / (NT / 64))), dim3(NT), 0,
+                     s, masks, nb, block_cnt);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  mark("regions/count_root_bits", s);
+  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, block_cnt, nb, d_R);
+  mark("regions/scan", s);
   return hipGetLastError();
 }
 
