@@ -363,8 +363,34 @@ __global__ __launch_bounds__(NTM) void k_ccl_merge(uint32_t* L, int W, int H, in
 //             the global union-find on P;
 //   kernel C: every record's root (the component minimum), root bits for the region
 //             numbering, and with `dense` the final label written back into P.
-constexpr int RTX = kHessTX, RTY = kHessTY, RTZ = kHessTZ, RTV = RTX * RTY * RTZ, NTR = 256;
+constexpr int RTX = kHessTX, RTY = kHessTY, RTZ = kHessTZ, RTV = RTX * RTY * RTZ, NTR = 512;
 constexpr int RROWS = RTY * RTZ;  // mask words per tile
+// Each thread keeps the positions of its first KREC records (slots t + k NTR) in registers,
+// loaded together: a tile's record list is then read once, with all loads in flight,
+// instead of one dependent load per record and phase.  Tiles with more than KREC * NTR
+// candidates take their remaining slots with a plain loop.
+constexpr int KREC = 4;
+struct TileRecs {
+  uint32_t v[KREC];
+  __device__ __forceinline__ void load(const float* rec, uint32_t n) {
+#pragma unroll
+    for (int k = 0; k < KREC; ++k) {
+      const uint32_t i = threadIdx.x + k * NTR;
+      v[k] = i < n ? __float_as_uint(rec[i]) : 0u;
+    }
+  }
+  // f(slot, position) for every record of this thread
+  template <typename F>
+  __device__ __forceinline__ void each(const float* rec, uint32_t n, F&& f) const {
+#pragma unroll
+    for (int k = 0; k < KREC; ++k) {
+      const uint32_t i = threadIdx.x + k * NTR;
+      if (i < n) f(i, v[k]);
+    }
+#pragma unroll 1
+    for (uint32_t i = threadIdx.x + KREC * NTR; i < n; i += NTR) f(i, __float_as_uint(rec[i]));
+  }
+};
 static_assert(RTX == 64 && RTV == kCandTile, "a tile row is one mask word");
 
 struct CandTile {
@@ -399,47 +425,47 @@ __global__ __launch_bounds__(NTR) void k_cand_local(const float* __restrict__ ca
   const uint32_t n = cand_cnt[tile];
   __shared__ unsigned long long bits[RROWS];  // the tile's mask rows (word = ly + RTY lz)
   __shared__ uint32_t lab[RTV];              // parents, valid at candidate positions
+  const float* rec = cand + (size_t)tile * kCandTile * 7;
+  TileRecs R;
+  R.load(rec, n);
   if (threadIdx.x < RROWS) bits[threadIdx.x] = 0ull;
   __syncthreads();
-  const float* rec = cand + (size_t)tile * kCandTile * 7;
-  for (uint32_t i = threadIdx.x; i < n; i += NTR) {
-    const uint32_t v = __float_as_uint(rec[i]);
+  R.each(rec, n, [&](uint32_t, uint32_t v) {
     lab[v] = v;
     atomicOr(bits + (v >> 6), 1ull << (v & 63));
-  }
+  });
   __syncthreads();
   if (threadIdx.x < RROWS) fgbits[(size_t)tile * RROWS + threadIdx.x] = bits[threadIdx.x];
   if (n == 0) return;  // uniform over the work-group
   const CandTile T = cand_tile(tile, W, H, D, nbx, nby);
   const uint32_t Wu = (uint32_t)W, WH = Wu * (uint32_t)H;
-#pragma unroll 1
-  for (uint32_t i = threadIdx.x; links && i < n; i += NTR) {
-    const int v = (int)__float_as_uint(rec[i]);
-    const int vx = v & (RTX - 1), vy = (v / RTX) & (RTY - 1), vz = v / (RTX * RTY);
-    uint32_t root = (uint32_t)v;  // a representative of v's set, refreshed by each link
+  if (links)
+    R.each(rec, n, [&](uint32_t, uint32_t pv) {
+      const int v = (int)pv;
+      const int vx = v & (RTX - 1), vy = (v / RTX) & (RTY - 1), vz = v / (RTX * RTY);
+      uint32_t root = (uint32_t)v;  // a representative of v's set, refreshed by each link
 #pragma unroll
-    for (int q = 0; q < 13; ++q) {
-      int dx, dy, dz;
-      fwd_offset(q, dx, dy, dz);
-      const int nx = vx + dx, ny = vy + dy, nz = vz + dz;
-      if (nx < 0 || nx >= T.ex || ny < 0 || ny >= T.ey || nz >= T.ez) continue;
-      if (!((bits[ny + RTY * nz] >> nx) & 1ull)) continue;
-      const int u = nx + RTX * (ny + RTY * nz);
-      if (lab[u] != root) root = lds_unite(lab, root, (uint32_t)u);
-    }
-  }
+      for (int q = 0; q < 13; ++q) {
+        int dx, dy, dz;
+        fwd_offset(q, dx, dy, dz);
+        const int nx = vx + dx, ny = vy + dy, nz = vz + dz;
+        if (nx < 0 || nx >= T.ex || ny < 0 || ny >= T.ey || nz >= T.ez) continue;
+        if (!((bits[ny + RTY * nz] >> nx) & 1ull)) continue;
+        const int u = nx + RTX * (ny + RTY * nz);
+        if (lab[u] != root) root = lds_unite(lab, root, (uint32_t)u);
+      }
+    });
   __syncthreads();
   auto global = [&](uint32_t v) {  // linear index of tile position v
     return (uint32_t)(T.z0 + (int)(v / (RTX * RTY))) * WH +
            (uint32_t)(T.y0 + (int)((v / RTX) & (RTY - 1))) * Wu + (uint32_t)T.x0 +
            (v & (RTX - 1));
   };
-  for (uint32_t i = threadIdx.x; i < n; i += NTR) {
-    const uint32_t v = __float_as_uint(rec[i]);
+  R.each(rec, n, [&](uint32_t i, uint32_t v) {
     const uint32_t label = global(lds_find(lab, v));
     P[global(v)] = label;
     rlab[(size_t)tile * kCandTile + i] = label;
-  }
+  });
 }
 
 __global__ __launch_bounds__(NTR) void k_cand_merge(const float* __restrict__ cand,
@@ -472,14 +498,54 @@ __global__ __launch_bounds__(NTR) void k_cand_merge(const float* __restrict__ ca
     }
     return;
   }
+  // rim candidates (those with a forward neighbour outside the clipped tile: the top
+  // face and the ring of every lower plane) are compacted into an LDS list first, so the
+  // neighbour work below runs on full waves
+  constexpr int RIM_MAX = RTX * RTY + (RTZ - 1) * (2 * RTX + 2 * (RTY - 2));
+  __shared__ uint16_t rim_v[RIM_MAX];
+  __shared__ uint32_t rim_me[RIM_MAX];
+  __shared__ uint32_t n_rim;
+  if (threadIdx.x == 0) n_rim = 0;
+  TileRecs R;
+  R.load(rec, n);
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  auto push = [&](bool on, uint32_t v, uint32_t me) {  // wave-uniform call
+    const unsigned long long bal = __ballot(on);
+    if (!bal) return;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(&n_rim, (uint32_t)__popcll(bal));
+    base = __shfl(base, 0, 64);
+    if (on) {
+      const uint32_t slot = base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+      rim_v[slot] = (uint16_t)v;
+      rim_me[slot] = me;
+    }
+  };
+  auto is_rim = [&](uint32_t v) {
+    const int lx = (int)v & (RTX - 1), ly = (int)(v / RTX) & (RTY - 1), lz = (int)(v / (RTX * RTY));
+    return lz == T.ez - 1 || lx == 0 || lx == T.ex - 1 || ly == 0 || ly == T.ey - 1;
+  };
+#pragma unroll
+  for (int k = 0; k < KREC; ++k) {
+    const uint32_t i = threadIdx.x + k * NTR;
+    const bool on = i < n && is_rim(R.v[k]);
+    push(on, R.v[k], on ? rlab[(size_t)tile * kCandTile + i] : 0u);
+  }
+  for (uint32_t i0 = KREC * NTR; i0 < n; i0 += NTR) {  // uniform trip count
+    const uint32_t i = i0 + threadIdx.x;
+    const uint32_t v = i < n ? __float_as_uint(rec[i]) : 0u;
+    const bool on = i < n && is_rim(v);
+    push(on, v, on ? rlab[(size_t)tile * kCandTile + i] : 0u);
+  }
+  __syncthreads();
+  const uint32_t nr = n_rim;
 #pragma unroll 1
-  for (uint32_t i = threadIdx.x; i < n; i += NTR) {
-    const int v = (int)__float_as_uint(rec[i]);
+  for (uint32_t j = threadIdx.x; j < nr; j += NTR) {
+    const int v = rim_v[j];
     const int lx = v & (RTX - 1), ly = (v / RTX) & (RTY - 1), lz = v / (RTX * RTY);
-    // every forward neighbour of an interior voxel lies in the clipped tile
-    if (!(lz == T.ez - 1 || lx == 0 || lx == T.ex - 1 || ly == 0 || ly == T.ey - 1)) continue;
     const int x = T.x0 + lx, y = T.y0 + ly, z = T.z0 + lz;
-    const uint32_t me = rlab[(size_t)tile * kCandTile + i];
+    const uint32_t me = rim_me[j];
     uint32_t lj[13];
 #pragma unroll
     for (int q = 0; q < 13; ++q) {
@@ -523,18 +589,33 @@ __global__ __launch_bounds__(NTR) void k_cand_flatten(const float* __restrict__ 
   const uint32_t Wu = (uint32_t)W, WH = Wu * (uint32_t)H;
   const float* rec = cand + (size_t)tile * kCandTile * 7;
   uint32_t* rl = rlab + (size_t)tile * kCandTile;
-  for (uint32_t i = threadIdx.x; i < n; i += NTR) {
-    const uint32_t v = __float_as_uint(rec[i]);
+  TileRecs R;
+  R.load(rec, n);
+  uint32_t l0[KREC];  // the records' tile-local root labels, loaded together
+#pragma unroll
+  for (int k = 0; k < KREC; ++k) {
+    const uint32_t i = threadIdx.x + k * NTR;
+    l0[k] = i < n ? rl[i] : 0u;
+  }
+  auto one = [&](uint32_t i, uint32_t v, uint32_t l) {
     const uint32_t idx = (uint32_t)(T.z0 + (int)(v / (RTX * RTY))) * WH +
                          (uint32_t)(T.y0 + (int)((v / RTX) & (RTY - 1))) * Wu + (uint32_t)T.x0 +
                          (v & (RTX - 1));
     // any value of P met on the way is an ancestor, also where another thread has already
     // written a root
-    const uint32_t r = find_root_readonly(P, rl[i]);
+    const uint32_t r = find_root_readonly(P, l);
     rl[i] = r;
     if (dense && P[idx] != r) P[idx] = r;
     if (r == idx) atomicOr(roots + (r >> 6), 1ull << (r & 63));
+  };
+#pragma unroll
+  for (int k = 0; k < KREC; ++k) {
+    const uint32_t i = threadIdx.x + k * NTR;
+    if (i < n) one(i, R.v[k], l0[k]);
   }
+#pragma unroll 1
+  for (uint32_t i = threadIdx.x + KREC * NTR; i < n; i += NTR)
+    one(i, __float_as_uint(rec[i]), rl[i]);
 }
 
 // The distinct positive linear offsets dz W H + dy W + dx of the 27-neighbourhood
