@@ -54,9 +54,9 @@ KERNEL_BYTES = {
     "dog/gauss_y": lambda sn, sd, sh, cf: 8 + 8,
     "dog/gauss_xy": lambda sn, sd, sh, cf: sn + 8,
     "dog/gauss_z_dog": lambda sn, sd, sh, cf: 8 + sd,
-    # candidate bit mask, plus a 28-byte record (index + six elements) per candidate voxel
-    "hessian/hessian_pd": lambda sn, sd, sh, cf: sd + 0.125 + 28 * cf,
-    # per candidate: record index, parent entry, record label (+ the tile's mask rows)
+    # a 28-byte record (position in the tile + six elements) per candidate voxel
+    "hessian/hessian_pd": lambda sn, sd, sh, cf: sd + 28 * cf,
+    # per candidate: record position, parent entry, record label (+ the tile's mask rows)
     "ccl/cand_local": lambda sn, sd, sh, cf: 0.125 + 12 * cf,
     # per candidate: record index and label (rim neighbours are L2 reads of the same data)
     "ccl/cand_merge": lambda sn, sd, sh, cf: 8 * cf,
@@ -129,12 +129,10 @@ def run_form(hd, img, shape, precision, steps, warmup, keep_pd=False):
                             C1["gamma"], False, precision)
     bufs = hd.alloc_intermediates(shape, precision)
     # the step's product is the region arrays: the Hessian volumes, the label volume and
-    # (unless the forms' candidate sets are compared) the byte mask are not requested, so
-    # the library does not materialize them (labeling and the region pass work from the
-    # candidate records and the candidate bit mask)
-    del bufs["hess"], bufs["labels"]
-    if not keep_pd:
-        del bufs["pd"]
+    # the byte mask are not requested, so the library does not materialize them (labeling
+    # and the region pass work from the candidate records and the candidate bit mask); the
+    # mask for the candidate count and the forms' comparison comes from one untimed pass
+    del bufs["hess"], bufs["labels"], bufs["pd"]
     hd.set_profiling(False)
     R = 0
     for i in range(warmup):
@@ -160,9 +158,8 @@ def run_form(hd, img, shape, precision, steps, warmup, keep_pd=False):
         for name, ms in hd.stage_times():
             kms[name].append(ms)
     hd.set_profiling(False)
-    if "pd" not in bufs:  # one more, untimed pass for the candidate count
-        bufs["pd"] = torch.empty(shape, dtype=torch.uint8, device=img.device)
-        hd.run_device(img, params, bufs)
+    bufs["pd"] = torch.empty(shape, dtype=torch.uint8, device=img.device)
+    hd.run_device(img, params, bufs)  # untimed: the candidate mask
     pd = bufs["pd"].clone() if keep_pd else None
     cand = int(bufs["pd"].sum(dtype=torch.int64).item())
     del bufs

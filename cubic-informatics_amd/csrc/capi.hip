@@ -248,9 +248,9 @@ int run_stages(hdog_ctx* ctx, hipStream_t s, const hdog_params* p, const uint16_
   cand.cnt = (const uint32_t*)ctx->cand_cnt.p;
   cand.rlab = (const uint32_t*)ctx->cand_lab.p;
   HIPCHK(launch_hessian<TD, TH>(dog, hess, pd, W, H, D, s, (float*)ctx->cand.p,
-                                (uint32_t*)ctx->cand_cnt.p, (unsigned long long*)ctx->fgbits.p));
+                                (uint32_t*)ctx->cand_cnt.p));
   HIPCHK(launch_ccl_candidates(cand.rec, cand.cnt, cand.ntiles,
-                               (const unsigned long long*)ctx->fgbits.p, labels,
+                               (unsigned long long*)ctx->fgbits.p, labels,
                                (uint32_t*)ctx->cand_lab.p, (unsigned long long*)ctx->rmask.p,
                                nb * (kRootBlock / 64), dense, W, H, D, s));
   int rc = regional_features_t<TH, TN>(ctx, s, labels, hess, norm, img, W, H, D, p->is_ave_mode,
@@ -357,15 +357,14 @@ hipError_t batch_tile(hdog_ctx* ctx, BatchState* B, int t, hipStream_t s) {
     return e;
   const i64 ntiles = hessian_tiles(W, H, D);
   if ((e = launch_hessian<TD, TH>(dog, hess, nullptr, W, H, D, s, (float*)B->cand.p,
-                                  (uint32_t*)B->cand_cnt.p, (unsigned long long*)B->fgbits.p)) !=
-      hipSuccess)
+                                  (uint32_t*)B->cand_cnt.p)) != hipSuccess)
     return e;
   i64 nbo = 0;
   uint32_t* dR = (uint32_t*)B->dR.p;
   unsigned long long* masks = (unsigned long long*)B->rmask.p;
   const i64 nb = (N + kRootBlock - 1) / kRootBlock;
   if ((e = launch_ccl_candidates((const float*)B->cand.p, (const uint32_t*)B->cand_cnt.p, ntiles,
-                                 (const unsigned long long*)B->fgbits.p, labels,
+                                 (unsigned long long*)B->fgbits.p, labels,
                                  (uint32_t*)B->cand_lab.p, masks, nb * (kRootBlock / 64), false, W,
                                  H, D, s)) != hipSuccess)
     return e;

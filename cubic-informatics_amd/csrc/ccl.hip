@@ -354,10 +354,10 @@ __global__ __launch_bounds__(NTM) void k_ccl_merge(uint32_t* L, int W, int H, in
 // (the foreground of the CCL) in the tile.  Labeling then touches only candidates: P (a
 // volume-sized parent array) is written and read at candidate voxels only, and the
 // per-record labels go to rlab (slot-aligned with the records).
-//   kernel A: per tile, a union-find in LDS over the tile's candidates (links inside the
-//             clipped tile, no wrap; neighbours tested in the tile's rows of the candidate
-//             bit mask, which the Hessian pass stores tile-major); rlab = the global index
-//             of v's tile-local root, the tile-local minimum, also in P where P is read;
+//   kernel A: per tile, the tile's candidate bit mask (one word per row, stored tile-major
+//             for kernel B) and a union-find in LDS over the tile's candidates (links
+//             inside the clipped tile, no wrap); rlab = the global index of v's tile-local
+//             root, the tile-local minimum, also in P where P is read;
 //   kernel B: per tile, the candidates on the tile's rim link each candidate neighbour
 //             outside the clipped tile (by linear offset, wrapping like the reference) with
 //             the global union-find on P;
@@ -417,7 +417,7 @@ __device__ __forceinline__ size_t mask_word(uint32_t x, uint32_t y, uint32_t z, 
 
 __global__ __launch_bounds__(NTR) void k_cand_local(const float* __restrict__ cand,
                                                    const uint32_t* __restrict__ cand_cnt,
-                                                   const unsigned long long* __restrict__ fgbits,
+                                                   unsigned long long* __restrict__ fgbits,
                                                    uint32_t* __restrict__ P,
                                                    uint32_t* __restrict__ rlab, int W, int H,
                                                    int D, int nbx, int nby, int links) {
@@ -425,13 +425,18 @@ __global__ __launch_bounds__(NTR) void k_cand_local(const float* __restrict__ ca
   const uint32_t n = cand_cnt[tile];
   __shared__ unsigned long long bits[RROWS];  // the tile's mask rows (word = ly + RTY lz)
   __shared__ uint32_t lab[RTV];              // parents, valid at candidate positions
-  if (n == 0) return;  // uniform over the work-group
   const float* rec = cand + (size_t)tile * kCandTile * 7;
   TileRecs R;
   R.load(rec, n);
-  if (threadIdx.x < RROWS) bits[threadIdx.x] = fgbits[(size_t)tile * RROWS + threadIdx.x];
-  R.each(rec, n, [&](uint32_t, uint32_t v) { lab[v] = v; });
+  if (threadIdx.x < RROWS) bits[threadIdx.x] = 0ull;
   __syncthreads();
+  R.each(rec, n, [&](uint32_t, uint32_t v) {
+    lab[v] = v;
+    atomicOr(bits + (v >> 6), 1ull << (v & 63));
+  });
+  __syncthreads();
+  if (threadIdx.x < RROWS) fgbits[(size_t)tile * RROWS + threadIdx.x] = bits[threadIdx.x];
+  if (n == 0) return;  // uniform over the work-group
   const CandTile T = cand_tile(tile, W, H, D, nbx, nby);
   const uint32_t Wu = (uint32_t)W, WH = Wu * (uint32_t)H;
   if (links)
@@ -674,7 +679,7 @@ hipError_t launch_ccl(const uint8_t* pd, uint32_t* labels, int W, int H, int D, 
 }
 
 hipError_t launch_ccl_candidates(const float* cand, const uint32_t* cand_cnt, i64 ntiles,
-                                 const unsigned long long* fgbits, uint32_t* P, uint32_t* rlab,
+                                 unsigned long long* fgbits, uint32_t* P, uint32_t* rlab,
                                  unsigned long long* roots, i64 root_words, bool dense, int W,
                                  int H, int D, hipStream_t s) {
   const int nbx = (W + RTX - 1) / RTX, nby = (H + RTY - 1) / RTY;
@@ -690,7 +695,7 @@ hipError_t launch_ccl_candidates(const float* cand, const uint32_t* cand_cnt, i6
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark("ccl/cand_local", s);
   hipLaunchKernelGGL(k_cand_merge, dim3((unsigned)ntiles), dim3(NTR), 0, s, cand, cand_cnt,
-                     fgbits, P, rlab, W, H, D, nbx, nby, generic,
+                     (const unsigned long long*)fgbits, P, rlab, W, H, D, nbx, nby, generic,
                      positive_offsets(W, H));
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark("ccl/cand_merge", s);

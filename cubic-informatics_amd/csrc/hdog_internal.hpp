@@ -184,13 +184,9 @@ template <typename TD, typename TH>
 // (kHessTX x kHessTY x kHessTZ, tiles x-fastest), the number of candidate voxels and their
 // records: the voxel's position in the tile, lx + kHessTX (ly + kHessTY lz), then the six
 // elements rounded to TH, as 7 SoA words of kCandTile slots; cand holds hessian_tiles() *
-// kCandTile * 7 words.  fgbits (optional, fg_words() words): the candidate bit mask, one
-// word per (z, y) row of each tile, tile-major (word (z - z0) kHessTY + (y - y0) of the
-// tile); words of rows beyond the volume's depth are not written.  Voxel 0 never has a
-// record or a bit (its label is 0 whatever its mask).
+// kCandTile * 7 words.  Voxel 0 never has a record (its label is 0 whatever its mask).
 hipError_t launch_hessian(const TD* dog, TH* hess, uint8_t* pd, int W, int H, int D,
-                          hipStream_t s, float* cand = nullptr, uint32_t* cand_cnt = nullptr,
-                          unsigned long long* fgbits = nullptr);
+                          hipStream_t s, float* cand = nullptr, uint32_t* cand_cnt = nullptr);
 // CCL: labels out; union-find in place
 // flatten = false leaves non-root voxels pointing at an ancestor of their root; the region
 // numbering (launch_region_ids with flatten = true) then finishes the labels in its pass.
@@ -237,14 +233,14 @@ hipError_t launch_tile_reserve(const uint32_t* d_R, i64 acc_cap, i64 arena_cap, 
                                TileOut* t, hipStream_t s);
 hipError_t launch_fill_u64(u64* p, i64 n, u64 value, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, i64 n, uint32_t value, hipStream_t s);
-// Labeling of the candidates the Hessian pass recorded (cand / cand_cnt / fgbits, see
-// launch_hessian): P is a volume-sized parent array written and read at candidate voxels
-// only; rlab (slot-aligned
+// Labeling of the candidates the Hessian pass recorded (cand / cand_cnt, see
+// launch_hessian): fgbits (fg_words() words) receives the candidate bit mask; P is a
+// volume-sized parent array written and read at candidate voxels only; rlab (slot-aligned
 // with the records) receives each record's label, the component's minimum linear index;
 // roots (root_words 64-bit words, cleared here) receives the root bit of every component,
 // in linear order.  dense: P becomes the full label volume (0 for the background).
 hipError_t launch_ccl_candidates(const float* cand, const uint32_t* cand_cnt, i64 ntiles,
-                                 const unsigned long long* fgbits, uint32_t* P, uint32_t* rlab,
+                                 unsigned long long* fgbits, uint32_t* P, uint32_t* rlab,
                                  unsigned long long* roots, i64 root_words, bool dense, int W,
                                  int H, int D, hipStream_t s);
 // Roots per numbering block from a root bit mask in linear order, then the exclusive scan

@@ -75,8 +75,7 @@ template <typename TD, typename TS, bool BAND>
 __global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* __restrict__ hess,
                                                 uint8_t* __restrict__ pd, int W, int H, int D,
                                                 int nbx, int nby, float* __restrict__ cand,
-                                                uint32_t* __restrict__ cand_cnt,
-                                                unsigned long long* __restrict__ fgbits) {
+                                                uint32_t* __restrict__ cand_cnt) {
   constexpr int DX = TX + 4, DY = TY + 4;        // dog slice with halo 2
   constexpr int XX = TX + 2, XY = TY + 2;        // Hx slice with halo 1 in x and y
   constexpr int YX = TX, YY = TY + 2;            // Hy slice with halo 1 in y
@@ -91,8 +90,6 @@ __global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* 
   if (threadIdx.x == 0) s_cnt = 0;
   const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
   float* rec = cand ? cand + (size_t)tile * kCandTile * 7 : nullptr;
-  // the tile's rows of the candidate bit mask (tile-major; word (z - z0) TY + ty)
-  unsigned long long* rows = fgbits ? fgbits + (size_t)tile * (TY * ZC) : nullptr;
   const int lane = threadIdx.x & 63;
   unsigned b = tile;
   const int bx = b % nbx; b /= nbx;
@@ -266,7 +263,6 @@ __global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* 
         // candidate records: one LDS reservation per wave (a wave is one row of 64 x),
         // consecutive slots for its candidates in lane order
         const unsigned long long bal = __ballot(is_cand);
-        if (rows && lane == 0) rows[(z - z0) * TY + ty] = bal;
         if (bal) {
           uint32_t base = 0;
           if (lane == 0) base = atomicAdd(&s_cnt, (uint32_t)__popcll(bal));
@@ -302,8 +298,7 @@ __global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* 
 
 template <typename TD, typename TH>
 hipError_t launch_hessian(const TD* dog, TH* hess, uint8_t* pd, int W, int H, int D,
-                          hipStream_t s, float* cand, uint32_t* cand_cnt,
-                          unsigned long long* fgbits) {
+                          hipStream_t s, float* cand, uint32_t* cand_cnt) {
   const int nbx = (W + TX - 1) / TX, nby = (H + TY - 1) / TY, nbz = (D + ZC - 1) / ZC;
   const i64 nblk = (i64)nbx * nby * nbz;
   // slices are read through buffer descriptors with 32-bit byte offsets
@@ -311,15 +306,14 @@ hipError_t launch_hessian(const TD* dog, TH* hess, uint8_t* pd, int W, int H, in
   // reduced-precision storage anywhere on the path: FP64 re-evaluation band for the minors
   constexpr bool band = !std::is_same<TD, float>::value || !std::is_same<TH, float>::value;
   hipLaunchKernelGGL((k_hessian<TD, TH, band>), dim3((unsigned)nblk), dim3(NT), 0, s, dog, hess,
-                     pd, W, H, D, nbx, nby, cand, cand_cnt, fgbits);
+                     pd, W, H, D, nbx, nby, cand, cand_cnt);
   mark("hessian/hessian_pd", s);
   return hipGetLastError();
 }
 
 #define HDOG_INST_HESS(TD, TH)                                                          \
   template hipError_t launch_hessian<TD, TH>(const TD*, TH*, uint8_t*, int, int, int, \
-                                             hipStream_t, float*, uint32_t*,                \
-                                             unsigned long long*);
+                                             hipStream_t, float*, uint32_t*);
 HDOG_INST_HESS(float, float)
 HDOG_INST_HESS(__half, __half)
 HDOG_INST_HESS(__hip_bfloat16, __hip_bfloat16)
