@@ -131,11 +131,87 @@ __device__ __forceinline__ uint32_t lds_unite(uint32_t* lab, uint32_t a, uint32_
   }
 }
 
+// The same union-find on 16-bit parents (tile positions below 2^16): half the LDS, so
+// twice the work-groups per CU.  LDS has no 16-bit atomic minimum; the link is a 32-bit
+// compare-and-swap on the word holding the entry, which fails and retries when either
+// half changed.  Path-halving stores stay plain 16-bit stores, as above.
+__device__ __forceinline__ uint32_t lds16_find(const uint16_t* lab, uint32_t x) {
+  uint32_t p = lab[x];
+  while (p != x) {
+    x = p;
+    p = lab[x];
+  }
+  return x;
+}
+__device__ __forceinline__ uint32_t lds16_find_halve(uint16_t* lab, uint32_t x) {
+  uint32_t p = lab[x];
+  while (p != x) {
+    const uint32_t gp = lab[p];
+    if (gp != p) lab[x] = (uint16_t)gp;
+    x = p;
+    p = gp;
+  }
+  return x;
+}
+// lab[a] = min(lab[a], b); returns the old lab[a]
+__device__ __forceinline__ uint32_t lds16_min(uint16_t* lab, uint32_t a, uint32_t b) {
+  uint32_t* w = reinterpret_cast<uint32_t*>(lab + (a & ~1u));
+  const int sh = (int)(a & 1u) * 16;
+  uint32_t cur = *w;
+  while (true) {
+    const uint32_t old = (cur >> sh) & 0xFFFFu;
+    if (old <= b) return old;
+    const uint32_t prev = atomicCAS(w, cur, (cur & ~(0xFFFFu << sh)) | (b << sh));
+    if (prev == cur) return old;
+    cur = prev;
+  }
+}
+__device__ __forceinline__ uint32_t lds16_unite(uint16_t* lab, uint32_t a, uint32_t b) {
+  while (true) {
+    a = lds16_find_halve(lab, a);
+    b = lds16_find_halve(lab, b);
+    if (a == b) return a;
+    if (a < b) { const uint32_t t = a; a = b; b = t; }
+    const uint32_t old = lds16_min(lab, a, b);
+    if (old == a) return b;
+    a = old;
+  }
+}
+
 __device__ __forceinline__ void fwd_offset(int k, int& dx, int& dy, int& dz) {
   if (k < 9) { dz = 1; dy = k / 3 - 1; dx = k % 3 - 1; }
   else if (k < 12) { dz = 0; dy = 1; dx = k - 10; }
   else { dz = 0; dy = 0; dx = 1; }
 }
+
+// Redundant links.  Every 26-adjacent candidate pair is linked by its lower voxel (in
+// the tile's local pass when both lie in the tile, else in the merge), so a voxel v can
+// skip a foreground forward neighbour b that is adjacent to another forward neighbour w
+// already known to be in v's set: b and w are both above v, so the pair (w, b) is linked
+// by a voxel above v, and by induction from the top voxel down it ends in one set.  Two
+// forward offsets are adjacent when they differ by at most 1 on every axis (the linear
+// offset between the two voxels is then one of the reference's offsets, wrapped or not).
+// (0, 0, +1) is adjacent to all other twelve, so it is visited first: with it present, one
+// link replaces up to thirteen.
+constexpr int fdx(int k) { return k < 9 ? k % 3 - 1 : k < 12 ? k - 10 : 1; }
+constexpr int fdy(int k) { return k < 9 ? k / 3 - 1 : k < 12 ? 1 : 0; }
+constexpr int fdz(int k) { return k < 9 ? 1 : 0; }
+constexpr uint32_t fwd_adj(int q) {  // bit p: forward offsets p != q, q adjacent
+  uint32_t m = 0;
+  for (int p = 0; p < 13; ++p) {
+    const int ax = fdx(p) - fdx(q), ay = fdy(p) - fdy(q), az = fdz(p) - fdz(q);
+    if (p != q && ax >= -1 && ax <= 1 && ay >= -1 && ay <= 1 && az >= -1 && az <= 1)
+      m |= 1u << p;
+  }
+  return m;
+}
+constexpr int fwd_order(int j) {
+  return j == 0 ? 4 : j == 1 ? 10 : j <= 5 ? j - 2 : j <= 9 ? j - 1 : j == 10 ? 9 : j;
+}
+static_assert(fwd_adj(4) == 0x1FFFu - (1u << 4), "(0, 0, +1) neighbours all others");
+static_assert(fwd_order(2) == 0 && fwd_order(5) == 3 && fwd_order(6) == 5 && fwd_order(9) == 8 &&
+                  fwd_order(10) == 9 && fwd_order(11) == 11 && fwd_order(12) == 12,
+              "every forward offset once");
 
 // Kernel 1.  With 256 threads on a 32 x 8 x 8 tile, thread t owns the column (t % 32,
 // t / 32) of the tile and k = 0..7 walks its z; every index is 32-bit (N < 2^32).  The
@@ -363,13 +439,15 @@ __global__ __launch_bounds__(NTM) void k_ccl_merge(uint32_t* L, int W, int H, in
 //             the global union-find on P;
 //   kernel C: every record's root (the component minimum), root bits for the region
 //             numbering, and with `dense` the final label written back into P.
-constexpr int RTX = kHessTX, RTY = kHessTY, RTZ = kHessTZ, RTV = RTX * RTY * RTZ, NTR = 512;
+// 256 threads and 16-bit LDS parents (17 KB of LDS per tile): eight tiles share a CU.
+constexpr int RTX = kHessTX, RTY = kHessTY, RTZ = kHessTZ, RTV = RTX * RTY * RTZ, NTR = 256;
 constexpr int RROWS = RTY * RTZ;  // mask words per tile
+static_assert(RTV <= 65536, "tile positions fit the 16-bit parents");
 // Each thread keeps the positions of its first KREC records (slots t + k NTR) in registers,
 // loaded together: a tile's record list is then read once, with all loads in flight,
 // instead of one dependent load per record and phase.  Tiles with more than KREC * NTR
 // candidates take their remaining slots with a plain loop.
-constexpr int KREC = 4;
+constexpr int KREC = 2048 / NTR;
 struct TileRecs {
   uint32_t v[KREC];
   __device__ __forceinline__ void load(const float* rec, uint32_t n) {
@@ -389,6 +467,20 @@ struct TileRecs {
     }
 #pragma unroll 1
     for (uint32_t i = threadIdx.x + KREC * NTR; i < n; i += NTR) f(i, __float_as_uint(rec[i]));
+  }
+  // the same visit with f at one call site (a rolled loop; the held positions rotate
+  // down one register per record), for bodies too large to copy once per record
+  template <typename F>
+  __device__ __forceinline__ void each_rolled(const float* rec, uint32_t n, F&& f) const {
+    uint32_t w[KREC];
+#pragma unroll
+    for (int k = 0; k < KREC; ++k) w[k] = v[k];
+#pragma unroll 1
+    for (uint32_t i = threadIdx.x; i < n; i += NTR) {
+      f(i, i < (uint32_t)(KREC * NTR) ? w[0] : __float_as_uint(rec[i]));
+#pragma unroll
+      for (int k = 0; k + 1 < KREC; ++k) w[k] = w[k + 1];
+    }
   }
 };
 static_assert(RTX == 64 && RTV == kCandTile, "a tile row is one mask word");
@@ -424,7 +516,7 @@ __global__ __launch_bounds__(NTR) void k_cand_local(const float* __restrict__ ca
   const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
   const uint32_t n = cand_cnt[tile];
   __shared__ unsigned long long bits[RROWS];  // the tile's mask rows (word = ly + RTY lz)
-  __shared__ uint32_t lab[RTV];              // parents, valid at candidate positions
+  __shared__ uint16_t lab[RTV];              // parents, valid at candidate positions
   const float* rec = cand + (size_t)tile * kCandTile * 7;
   TileRecs R;
   R.load(rec, n);
@@ -440,19 +532,27 @@ __global__ __launch_bounds__(NTR) void k_cand_local(const float* __restrict__ ca
   const CandTile T = cand_tile(tile, W, H, D, nbx, nby);
   const uint32_t Wu = (uint32_t)W, WH = Wu * (uint32_t)H;
   if (links)
-    R.each(rec, n, [&](uint32_t, uint32_t pv) {
+    R.each_rolled(rec, n, [&](uint32_t, uint32_t pv) __attribute__((always_inline)) {
       const int v = (int)pv;
       const int vx = v & (RTX - 1), vy = (v / RTX) & (RTY - 1), vz = v / (RTX * RTY);
-      uint32_t root = (uint32_t)v;  // a representative of v's set, refreshed by each link
+      uint32_t fg = 0u;  // bit q: forward neighbour q is a candidate inside the clipped tile
 #pragma unroll
       for (int q = 0; q < 13; ++q) {
-        int dx, dy, dz;
-        fwd_offset(q, dx, dy, dz);
-        const int nx = vx + dx, ny = vy + dy, nz = vz + dz;
+        const int nx = vx + fdx(q), ny = vy + fdy(q), nz = vz + fdz(q);
         if (nx < 0 || nx >= T.ex || ny < 0 || ny >= T.ey || nz >= T.ez) continue;
-        if (!((bits[ny + RTY * nz] >> nx) & 1ull)) continue;
-        const int u = nx + RTX * (ny + RTY * nz);
-        if (lab[u] != root) root = lds_unite(lab, root, (uint32_t)u);
+        if ((bits[ny + RTY * nz] >> nx) & 1ull) fg |= 1u << q;
+      }
+      uint32_t root = (uint32_t)v;  // a representative of v's set, refreshed by each link
+      uint32_t known = 0u;          // neighbours already in v's set
+#pragma unroll
+      for (int j = 0; j < 13; ++j) {
+        const int q = fwd_order(j);
+        if (!((fg >> q) & 1u)) continue;
+        const bool skip = (fwd_adj(q) & known) != 0u;
+        known |= 1u << q;
+        if (skip) continue;
+        const int u = (vx + fdx(q)) + RTX * ((vy + fdy(q)) + RTY * (vz + fdz(q)));
+        if (lab[u] != root) root = lds16_unite(lab, root, (uint32_t)u);
       }
     });
   __syncthreads();
@@ -465,7 +565,7 @@ __global__ __launch_bounds__(NTR) void k_cand_local(const float* __restrict__ ca
   // forward neighbours reach (the bottom face and the x / y faces of the clipped tile);
   // interior entries stay unwritten (kernel C writes every entry when `dense`)
   R.each(rec, n, [&](uint32_t i, uint32_t v) {
-    const uint32_t r = lds_find(lab, v);
+    const uint32_t r = lds16_find(lab, v);
     const uint32_t label = global(r);
     const int lx = (int)v & (RTX - 1), ly = (int)(v / RTX) & (RTY - 1), lz = (int)(v / (RTX * RTY));
     if (r == v || lz == 0 || lx == 0 || lx == T.ex - 1 || ly == 0 || ly == T.ey - 1)
@@ -511,7 +611,9 @@ __global__ __launch_bounds__(NTR) void k_cand_merge(const float* __restrict__ ca
   __shared__ uint16_t rim_v[RIM_MAX];
   __shared__ uint32_t rim_me[RIM_MAX];
   __shared__ uint32_t n_rim;
+  __shared__ unsigned long long tb[RROWS];  // the tile's own mask rows
   if (threadIdx.x == 0) n_rim = 0;
+  if (threadIdx.x < RROWS) tb[threadIdx.x] = fgbits[(size_t)tile * RROWS + threadIdx.x];
   TileRecs R;
   R.load(rec, n);
   __syncthreads();
@@ -552,14 +654,17 @@ __global__ __launch_bounds__(NTR) void k_cand_merge(const float* __restrict__ ca
     const int lx = v & (RTX - 1), ly = (v / RTX) & (RTY - 1), lz = v / (RTX * RTY);
     const int x = T.x0 + lx, y = T.y0 + ly, z = T.z0 + lz;
     const uint32_t me = rim_me[j];
-    uint32_t lj[13];
+    // candidate neighbours inside the clipped tile (linked to v by kernel A) and outside
+    uint32_t in_fg = 0u, out_fg = 0u, uidx[13];
 #pragma unroll
     for (int q = 0; q < 13; ++q) {
-      int dx, dy, dz;
-      fwd_offset(q, dx, dy, dz);
-      lj[q] = 0u;
+      const int dx = fdx(q), dy = fdy(q), dz = fdz(q);
+      uidx[q] = 0u;
       const int nx = lx + dx, ny = ly + dy, nz = lz + dz;
-      if (nx >= 0 && nx < T.ex && ny >= 0 && ny < T.ey && nz < T.ez) continue;  // kernel A
+      if (nx >= 0 && nx < T.ex && ny >= 0 && ny < T.ey && nz < T.ez) {
+        if ((tb[ny + RTY * nz] >> nx) & 1ull) in_fg |= 1u << q;
+        continue;
+      }
       // the neighbour at linear offset dz WH + dy W + dx (wrapping across x and y like the
       // reference; one carry per axis for width, height >= 2); it exists below N
       int ux = x + dx, uy = y + dy, uz = z + dz;
@@ -567,9 +672,23 @@ __global__ __launch_bounds__(NTR) void k_cand_merge(const float* __restrict__ ca
       if (uy < 0) { uy += H; --uz; } else if (uy >= H) { uy -= H; ++uz; }
       if (uz >= D) continue;
       if ((fgbits[mask_word((uint32_t)ux, (uint32_t)uy, (uint32_t)uz, nbx, nby)] >> (ux & 63)) &
-          1ull)
-        lj[q] = P[(uint32_t)uz * WH + (uint32_t)uy * Wu + (uint32_t)ux];
+          1ull) {
+        out_fg |= 1u << q;
+        uidx[q] = (uint32_t)uz * WH + (uint32_t)uy * Wu + (uint32_t)ux;
+      }
     }
+    // outside neighbours not adjacent to one already in v's set (those inside included)
+    uint32_t known = in_fg, link = 0u;
+#pragma unroll
+    for (int k = 0; k < 13; ++k) {
+      const int q = fwd_order(k);
+      if (!((out_fg >> q) & 1u)) continue;
+      if (!(fwd_adj(q) & known)) link |= 1u << q;
+      known |= 1u << q;
+    }
+    uint32_t lj[13];
+#pragma unroll
+    for (int q = 0; q < 13; ++q) lj[q] = ((link >> q) & 1u) ? P[uidx[q]] : 0u;
     uint32_t seen1 = 0u, seen2 = 0u;
 #pragma unroll
     for (int q = 0; q < 13; ++q) {
