@@ -21,8 +21,8 @@ struct Buf {
 struct BatchState {
   std::vector<hdog_tile> tiles;
   hdog_params p{};
-  Buf tmp1, tmp2, norm, dog, pd, labels, rid, bcnt, rmask, dR, acc, weights, tout, running, cand,
-      cand_cnt, fgbits, cand_lab;
+  Buf tmp1, tmp2, norm, dog, pd, labels, rid, bcnt, rmask, rmulti, dR, acc, weights, tout,
+      running, cand, cand_cnt, fgbits, cand_lab;
   Buf a_label, a_size, a_vox, a_cen, a_eig, a_nrm, a_hess, a_isum, a_imax;
   i64 acc_cap = 0, arena_cap = 0;
   hipStream_t cs = nullptr;
@@ -36,8 +36,8 @@ struct BatchState {
 struct hdog_ctx {
   int device = 0;
   std::string err;
-  Buf tmp1, tmp2, rid, bcnt, rmask, dR, acc, weights, blobs, sw, sums, ddog, cand, cand_cnt,
-      fgbits, cand_lab;
+  Buf tmp1, tmp2, rid, bcnt, rmask, rmulti, dR, acc, weights, blobs, sw, sums, ddog, cand,
+      cand_cnt, fgbits, cand_lab;
   Buf o_label, o_size, o_vox, o_cen, o_eig, o_nrm, o_hess, o_isum, o_imax;
   i64 out_cap = 0;
   Buf img, norm, dog, hess, pd, labels;  // substack buffers of hdog_process_substack
@@ -149,6 +149,7 @@ struct CandRecords {
   const float* rec = nullptr;
   const uint32_t* cnt = nullptr;
   const uint32_t* rlab = nullptr;  // each record's label (launch_ccl_candidates)
+  const unsigned long long* multi = nullptr;  // roots of multi-component regions (same)
   i64 ntiles = 0;
 };
 
@@ -185,7 +186,7 @@ int regional_features_t(hdog_ctx* ctx, hipStream_t s, uint32_t* labels, const TH
   HIPCHK(launch_region_assign(masks, (const uint32_t*)ctx->bcnt.p, (uint32_t*)ctx->rid.p, acc, R, R,
                               nullptr, N, s));
   if (cand.rec)
-    HIPCHK(launch_region_accum_cand<TN>(cand.rec, cand.cnt, cand.ntiles, cand.rlab,
+    HIPCHK(launch_region_accum_cand<TN>(cand.rec, cand.cnt, cand.ntiles, cand.rlab, cand.multi,
                                         (const uint32_t*)ctx->rid.p, norm, img, acc, R, W, H,
                                         is_ave, s));
   else
@@ -244,15 +245,18 @@ int run_stages(hdog_ctx* ctx, hipStream_t s, const hdog_params* p, const uint16_
   HIPCHK(ensure(ctx->cand_lab, (size_t)cand.ntiles * kCandTile * 4));
   HIPCHK(ensure(ctx->fgbits, (size_t)fg_words(W, H, D) * 8));
   HIPCHK(ensure(ctx->rmask, (size_t)nb * (kRootBlock / 8)));
+  HIPCHK(ensure(ctx->rmulti, (size_t)nb * (kRootBlock / 8)));
   cand.rec = (const float*)ctx->cand.p;
   cand.cnt = (const uint32_t*)ctx->cand_cnt.p;
   cand.rlab = (const uint32_t*)ctx->cand_lab.p;
+  cand.multi = (const unsigned long long*)ctx->rmulti.p;
   HIPCHK(launch_hessian<TD, TH>(dog, hess, pd, W, H, D, s, (float*)ctx->cand.p,
                                 (uint32_t*)ctx->cand_cnt.p));
   HIPCHK(launch_ccl_candidates(cand.rec, cand.cnt, cand.ntiles,
                                (unsigned long long*)ctx->fgbits.p, labels,
                                (uint32_t*)ctx->cand_lab.p, (unsigned long long*)ctx->rmask.p,
-                               nb * (kRootBlock / 64), dense, W, H, D, s));
+                               (unsigned long long*)ctx->rmulti.p, nb * (kRootBlock / 64), dense,
+                               W, H, D, s));
   int rc = regional_features_t<TH, TN>(ctx, s, labels, hess, norm, img, W, H, D, p->is_ave_mode,
                                        num_regions, cand);
   if (rc) return rc;
@@ -326,7 +330,7 @@ void batch_free(BatchState* B) {
   if (B->graph) (void)hipGraphDestroy(B->graph);
   if (B->cs) (void)hipStreamDestroy(B->cs);
   if (B->h_tout) (void)hipHostFree(B->h_tout);
-  for (Buf* b : {&B->tmp1, &B->tmp2, &B->norm, &B->dog, &B->pd, &B->labels, &B->rid, &B->rmask,
+  for (Buf* b : {&B->tmp1, &B->tmp2, &B->norm, &B->dog, &B->pd, &B->labels, &B->rid, &B->rmask, &B->rmulti,
                  &B->cand, &B->cand_cnt, &B->cand_lab, &B->fgbits,
                  &B->bcnt, &B->dR, &B->acc, &B->weights, &B->tout, &B->running, &B->a_label,
                  &B->a_size, &B->a_vox, &B->a_cen, &B->a_eig, &B->a_nrm, &B->a_hess, &B->a_isum,
@@ -365,8 +369,9 @@ hipError_t batch_tile(hdog_ctx* ctx, BatchState* B, int t, hipStream_t s) {
   const i64 nb = (N + kRootBlock - 1) / kRootBlock;
   if ((e = launch_ccl_candidates((const float*)B->cand.p, (const uint32_t*)B->cand_cnt.p, ntiles,
                                  (unsigned long long*)B->fgbits.p, labels,
-                                 (uint32_t*)B->cand_lab.p, masks, nb * (kRootBlock / 64), false, W,
-                                 H, D, s)) != hipSuccess)
+                                 (uint32_t*)B->cand_lab.p, masks,
+                                 (unsigned long long*)B->rmulti.p, nb * (kRootBlock / 64), false,
+                                 W, H, D, s)) != hipSuccess)
     return e;
   if ((e = launch_region_count_masks(masks, (uint32_t*)B->bcnt.p, dR, N, s, &nbo)) != hipSuccess)
     return e;
@@ -380,6 +385,7 @@ hipError_t batch_tile(hdog_ctx* ctx, BatchState* B, int t, hipStream_t s) {
     return e;
   if ((e = launch_region_accum_cand<TN>((const float*)B->cand.p, (const uint32_t*)B->cand_cnt.p,
                                         ntiles, (const uint32_t*)B->cand_lab.p,
+                                        (const unsigned long long*)B->rmulti.p,
                                         (const uint32_t*)B->rid.p, norm, T.image,
                                         acc, B->acc_cap, W, H, p->is_ave_mode, s)) != hipSuccess)
     return e;
@@ -444,7 +450,7 @@ int hdog_create(hdog_ctx** out, int device) {
 int hdog_release_scratch(hdog_ctx* ctx) {
   if (!ctx) return HDOG_ERR_ARG;
   (void)hipSetDevice(ctx->device);
-  for (Buf* b : {&ctx->tmp1, &ctx->tmp2, &ctx->rid, &ctx->bcnt, &ctx->rmask, &ctx->dR, &ctx->acc,
+  for (Buf* b : {&ctx->tmp1, &ctx->tmp2, &ctx->rid, &ctx->bcnt, &ctx->rmask, &ctx->rmulti, &ctx->dR, &ctx->acc,
                  &ctx->blobs, &ctx->cand_lab, &ctx->fgbits,
                  &ctx->sw, &ctx->sums, &ctx->ddog,
                  &ctx->o_label, &ctx->o_size, &ctx->o_vox, &ctx->o_cen, &ctx->o_eig, &ctx->o_nrm,
@@ -898,6 +904,7 @@ int hdog_batch_plan(hdog_ctx* ctx, const hdog_params* p, const hdog_tile* tiles,
               {&B->labels, (size_t)maxN * 4}, {&B->rid, (size_t)maxN * 4},
               {&B->bcnt, (size_t)((maxN + kRootBlock - 1) / kRootBlock) * 4},
               {&B->rmask, (size_t)((maxN + kRootBlock - 1) / kRootBlock) * (kRootBlock / 8)},
+              {&B->rmulti, (size_t)((maxN + kRootBlock - 1) / kRootBlock) * (kRootBlock / 8)},
               {&B->dR, 64},
               {&B->acc, (size_t)B->acc_cap * sizeof(RegionAcc)}, {&B->weights, ctx->weights.bytes},
               {&B->tout, (size_t)n_tiles * sizeof(TileOut)}, {&B->running, 64},

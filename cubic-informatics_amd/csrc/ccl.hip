@@ -705,6 +705,7 @@ __global__ __launch_bounds__(NTR) void k_cand_flatten(const float* __restrict__ 
                                                      const uint32_t* __restrict__ cand_cnt,
                                                      uint32_t* P, uint32_t* __restrict__ rlab,
                                                      unsigned long long* __restrict__ roots,
+                                                     unsigned long long* __restrict__ multi,
                                                      int dense, int W, int H, int D, int nbx,
                                                      int nby) {
   const unsigned tile = blockIdx.x;
@@ -731,7 +732,10 @@ __global__ __launch_bounds__(NTR) void k_cand_flatten(const float* __restrict__ 
     const uint32_t r = find_root_readonly(P, l);
     rl[i] = r;
     if (dense && P[idx] != r) P[idx] = r;
-    if (r == idx) atomicOr(roots + (r >> 6), 1ull << (r & 63));
+    if (r == idx)
+      atomicOr(roots + (r >> 6), 1ull << (r & 63));
+    else if (l == idx)  // the root of a tile-local component other than the region's first
+      atomicOr(multi + (r >> 6), 1ull << (r & 63));
   };
 #pragma unroll
   for (int k = 0; k < KREC; ++k) {
@@ -799,8 +803,9 @@ hipError_t launch_ccl(const uint8_t* pd, uint32_t* labels, int W, int H, int D, 
 
 hipError_t launch_ccl_candidates(const float* cand, const uint32_t* cand_cnt, i64 ntiles,
                                  unsigned long long* fgbits, uint32_t* P, uint32_t* rlab,
-                                 unsigned long long* roots, i64 root_words, bool dense, int W,
-                                 int H, int D, hipStream_t s) {
+                                 unsigned long long* roots, unsigned long long* multi,
+                                 i64 root_words, bool dense, int W, int H, int D,
+                                 hipStream_t s) {
   const int nbx = (W + RTX - 1) / RTX, nby = (H + RTY - 1) / RTY;
   hipError_t e;
   if (dense) {  // background of the materialized label volume
@@ -819,8 +824,9 @@ hipError_t launch_ccl_candidates(const float* cand, const uint32_t* cand_cnt, i6
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark("ccl/cand_merge", s);
   if ((e = launch_fill_u64((u64*)roots, root_words, 0ull, s)) != hipSuccess) return e;
+  if ((e = launch_fill_u64((u64*)multi, root_words, 0ull, s)) != hipSuccess) return e;
   hipLaunchKernelGGL(k_cand_flatten, dim3((unsigned)ntiles), dim3(NTR), 0, s, cand, cand_cnt, P,
-                     rlab, roots, dense ? 1 : 0, W, H, D, nbx, nby);
+                     rlab, roots, multi, dense ? 1 : 0, W, H, D, nbx, nby);
   mark("ccl/cand_flatten", s);
   return hipGetLastError();
 }

@@ -208,11 +208,15 @@ hipError_t launch_region_accum(const uint32_t* labels, const uint32_t* rid, cons
                                const TN* norm, const uint16_t* img, RegionAcc* acc, i64 cap,
                                int W, int H, int D, int is_ave, hipStream_t s);
 // Region sums from the candidate records of launch_hessian (ntiles = hessian_tiles()).
+// rlab and multi come from launch_ccl_candidates: each record's root, and the roots of
+// regions made of more than one tile-local component (all others are summed by one
+// work-group and stored without atomics).
 template <typename TN>
 hipError_t launch_region_accum_cand(const float* cand, const uint32_t* cand_cnt, i64 ntiles,
-                                    const uint32_t* rlab, const uint32_t* rid, const TN* norm,
-                                    const uint16_t* img, RegionAcc* acc, i64 cap, int W, int H,
-                                    int is_ave, hipStream_t s);
+                                    const uint32_t* rlab, const unsigned long long* multi,
+                                    const uint32_t* rid, const TN* norm, const uint16_t* img,
+                                    RegionAcc* acc, i64 cap, int W, int H, int is_ave,
+                                    hipStream_t s);
 struct RegionOut {
   uint32_t* label; uint16_t* size; u64* voxels; float* cx; float* cy; float* cz;
   float* st; float* bl; float* nrm; float* hess; u64* isum; uint16_t* imax;
@@ -238,11 +242,14 @@ hipError_t launch_fill_u32(uint32_t* p, i64 n, uint32_t value, hipStream_t s);
 // volume-sized parent array written and read at candidate voxels only; rlab (slot-aligned
 // with the records) receives each record's label, the component's minimum linear index;
 // roots (root_words 64-bit words, cleared here) receives the root bit of every component,
-// in linear order.  dense: P becomes the full label volume (0 for the background).
+// in linear order, and multi (same size, cleared here) the root bit of every component
+// made of more than one tile-local component.  dense: P becomes the full label volume (0
+// for the background).
 hipError_t launch_ccl_candidates(const float* cand, const uint32_t* cand_cnt, i64 ntiles,
                                  unsigned long long* fgbits, uint32_t* P, uint32_t* rlab,
-                                 unsigned long long* roots, i64 root_words, bool dense, int W,
-                                 int H, int D, hipStream_t s);
+                                 unsigned long long* roots, unsigned long long* multi,
+                                 i64 root_words, bool dense, int W, int H, int D,
+                                 hipStream_t s);
 // Roots per numbering block from a root bit mask in linear order, then the exclusive scan
 // of the counts; *d_R = 1 + number of roots.
 hipError_t launch_region_count_masks(const unsigned long long* masks, uint32_t* block_cnt,

@@ -198,21 +198,40 @@ __device__ __forceinline__ void flush_global(RegionAcc* a, const RunSums& v, int
   }
 }
 
+// A region whose voxels all lie in one work-group's tiles: its sums are final after the
+// work-group's fold and are stored, not added (the accumulator is zeroed beforehand).
+__device__ __forceinline__ void store_final(RegionAcc* a, const RunSums& v) {
+  a->sx = v.sx;
+  a->sy = v.sy;
+  a->sz = v.sz;
+  a->isum = v.isum;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) a->h[k] = v.h[k];
+  a->nsum = v.ns;
+  a->cnt = v.cnt;
+  a->nmax_key = v.nkey;
+  a->imax = v.imax;
+}
+
 // Per-work-group hash table in LDS keyed by region id (open addressing, LDS atomics, FP64
 // for the Hessian and normalized sums).  Each occupied slot ends as one set of global
 // atomics, so a region pays one set per work-group that touches it instead of one per
-// voxel; a voxel that finds the table full goes straight to the global atomics.  The
-// u32 partial sums cannot overflow for the <= 8192 voxels a work-group folds in.
+// voxel; a voxel that finds the table full goes straight to the global atomics.  A slot
+// whose region the caller marks `owned` (no other work-group holds any of its voxels) is
+// stored instead, unless some voxel of the work-group found the table full.  The u32
+// partial sums cannot overflow for the <= 16384 voxels a work-group folds in.
 template <int S>
 struct LdsTable {
   uint32_t key[S];
-  uint32_t cnt[S], sx[S], sy[S], sz[S], is[S], im[S], nk[S];
+  uint32_t cnt[S], sx[S], sy[S], sz[S], is[S], im[S], nk[S], own[S];
   double h[6][S], ns[S];
+  uint32_t spilled;
 
   __device__ void init() {
+    if (threadIdx.x == 0) spilled = 0u;
     for (int t = threadIdx.x; t < S; t += blockDim.x) {
       key[t] = EMPTY;
-      cnt[t] = sx[t] = sy[t] = sz[t] = is[t] = im[t] = nk[t] = 0u;
+      cnt[t] = sx[t] = sy[t] = sz[t] = is[t] = im[t] = nk[t] = own[t] = 0u;
 #pragma unroll
       for (int k = 0; k < 6; ++k) h[k][t] = 0.0;
       ns[t] = 0.0;
@@ -220,7 +239,8 @@ struct LdsTable {
   }
 
   __device__ void add(RegionAcc* acc, uint32_t r, uint32_t x, uint32_t y, uint32_t z,
-                      const float* hv, float nv, uint32_t iv, int is_ave, bool has_img) {
+                      const float* hv, float nv, uint32_t iv, int is_ave, bool has_img,
+                      bool owned = false) {
     uint32_t slot = (r * 2654435761u) >> (32 - __builtin_ctz(S));
     bool placed = false;
     for (int probes = 0; probes < S; ++probes) {
@@ -237,8 +257,10 @@ struct LdsTable {
 #pragma unroll
       for (int k = 0; k < 6; ++k) v.h[k] = (double)hv[k];
       flush_global(acc + r, v, is_ave, has_img);
+      spilled = 1u;
       return;
     }
+    if (owned) own[slot] = 1u;  // the same value from every voxel of the region
     atomicAdd(&cnt[slot], 1u);
     atomicAdd(&sx[slot], x);
     atomicAdd(&sy[slot], y);
@@ -264,7 +286,10 @@ struct LdsTable {
       v.isum = is[t]; v.imax = im[t]; v.nkey = nk[t]; v.ns = ns[t];
 #pragma unroll
       for (int k = 0; k < 6; ++k) v.h[k] = h[k][t];
-      flush_global(acc + rr, v, is_ave, has_img);
+      if (own[t] && !spilled)
+        store_final(acc + rr, v);
+      else
+        flush_global(acc + rr, v, is_ave, has_img);
     }
   }
 };
@@ -334,6 +359,7 @@ template <typename TN>
 __global__ __launch_bounds__(NT) void k_accum_cand(const float* __restrict__ cand,
                                                    const uint32_t* __restrict__ cand_cnt,
                                                    const uint32_t* __restrict__ rlab,
+                                                   const unsigned long long* __restrict__ multi,
                                                    const uint32_t* __restrict__ rid,
                                                    const TN* __restrict__ norm,
                                                    const uint16_t* __restrict__ img,
@@ -374,10 +400,13 @@ __global__ __launch_bounds__(NT) void k_accum_cand(const float* __restrict__ can
       float hv[6];
 #pragma unroll
       for (int k = 0; k < 6; ++k) hv[k] = t[(size_t)kCandTile * (1 + k) + i];
-      const uint32_t r = rid[rl[i]];
+      const uint32_t root = rl[i];
+      const uint32_t r = rid[root];
+      // a region of one tile-local component lies in one tile, so in this work-group
+      const bool owned = !((multi[root >> 6] >> (root & 63)) & 1ull);
       const float nv = ld(norm, (i64)idx);
       const uint32_t iv = has_img ? (uint32_t)img[idx] : 0u;
-      if (r < cap) tab.add(acc, r, x, y, z, hv, nv, iv, is_ave, has_img);
+      if (r < cap) tab.add(acc, r, x, y, z, hv, nv, iv, is_ave, has_img, owned);
     }
   }
   __syncthreads();
@@ -548,14 +577,15 @@ hipError_t launch_region_accum(const uint32_t* labels, const uint32_t* rid, cons
 
 template <typename TN>
 hipError_t launch_region_accum_cand(const float* cand, const uint32_t* cand_cnt, i64 ntiles,
-                                    const uint32_t* rlab, const uint32_t* rid, const TN* norm,
-                                    const uint16_t* img, RegionAcc* acc, i64 cap, int W, int H,
-                                    int is_ave, hipStream_t s) {
+                                    const uint32_t* rlab, const unsigned long long* multi,
+                                    const uint32_t* rid, const TN* norm, const uint16_t* img,
+                                    RegionAcc* acc, i64 cap, int W, int H, int is_ave,
+                                    hipStream_t s) {
   const unsigned plane = (unsigned)(((W + kHessTX - 1) / kHessTX) * ((H + kHessTY - 1) / kHessTY));
   const unsigned nbz = (unsigned)(ntiles / plane);
   const i64 ngroups = (i64)plane * ((nbz + TPW - 1) / TPW);
   hipLaunchKernelGGL((k_accum_cand<TN>), dim3((unsigned)ngroups), dim3(NT), 0, s, cand, cand_cnt,
-                     rlab, rid, norm, img, acc, (uint32_t)std::min<i64>(cap, 0xFFFFFFFFll),
+                     rlab, multi, rid, norm, img, acc, (uint32_t)std::min<i64>(cap, 0xFFFFFFFFll),
                      (uint32_t)W, (uint32_t)((i64)W * H), is_ave, plane, nbz,
                      (unsigned)((W + kHessTX - 1) / kHessTX));
   mark("regions/accum_candidates", s);
@@ -571,10 +601,9 @@ HDOG_INST_ACC(__half, __half)
 HDOG_INST_ACC(__hip_bfloat16, __hip_bfloat16)
 #undef HDOG_INST_ACC
 #define HDOG_INST_ACCC(TN)                                                                  \
-  template hipError_t launch_region_accum_cand<TN>(const float*, const uint32_t*, i64,      \
-                                                   const uint32_t*, const uint32_t*, const TN*, \
-                                                   const uint16_t*, RegionAcc*, i64, int, int, \
-                                                   int, hipStream_t);
+  template hipError_t launch_region_accum_cand<TN>(                                         \
+      const float*, const uint32_t*, i64, const uint32_t*, const unsigned long long*,         \
+      const uint32_t*, const TN*, const uint16_t*, RegionAcc*, i64, int, int, int, hipStream_t);
 HDOG_INST_ACCC(float)
 HDOG_INST_ACCC(__half)
 HDOG_INST_ACCC(__hip_bfloat16)
