@@ -732,10 +732,17 @@ __global__ __launch_bounds__(NTR) void k_cand_flatten(const float* __restrict__ 
     const uint32_t r = find_root_readonly(P, l);
     rl[i] = r;
     if (dense && P[idx] != r) P[idx] = r;
-    if (r == idx)
+    if (r == idx) {
       atomicOr(roots + (r >> 6), 1ull << (r & 63));
-    else if (l == idx)  // the root of a tile-local component other than the region's first
-      atomicOr(multi + (r >> 6), 1ull << (r & 63));
+    } else if (l == idx) {
+      // the root of a tile-local component other than the region's first: the region
+      // reaches beyond the region pass's work-group of r when this component lies outside it
+      const uint32_t rz = r / WH, rr = r - rz * WH, ry = rr / Wu, rx = rr - ry * Wu;
+      const int ty = T.y0 / RTY, tz = T.z0 / RTZ;
+      if (rx / RTX != (uint32_t)T.x0 / RTX || (ry / RTY) / kAccGY != (uint32_t)(ty / kAccGY) ||
+          (rz / RTZ) / kAccGZ != (uint32_t)(tz / kAccGZ))
+        atomicOr(multi + (r >> 6), 1ull << (r & 63));
+    }
   };
 #pragma unroll
   for (int k = 0; k < KREC; ++k) {

@@ -31,6 +31,15 @@ inline i64 hessian_tiles(int W, int H, int D) {
 }
 // words of the candidate bit mask: one per (z, y) row of each Hessian tile, tile-major
 inline i64 fg_words(int W, int H, int D) { return hessian_tiles(W, H, D) * kHessTY * kHessTZ; }
+// Region sums from candidate records: one work-group folds kAccGY x kAccGZ Hessian tiles
+// (y, z); a region inside one such group is summed by that work-group alone.
+#ifndef HDOG_ACC_GY
+#define HDOG_ACC_GY 2
+#endif
+#ifndef HDOG_ACC_GZ
+#define HDOG_ACC_GZ 2
+#endif
+constexpr int kAccGY = HDOG_ACC_GY, kAccGZ = HDOG_ACC_GZ;
 
 // ---- storage types of float intermediates --------------------------------------
 template <typename T> __device__ __forceinline__ float ld(const T* p, i64 i);
@@ -209,8 +218,8 @@ hipError_t launch_region_accum(const uint32_t* labels, const uint32_t* rid, cons
                                int W, int H, int D, int is_ave, hipStream_t s);
 // Region sums from the candidate records of launch_hessian (ntiles = hessian_tiles()).
 // rlab and multi come from launch_ccl_candidates: each record's root, and the roots of
-// regions made of more than one tile-local component (all others are summed by one
-// work-group and stored without atomics).
+// regions reaching beyond one work-group's tiles (all others are summed by one work-group
+// and stored without atomics).
 template <typename TN>
 hipError_t launch_region_accum_cand(const float* cand, const uint32_t* cand_cnt, i64 ntiles,
                                     const uint32_t* rlab, const unsigned long long* multi,
@@ -243,8 +252,8 @@ hipError_t launch_fill_u32(uint32_t* p, i64 n, uint32_t value, hipStream_t s);
 // with the records) receives each record's label, the component's minimum linear index;
 // roots (root_words 64-bit words, cleared here) receives the root bit of every component,
 // in linear order, and multi (same size, cleared here) the root bit of every component
-// made of more than one tile-local component.  dense: P becomes the full label volume (0
-// for the background).
+// with voxels outside the kAccGY x kAccGZ tile group of its root.  dense: P becomes the
+// full label volume (0 for the background).
 hipError_t launch_ccl_candidates(const float* cand, const uint32_t* cand_cnt, i64 ntiles,
                                  unsigned long long* fgbits, uint32_t* P, uint32_t* rlab,
                                  unsigned long long* roots, unsigned long long* multi,

@@ -219,7 +219,7 @@ __device__ __forceinline__ void store_final(RegionAcc* a, const RunSums& v) {
 // voxel; a voxel that finds the table full goes straight to the global atomics.  A slot
 // whose region the caller marks `owned` (no other work-group holds any of its voxels) is
 // stored instead, unless some voxel of the work-group found the table full.  The u32
-// partial sums cannot overflow for the <= 16384 voxels a work-group folds in.
+// partial sums cannot overflow for the <= 2^15 voxels a work-group folds in (2^31 / 2^16).
 template <int S>
 struct LdsTable {
   uint32_t key[S];
@@ -347,14 +347,10 @@ __global__ __launch_bounds__(NT) void k_accum(const uint32_t* __restrict__ L,
 }
 
 // Region sums from the candidate records of the Hessian kernel: one work-group per
-// Hessian tile folds that tile's records (voxel index + six elements, SoA) into the table,
-// gathering the label, normalized and raw intensity of each candidate voxel; no pass
-// over the non-candidate voxels.
-#ifndef HDOG_ACC_TPW
-#define HDOG_ACC_TPW 2
-#endif
-constexpr int TPW = HDOG_ACC_TPW;  // z-consecutive Hessian tiles folded per work-group
-
+// kAccGY x kAccGZ Hessian tiles folds their records (voxel index + six elements, SoA) into
+// the table, gathering the label, normalized and raw intensity of each candidate voxel; no
+// pass over the non-candidate voxels.
+static_assert(kAccGY * kAccGZ * kCandTile <= (1 << 15), "u32 partial sums of x, y, z");
 template <typename TN>
 __global__ __launch_bounds__(NT) void k_accum_cand(const float* __restrict__ cand,
                                                    const uint32_t* __restrict__ cand_cnt,
@@ -364,20 +360,24 @@ __global__ __launch_bounds__(NT) void k_accum_cand(const float* __restrict__ can
                                                    const TN* __restrict__ norm,
                                                    const uint16_t* __restrict__ img,
                                                    RegionAcc* acc, uint32_t cap, uint32_t W,
-                                                   uint32_t WH, int is_ave, unsigned plane,
-                                                   unsigned nbz, unsigned nbx) {
-  // work-group p takes the tiles (x, y, TPW z + q), q < TPW, of the Hessian tiling with
-  // `plane` tiles per z level; regions spanning those tiles pay one set of global atomics
+                                                   uint32_t WH, int is_ave, unsigned nbx,
+                                                   unsigned nby, unsigned nbz) {
+  // work-group p takes the tiles (bx, kAccGY gy + a, kAccGZ gz + b) of the Hessian tiling;
+  // a region within them is final here, others pay one set of global atomics
+  constexpr int TPW = kAccGY * kAccGZ;
   const unsigned p = xcd_tile(blockIdx.x, gridDim.x);
-  const unsigned xy = p % plane, zq = p / plane;
+  const unsigned ngy = (nby + kAccGY - 1) / kAccGY;
+  const unsigned bx = p % nbx, gy = (p / nbx) % ngy, gz = p / nbx / ngy;
   uint32_t n[TPW];
-  unsigned tiles[TPW];
+  unsigned tiles[TPW], tby[TPW], tbz[TPW];
   uint32_t any = 0;
 #pragma unroll
   for (int q = 0; q < TPW; ++q) {
-    const unsigned z = zq * TPW + q;
-    tiles[q] = xy + z * plane;
-    n[q] = z < nbz ? cand_cnt[tiles[q]] : 0u;
+    const unsigned by = gy * kAccGY + q % kAccGY, bz = gz * kAccGZ + q / kAccGY;
+    tiles[q] = (bz * nby + by) * nbx + bx;
+    tby[q] = by;
+    tbz[q] = bz;
+    n[q] = (by < nby && bz < nbz) ? cand_cnt[tiles[q]] : 0u;
     any |= n[q];
   }
   if (any == 0) return;  // uniform over the work-group
@@ -390,8 +390,7 @@ __global__ __launch_bounds__(NT) void k_accum_cand(const float* __restrict__ can
     const float* t = cand + (size_t)tiles[q] * kCandTile * 7;
     const uint32_t* rl = rlab + (size_t)tiles[q] * kCandTile;  // each record's root label
     // tile origin: records hold the voxel's position in the tile
-    const uint32_t tx0 = (xy % nbx) * kHessTX, ty0 = (xy / nbx) * kHessTY,
-                   tz0 = (zq * TPW + q) * kHessTZ;
+    const uint32_t tx0 = bx * kHessTX, ty0 = tby[q] * kHessTY, tz0 = tbz[q] * kHessTZ;
     for (uint32_t i = threadIdx.x; i < n[q]; i += NT) {
       const uint32_t v = __float_as_uint(t[i]);
       const uint32_t x = tx0 + (v % kHessTX), y = ty0 + (v / kHessTX) % kHessTY,
@@ -402,7 +401,7 @@ __global__ __launch_bounds__(NT) void k_accum_cand(const float* __restrict__ can
       for (int k = 0; k < 6; ++k) hv[k] = t[(size_t)kCandTile * (1 + k) + i];
       const uint32_t root = rl[i];
       const uint32_t r = rid[root];
-      // a region of one tile-local component lies in one tile, so in this work-group
+      // the labeling marks every region that reaches beyond one work-group's tiles
       const bool owned = !((multi[root >> 6] >> (root & 63)) & 1ull);
       const float nv = ld(norm, (i64)idx);
       const uint32_t iv = has_img ? (uint32_t)img[idx] : 0u;
@@ -581,13 +580,14 @@ hipError_t launch_region_accum_cand(const float* cand, const uint32_t* cand_cnt,
                                     const uint32_t* rid, const TN* norm, const uint16_t* img,
                                     RegionAcc* acc, i64 cap, int W, int H, int is_ave,
                                     hipStream_t s) {
-  const unsigned plane = (unsigned)(((W + kHessTX - 1) / kHessTX) * ((H + kHessTY - 1) / kHessTY));
-  const unsigned nbz = (unsigned)(ntiles / plane);
-  const i64 ngroups = (i64)plane * ((nbz + TPW - 1) / TPW);
+  const unsigned nbx = (unsigned)((W + kHessTX - 1) / kHessTX),
+                 nby = (unsigned)((H + kHessTY - 1) / kHessTY);
+  const unsigned nbz = (unsigned)(ntiles / ((i64)nbx * nby));
+  const i64 ngroups =
+      (i64)nbx * ((nby + kAccGY - 1) / kAccGY) * ((nbz + kAccGZ - 1) / kAccGZ);
   hipLaunchKernelGGL((k_accum_cand<TN>), dim3((unsigned)ngroups), dim3(NT), 0, s, cand, cand_cnt,
                      rlab, multi, rid, norm, img, acc, (uint32_t)std::min<i64>(cap, 0xFFFFFFFFll),
-                     (uint32_t)W, (uint32_t)((i64)W * H), is_ave, plane, nbz,
-                     (unsigned)((W + kHessTX - 1) / kHessTX));
+                     (uint32_t)W, (uint32_t)((i64)W * H), is_ave, nbx, nby, nbz);
   mark("regions/accum_candidates", s);
   return hipGetLastError();
 }
