@@ -71,8 +71,10 @@ __device__ __forceinline__ uint8_t negdef_band(float xx, float xy, float xz, flo
 // second pass.  Each thread's positions in the slices do not change with z, so their
 // LDS and global offsets and their bounds tests are computed once, before the walk; ring
 // slots rotate instead of being recomputed; global indices are 32-bit (N < 2^32).
+// (the second bound keeps the register allocation within five waves per SIMD, the most
+// that the LDS rings allow)
 template <typename TD, typename TS, bool BAND>
-__global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* __restrict__ hess,
+__global__ __launch_bounds__(NT, 5) void k_hessian(const TD* __restrict__ dog, TS* __restrict__ hess,
                                                 uint8_t* __restrict__ pd, int W, int H, int D,
                                                 int nbx, int nby, float* __restrict__ cand,
                                                 uint32_t* __restrict__ cand_cnt) {
@@ -120,14 +122,14 @@ __global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* 
     const int e = threadIdx.x + j * NT, lx = e % XX, ly = e / XX;
     hxe[j] = e < XY * XX;
     hxv[j] = hxe[j] && inplane(x0 - 1 + lx, y0 - 1 + ly);
-    hxi[j] = (ly + 1) * DX + lx;  // Hx = g2(d[hxi + 2], d[hxi])
+    hxi[j] = hxe[j] ? (ly + 1) * DX + lx : 0;  // Hx = g2(d[hxi + 2], d[hxi])
   }
 #pragma unroll
   for (int j = 0; j < NHY; ++j) {
     const int e = threadIdx.x + j * NT, lx = e % YX, ly = e / YX;
     hye[j] = e < YY * YX;
     hyv[j] = hye[j] && inplane(x0 + lx, y0 - 1 + ly);
-    hyi[j] = ly * DX + lx + 2;  // Hy = g2(d[hyi + 2 DX], d[hyi])
+    hyi[j] = hye[j] ? ly * DX + lx + 2 : 0;  // Hy = g2(d[hyi + 2 DX], d[hyi])
   }
 #pragma unroll
   for (int j = 0; j < NHZ; ++j) {
@@ -198,17 +200,40 @@ __global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* 
       float* hx = sHx[dn];
       float* hy = sHy[dn];
       float* hz = sHz[dn];
+      // every operand is read first (all offsets are inside the slot), then the zero
+      // padding is a select: a read under the position's condition would wait for its
+      // own LDS round trip before the next one is issued
+      float xa[NHX], xb[NHX], ya[NHY], yb[NHY], za[NHZ], zb[NHZ];
 #pragma unroll
-      for (int j = 0; j < NHX; ++j)
-        if (hxe[j])
-          hx[threadIdx.x + j * NT] = (zin && hxv[j]) ? g2(Dc[hxi[j] + 2], Dc[hxi[j]]) : 0.0f;
+      for (int j = 0; j < NHX; ++j) {
+        xa[j] = Dc[hxi[j] + 2];
+        xb[j] = Dc[hxi[j]];
+      }
 #pragma unroll
-      for (int j = 0; j < NHY; ++j)
-        if (hye[j])
-          hy[threadIdx.x + j * NT] = (zin && hyv[j]) ? g2(Dc[hyi[j] + 2 * DX], Dc[hyi[j]]) : 0.0f;
+      for (int j = 0; j < NHY; ++j) {
+        ya[j] = Dc[hyi[j] + 2 * DX];
+        yb[j] = Dc[hyi[j]];
+      }
 #pragma unroll
-      for (int j = 0; j < NHZ; ++j)
-        hz[threadIdx.x + j * NT] = (zin && hzv[j]) ? g2(Dn[hzi[j]], Dp[hzi[j]]) : 0.0f;
+      for (int j = 0; j < NHZ; ++j) {
+        za[j] = Dn[hzi[j]];
+        zb[j] = Dp[hzi[j]];
+      }
+#pragma unroll
+      for (int j = 0; j < NHX; ++j) {
+        const float v = g2(xa[j], xb[j]);
+        if (hxe[j]) hx[threadIdx.x + j * NT] = (zin && hxv[j]) ? v : 0.0f;
+      }
+#pragma unroll
+      for (int j = 0; j < NHY; ++j) {
+        const float v = g2(ya[j], yb[j]);
+        if (hye[j]) hy[threadIdx.x + j * NT] = (zin && hyv[j]) ? v : 0.0f;
+      }
+#pragma unroll
+      for (int j = 0; j < NHZ; ++j) {
+        const float v = g2(za[j], zb[j]);
+        hz[threadIdx.x + j * NT] = (zin && hzv[j]) ? v : 0.0f;
+      }
     }
     __syncthreads();
     // 2. outputs of slice z = s - 2
@@ -223,6 +248,23 @@ __global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* 
       const float* hzm = sHz[dp];
       const float* hzp = sHz[dn];
       const uint32_t zb = (uint32_t)z * WH;
+      // the elements of every output row first (positions outside the plane read the
+      // zero-padded slots and are dropped below), so that all their reads are in flight
+      // together
+      float el[NOUT][6];
+#pragma unroll
+      for (int k = 0; k < NOUT; ++k) {
+        const int ty = ty0 + k * (NT / TX);
+        const int cx = (ty + 1) * XX + tx + 1;  // Hx centre
+        const int cy = (ty + 1) * YX + tx;      // Hy centre
+        const int cz = ty * TX + tx;            // Hz centre
+        el[k][0] = g2(hx0[cx + 1], hx0[cx - 1]);
+        el[k][1] = g2(hx0[cx + XX], hx0[cx - XX]);
+        el[k][2] = g2(hxp[cx], hxm[cx]);
+        el[k][3] = g2(hy0[cy + YX], hy0[cy - YX]);
+        el[k][4] = g2(hyp[cy], hym[cy]);
+        el[k][5] = g2(hzp[cz], hzm[cz]);
+      }
 #pragma unroll
       for (int k = 0; k < NOUT; ++k) {
         const int ty = ty0 + k * (NT / TX);
@@ -230,15 +272,8 @@ __global__ __launch_bounds__(NT) void k_hessian(const TD* __restrict__ dog, TS* 
         float h6[6];
         uint32_t cidx = 0;
         if (ov[k]) {
-          const int cx = (ty + 1) * XX + tx + 1;  // Hx centre
-          const int cy = (ty + 1) * YX + tx;      // Hy centre
-          const int cz = ty * TX + tx;            // Hz centre
-          const float xx = g2(hx0[cx + 1], hx0[cx - 1]);
-          const float xy = g2(hx0[cx + XX], hx0[cx - XX]);
-          const float xz = g2(hxp[cx], hxm[cx]);
-          const float yy = g2(hy0[cy + YX], hy0[cy - YX]);
-          const float yz = g2(hyp[cy], hym[cy]);
-          const float zz = g2(hzp[cz], hzm[cz]);
+          const float xx = el[k][0], xy = el[k][1], xz = el[k][2], yy = el[k][3],
+                      yz = el[k][4], zz = el[k][5];
           const uint32_t idx = zb + po[k];
           if (hess) {  // NULL: mask only (the region pass reads the candidate records)
             const i64 N = (i64)WH * D;
