@@ -63,11 +63,9 @@ KERNEL_BYTES = {
     # root bits cleared; per candidate: index, label read and written, parent read
     "ccl/cand_flatten": lambda sn, sd, sh, cf: 0.125 + 16 * cf,
     "regions/count_root_bits": lambda sn, sd, sh, cf: 0.125,
-    "ccl/local": lambda sn, sd, sh, cf: 1 + 4, "ccl/merge": lambda sn, sd, sh, cf: 1,
-    "ccl/init": lambda sn, sd, sh, cf: 1 + 4, "ccl/union": lambda sn, sd, sh, cf: 1,
-    "ccl/flatten": lambda sn, sd, sh, cf: 4 + 4,
+    # labeling of a given mask (the per-stage call): the mask read, candidate positions out
+    "ccl/mask_records": lambda sn, sd, sh, cf: 1 + 4 * cf,
     "regions/count_roots": lambda sn, sd, sh, cf: 4,
-    "regions/flatten_count_roots": lambda sn, sd, sh, cf: 4 + 4 * cf,
     "regions/scan": lambda sn, sd, sh, cf: 0,
     "regions/zero": lambda sn, sd, sh, cf: 0, "regions/assign_ids": lambda sn, sd, sh, cf: 4,
     "regions/accum": lambda sn, sd, sh, cf: 4 + 6 * sh + sn,

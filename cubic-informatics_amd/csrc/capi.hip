@@ -155,14 +155,12 @@ struct CandRecords {
 
 // With cand.rec the components come from launch_ccl_candidates (their root bits already
 // in ctx->rmask, each record's label in cand.rlab) and the region sums from the records
-// (labels and hess unused); else from the label volume and the materialized Hessian.
-// flatten: `labels` come from launch_ccl(flatten = false) and are finished here (written);
-// otherwise they are final and only read.
+// (labels and hess unused); else from the label volume (final labels, only read) and the
+// materialized Hessian.
 template <typename TH, typename TN>
 int regional_features_t(hdog_ctx* ctx, hipStream_t s, uint32_t* labels, const TH* hess,
                         const TN* norm, const uint16_t* img, int W, int H, int D, int is_ave,
-                        int64_t* num_regions, CandRecords cand = CandRecords(),
-                        bool flatten = false) {
+                        int64_t* num_regions, CandRecords cand = CandRecords()) {
   const i64 N = (i64)W * H * D;
   HIPCHK(ensure(ctx->rid, (size_t)N * 4));
   const i64 nb = (N + kRootBlock - 1) / kRootBlock;
@@ -175,8 +173,8 @@ int regional_features_t(hdog_ctx* ctx, hipStream_t s, uint32_t* labels, const TH
     HIPCHK(launch_region_count_masks(masks, (uint32_t*)ctx->bcnt.p, (uint32_t*)ctx->dR.p, N, s,
                                      &nbo));
   else
-    HIPCHK(launch_region_ids(labels, (uint32_t*)ctx->rid.p, (uint32_t*)ctx->bcnt.p, masks,
-                             (uint32_t*)ctx->dR.p, N, s, &nbo, flatten));
+    HIPCHK(launch_region_ids(labels, (uint32_t*)ctx->bcnt.p, masks, (uint32_t*)ctx->dR.p, N, s,
+                             &nbo));
   HIPCHK(hipMemcpyAsync(ctx->h_R, ctx->dR.p, 4, hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   const i64 R = *ctx->h_R;
@@ -617,7 +615,24 @@ int hdog_ccl3d(hdog_ctx* ctx, void* stream, const uint8_t* d_pd, uint32_t* d_lab
   if (rc) return rc;
   if (!d_pd || !d_labels) return fail(ctx, HDOG_ERR_ARG, "NULL device pointer");
   HIPCHK(hipSetDevice(ctx->device));
-  HIPCHK(launch_ccl(d_pd, d_labels, width, height, depth, (hipStream_t)stream));
+  // the mask's candidate positions in the record layout, then the whole path's labeling
+  // of candidates with the label volume written in full
+  const hipStream_t s = (hipStream_t)stream;
+  const i64 N = (i64)width * height * depth, ntiles = hessian_tiles(width, height, depth);
+  const i64 nb = (N + kRootBlock - 1) / kRootBlock;
+  HIPCHK(ensure(ctx->cand, (size_t)ntiles * kCandTile * 4));
+  HIPCHK(ensure(ctx->cand_cnt, (size_t)ntiles * 4));
+  HIPCHK(ensure(ctx->cand_lab, (size_t)ntiles * kCandTile * 4));
+  HIPCHK(ensure(ctx->fgbits, (size_t)fg_words(width, height, depth) * 8));
+  HIPCHK(ensure(ctx->rmask, (size_t)nb * (kRootBlock / 8)));
+  HIPCHK(ensure(ctx->rmulti, (size_t)nb * (kRootBlock / 8)));
+  HIPCHK(launch_mask_records(d_pd, (float*)ctx->cand.p, (uint32_t*)ctx->cand_cnt.p, width, height,
+                             depth, s));
+  HIPCHK(launch_ccl_candidates((const float*)ctx->cand.p, (const uint32_t*)ctx->cand_cnt.p, ntiles,
+                               (unsigned long long*)ctx->fgbits.p, d_labels,
+                               (uint32_t*)ctx->cand_lab.p, (unsigned long long*)ctx->rmask.p,
+                               (unsigned long long*)ctx->rmulti.p, nb * (kRootBlock / 64), true,
+                               width, height, depth, s, 1));
   return HDOG_OK;
 }
 
@@ -630,7 +645,6 @@ int hdog_regional_features(hdog_ctx* ctx, void* stream, const uint32_t* d_labels
   if (rc) return rc;
   if (!d_labels || !d_hessian || !d_norm) return fail(ctx, HDOG_ERR_ARG, "NULL device pointer");
   HIPCHK(hipSetDevice(ctx->device));
-  // flatten = false: the caller's labels are only read
   return regional_features_t<float, float>(ctx, (hipStream_t)stream, const_cast<uint32_t*>(d_labels),
                                            d_hessian, d_norm, d_img, width, height, depth,
                                            is_ave_mode, num_regions);

@@ -6,17 +6,17 @@
 // dz*W*H + dy*W + dx kept when the target is inside [0, N), so rows and slices wrap;
 // voxel 0 is never foreground (label = index * mask).
 //
-// MI355X design: no host round trips.  Union-find over the same adjacency in three
-// kernels: init (label = own index), union (each foreground voxel links with its
-// foreground neighbours at the distinct positive offsets; roots are linked larger-under-
-// smaller with atomicMin, so the root of every tree is the component minimum), and
-// flatten (every voxel points at its root).  Each edge is visited once.
+// MI355X design: no host round trips, and only foreground (candidate) voxels are touched.
+// The labeling reads the candidate positions per 64 x 8 x 16 tile that the Hessian pass
+// records (or that launch_mask_records extracts from a byte mask) and runs a union-find in
+// three kernels: per tile in LDS (links inside the tile), per tile over its rim (links
+// that leave the tile or wrap, in a volume-sized parent array touched only at candidates),
+// then every candidate's root.  Roots are linked larger-under-smaller, so the root of every
+// tree is the component minimum; each adjacent pair is linked once, by its lower voxel.
 #include "hdog_internal.hpp"
 
 namespace hdog {
 namespace {
-
-constexpr int NT = 256;
 
 struct Offsets {
   i64 o[13];
@@ -29,8 +29,8 @@ struct Offsets {
 // current, so each retry of unite strictly lowers `a` and the loop ends.  Path halving
 // (x's parent replaced by its grandparent) keeps the trees shallow; the plain store only
 // ever replaces a parent by one of its ancestors, so no union is lost.
-// Only the union kernel may halve paths: in the flatten kernel a halving store could
-// overwrite another voxel's final root with an intermediate ancestor.
+// Only the merge may halve paths: in the flatten kernel a halving store could overwrite
+// another voxel's final root with an intermediate ancestor.
 __device__ __forceinline__ uint32_t find_root(uint32_t* L, uint32_t x) {
   uint32_t p = L[x];
   while (p != x) {
@@ -55,86 +55,11 @@ __device__ uint32_t unite(uint32_t* L, uint32_t a, uint32_t b) {
   }
 }
 
-__global__ __launch_bounds__(NT) void k_ccl_init(const uint8_t* __restrict__ pd,
-                                                 uint32_t* __restrict__ L, i64 N) {
-  for (i64 i = (i64)blockIdx.x * NT + threadIdx.x; i < N; i += (i64)gridDim.x * NT)
-    L[i] = (pd[i] && i > 0) ? (uint32_t)i : 0u;
-}
-
-__global__ __launch_bounds__(NT) void k_ccl_union(const uint8_t* __restrict__ pd, uint32_t* L,
-                                                  i64 N, Offsets offs) {
-  for (i64 i = (i64)blockIdx.x * NT + threadIdx.x; i < N; i += (i64)gridDim.x * NT) {
-    if (i == 0 || !pd[i]) continue;
-    uint32_t root = (uint32_t)i;  // a representative of i's set, refreshed by each link
-#pragma unroll
-    for (int k = 0; k < 13; ++k) {
-      if (k < offs.n) {
-        const i64 j = i + offs.o[k];
-        if (j < N && pd[j]) {
-          // a neighbour whose parent already is our representative is in our set
-          if (L[j] != root) root = unite(L, root, (uint32_t)j);
-        }
-      }
-    }
-  }
-}
-
-__global__ __launch_bounds__(NT) void k_ccl_flatten(uint32_t* L, i64 N) {
-  for (i64 i = (i64)blockIdx.x * NT + threadIdx.x; i < N; i += (i64)gridDim.x * NT) {
-    const uint32_t l = L[i];
-    if (l && l != (uint32_t)i) L[i] = find_root_readonly(L, l);
-  }
-}
-
-// ---- tiled form (width >= 2 and height >= 2) ----------------------------------------
-// Kernel 1 labels each 32 x 8 x 8 tile on its own with a union-find in LDS (links of
-// voxel pairs whose neighbour lies inside the tile and inside the volume, i.e. without
-// wrap); every foreground voxel then points at the global index of its tile-local root,
-// which is the tile-local minimum because local order follows global order.  Kernel 2
-// links, with the global union-find, only the pairs that leave the tile or wrap.  For
-// width, height >= 2 the 13 forward neighbours (dz = 1; dz = 0, dy = 1; dz = dy = 0,
-// dx = 1) are exactly the positive linear offsets of the reference's adjacency.
-constexpr int CX = 32, CY = 8, CZ = 8, CV = CX * CY * CZ, NTC = 256, CPT = CV / NTC;
-static_assert(CV % NTC == 0, "tile shape");
-
-__device__ __forceinline__ uint32_t lds_find(const uint32_t* lab, uint32_t x) {
-  uint32_t p = lab[x];
-  while (p != x) {
-    x = p;
-    p = lab[x];
-  }
-  return x;
-}
-
-// union phase only: path halving keeps the LDS trees shallow (same argument as
-// find_root: a parent is only ever replaced by one of its ancestors)
-__device__ __forceinline__ uint32_t lds_find_halve(uint32_t* lab, uint32_t x) {
-  uint32_t p = lab[x];
-  while (p != x) {
-    const uint32_t gp = lab[p];
-    if (gp != p) lab[x] = gp;
-    x = p;
-    p = gp;
-  }
-  return x;
-}
-
-__device__ __forceinline__ uint32_t lds_unite(uint32_t* lab, uint32_t a, uint32_t b) {
-  while (true) {
-    a = lds_find_halve(lab, a);
-    b = lds_find_halve(lab, b);
-    if (a == b) return a;
-    if (a < b) { const uint32_t t = a; a = b; b = t; }
-    const uint32_t old = atomicMin(lab + a, b);
-    if (old == a) return b;
-    a = old;
-  }
-}
-
-// The same union-find on 16-bit parents (tile positions below 2^16): half the LDS, so
-// twice the work-groups per CU.  LDS has no 16-bit atomic minimum; the link is a 32-bit
-// compare-and-swap on the word holding the entry, which fails and retries when either
-// half changed.  Path-halving stores stay plain 16-bit stores, as above.
+// The tile-local union-find, in LDS on 16-bit parents (tile positions below 2^16): half
+// the LDS of 32-bit parents, so twice the work-groups per CU.  LDS has no 16-bit atomic
+// minimum; the link is a 32-bit compare-and-swap on the word holding the entry, which
+// fails and retries when either half changed.  Path-halving stores are plain 16-bit
+// stores (a parent is only ever replaced by one of its ancestors, as above).
 __device__ __forceinline__ uint32_t lds16_find(const uint16_t* lab, uint32_t x) {
   uint32_t p = lab[x];
   while (p != x) {
@@ -178,12 +103,6 @@ __device__ __forceinline__ uint32_t lds16_unite(uint16_t* lab, uint32_t a, uint3
   }
 }
 
-__device__ __forceinline__ void fwd_offset(int k, int& dx, int& dy, int& dz) {
-  if (k < 9) { dz = 1; dy = k / 3 - 1; dx = k % 3 - 1; }
-  else if (k < 12) { dz = 0; dy = 1; dx = k - 10; }
-  else { dz = 0; dy = 0; dx = 1; }
-}
-
 // Redundant links.  Every 26-adjacent candidate pair is linked by its lower voxel (in
 // the tile's local pass when both lie in the tile, else in the merge), so a voxel v can
 // skip a foreground forward neighbour b that is adjacent to another forward neighbour w
@@ -217,214 +136,6 @@ static_assert(fwd_order(2) == 0 && fwd_order(5) == 3 && fwd_order(6) == 5 && fwd
 // t / 32) of the tile and k = 0..7 walks its z; every index is 32-bit (N < 2^32).  The
 // foreground voxels are compacted into an LDS list (wave ballots), so the union phase runs
 // one dense pass over the few foreground voxels instead of eight divergent passes over all.
-static_assert(NTC == CX * CY && CPT == CZ, "k_ccl_local maps one thread to one z column");
-
-__global__ __launch_bounds__(NTC) void k_ccl_local(const uint8_t* __restrict__ pd,
-                                                  uint32_t* __restrict__ L, int W, int H, int D,
-                                                  int nbx, int nby) {
-  constexpr uint32_t NONE = 0xFFFFFFFFu;
-  __shared__ uint32_t lab[CV];
-  __shared__ uint16_t list[CV];
-  __shared__ uint32_t nfg;
-  if (threadIdx.x == 0) nfg = 0;
-  unsigned b = xcd_tile(blockIdx.x, gridDim.x);
-  const int bx = b % nbx; b /= nbx;
-  const int by = b % nby; b /= nby;
-  const int x0 = bx * CX, y0 = by * CY, z0 = (int)b * CZ;
-  const uint32_t WH = (uint32_t)W * (uint32_t)H;
-  const int lx = threadIdx.x & (CX - 1), ly = threadIdx.x / CX;
-  const int gx = x0 + lx, gy = y0 + ly;
-  const bool col = gx < W && gy < H;
-  const uint32_t g0 = (uint32_t)z0 * WH + (uint32_t)gy * (uint32_t)W + (uint32_t)gx;
-  const int lane = threadIdx.x & 63;
-  uint8_t fg[CPT];
-#pragma unroll
-  for (int k = 0; k < CPT; ++k) {
-    const uint32_t g = g0 + (uint32_t)k * WH;
-    fg[k] = (col && z0 + k < D && g > 0) ? pd[g] : (uint8_t)0;
-  }
-  __syncthreads();  // nfg initialised
-#pragma unroll
-  for (int k = 0; k < CPT; ++k) {
-    const int v = threadIdx.x + k * NTC;
-    lab[v] = fg[k] ? (uint32_t)v : NONE;
-    const unsigned long long bal = __ballot(fg[k] != 0);
-    if (bal) {
-      uint32_t base = 0;
-      if (lane == 0) base = atomicAdd(&nfg, (uint32_t)__popcll(bal));
-      base = __shfl(base, 0, 64);
-      if (fg[k]) list[base + __popcll(bal & ((1ull << lane) - 1ull))] = (uint16_t)v;
-    }
-  }
-  __syncthreads();
-  const int n = (int)nfg;
-#pragma unroll 1
-  for (int i = threadIdx.x; i < n; i += NTC) {
-    const int v = list[i];
-    const int vx = v & (CX - 1), vy = (v / CX) & (CY - 1), vz = v / (CX * CY);
-    uint32_t root = (uint32_t)v;  // a representative of v's set, refreshed by each link
-#pragma unroll
-    for (int q = 0; q < 13; ++q) {
-      int dx, dy, dz;
-      fwd_offset(q, dx, dy, dz);
-      const int nx = vx + dx, ny = vy + dy, nz = vz + dz;
-      if (nx < 0 || nx >= CX || ny < 0 || ny >= CY || nz >= CZ) continue;
-      const int u = v + dx + CX * (dy + CY * dz);
-      const uint32_t lu = lab[u];
-      // outside-volume and background positions hold NONE; a parent equal to our
-      // representative means u is already in our set
-      if (lu != NONE && lu != root) root = lds_unite(lab, root, (uint32_t)u);
-    }
-  }
-  __syncthreads();
-  if (!col) return;
-#pragma unroll
-  for (int k = 0; k < CPT; ++k) {
-    if (z0 + k >= D) break;
-    uint32_t out = 0u;
-    if (fg[k]) {
-      const uint32_t r = lds_find(lab, (uint32_t)(threadIdx.x + k * NTC));
-      out = (uint32_t)(z0 + (int)(r / (CX * CY))) * WH +
-            (uint32_t)(y0 + (int)((r / CX) & (CY - 1))) * (uint32_t)W + (uint32_t)x0 +
-            (r & (CX - 1));
-    }
-    L[g0 + (uint32_t)k * WH] = out;
-  }
-}
-
-// Kernel 2 visits only the voxels that can have a neighbour outside their tile or the
-// volume: with the tile clipped to the volume as ex x ey x ez, its top z-face (lz = ez-1)
-// and the rim of every lower plane (lx in {0, ex-1} or ly in {0, ey-1}); every other
-// voxel's 13 forward neighbours lie inside the clipped tile.  Those voxels are numbered
-// densely so that each wave carries only rim work.
-__device__ __forceinline__ void rim_voxel(int t, int ex, int ey, int ez, int& lx, int& ly,
-                                          int& lz) {
-  const int top = ex * ey;
-  if (t < top) {
-    lz = ez - 1;
-    ly = t / ex;
-    lx = t - ly * ex;
-    return;
-  }
-  t -= top;
-  const int rows = (ey >= 2 ? 2 : 1) * ex, nc = ex >= 2 ? 2 : 1;
-  const int ring = rows + (ey >= 2 ? ey - 2 : 0) * nc;
-  lz = t / ring;
-  const int r = t - lz * ring;
-  if (r < rows) {
-    ly = r < ex ? 0 : ey - 1;
-    lx = r < ex ? r : r - ex;
-  } else {
-    const int r2 = r - rows;
-    ly = 1 + r2 / nc;
-    lx = (r2 % nc) ? ex - 1 : 0;
-  }
-}
-
-__device__ __forceinline__ int rim_count(int ex, int ey, int ez) {
-  const int rows = (ey >= 2 ? 2 : 1) * ex, nc = ex >= 2 ? 2 : 1;
-  return ex * ey + (ez - 1) * (rows + (ey >= 2 ? ey - 2 : 0) * nc);
-}
-
-constexpr int NTM = 256;
-constexpr int KR = 4;  // rim voxels per thread: a 32 x 8 x 8 tile has at most 788 <= KR * NTM
-static_assert(32 * 8 + 7 * (2 * 32 + 6 * 2) <= KR * 256, "rim of a full tile fits one pass");
-
-// Kernel 2.  Background voxels carry label 0 after kernel 1, so labels double as the
-// mask.  Each distinct neighbour label is linked with the voxel's own label by the global
-// union-find (every voxel of a tile-local component carries the same label, so few links
-// remain).  Labels read here may already have been linked by other work-groups; any value
-// read is an ancestor, so every link still joins the right sets.
-// Round 1: the labels of up to KR rim voxels per thread (in flight together); the
-// foreground ones (label != 0) are compacted into an LDS list.  Round 2: one dense pass
-// over that list loads the labels of each voxel's neighbours outside the clipped tile and
-// links every distinct one with the voxel's own label.
-template <bool FULL>
-__device__ __forceinline__ void merge_tile(uint32_t* L, int W, uint32_t WH, uint32_t N, int x0,
-                                           int y0, int z0, int ex, int ey, int ez,
-                                           uint16_t* list_t, uint32_t* list_own,
-                                           uint32_t* n_list) {
-  if (FULL) { ex = CX; ey = CY; ez = CZ; }  // compile-time shape: the rim math folds
-  const int F = rim_count(ex, ey, ez);
-  const int lane = threadIdx.x & 63;
-  uint32_t own[KR];
-#pragma unroll
-  for (int k = 0; k < KR; ++k) {
-    const int t = threadIdx.x + k * NTM;
-    own[k] = 0u;
-    if (t < F) {
-      int lx, ly, lz;
-      rim_voxel(t, ex, ey, ez, lx, ly, lz);
-      own[k] = L[(uint32_t)(z0 + lz) * WH + (uint32_t)(y0 + ly) * (uint32_t)W +
-                 (uint32_t)(x0 + lx)];  // 0: background (voxel 0 is always background)
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < KR; ++k) {
-    const unsigned long long bal = __ballot(own[k] != 0u);
-    if (bal) {
-      uint32_t base = 0;
-      if (lane == 0) base = atomicAdd(n_list, (uint32_t)__popcll(bal));
-      base = __shfl(base, 0, 64);
-      if (own[k]) {
-        const uint32_t slot = base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
-        list_t[slot] = (uint16_t)(threadIdx.x + k * NTM);
-        list_own[slot] = own[k];
-      }
-    }
-  }
-  __syncthreads();
-  const int n = (int)*n_list;
-#pragma unroll 1
-  for (int i = threadIdx.x; i < n; i += NTM) {
-    int lx, ly, lz;
-    rim_voxel(list_t[i], ex, ey, ez, lx, ly, lz);
-    const uint32_t me = list_own[i];
-    const uint32_t g = (uint32_t)(z0 + lz) * WH + (uint32_t)(y0 + ly) * (uint32_t)W +
-                       (uint32_t)(x0 + lx);
-    uint32_t lj[13];
-#pragma unroll
-    for (int q = 0; q < 13; ++q) {
-      int dx, dy, dz;
-      fwd_offset(q, dx, dy, dz);
-      // forward linear offset (positive for W, H >= 2); wraps like the reference
-      const i64 off64 = (i64)dz * WH + (i64)dy * W + dx;
-      const uint32_t off = (uint32_t)off64;
-      const int nx = lx + dx, ny = ly + dy, nz = lz + dz;
-      const bool local = nx >= 0 && nx < ex && ny >= 0 && ny < ey && nz < ez;
-      lj[q] = (!local && off64 < (i64)N && g < N - off) ? L[g + off] : 0u;
-    }
-    uint32_t seen1 = 0u, seen2 = 0u;
-#pragma unroll
-    for (int q = 0; q < 13; ++q) {
-      const uint32_t l = lj[q];
-      if (!l || l == me || l == seen1 || l == seen2) continue;
-      unite(L, me, l);
-      seen2 = seen1;
-      seen1 = l;
-    }
-  }
-}
-
-__global__ __launch_bounds__(NTM) void k_ccl_merge(uint32_t* L, int W, int H, int D, int nbx,
-                                                   int nby) {
-  __shared__ uint16_t list_t[KR * NTM];
-  __shared__ uint32_t list_own[KR * NTM];
-  __shared__ uint32_t n_list;
-  if (threadIdx.x == 0) n_list = 0;
-  __syncthreads();
-  unsigned b = xcd_tile(blockIdx.x, gridDim.x);
-  const int bx = b % nbx; b /= nbx;
-  const int by = b % nby; b /= nby;
-  const int x0 = bx * CX, y0 = by * CY, z0 = (int)b * CZ;
-  const int ex = min(CX, W - x0), ey = min(CY, H - y0), ez = min(CZ, D - z0);
-  const uint32_t WH = (uint32_t)W * (uint32_t)H, N = WH * (uint32_t)D;
-  if (ex == CX && ey == CY && ez == CZ)
-    merge_tile<true>(L, W, WH, N, x0, y0, z0, ex, ey, ez, list_t, list_own, &n_list);
-  else
-    merge_tile<false>(L, W, WH, N, x0, y0, z0, ex, ey, ez, list_t, list_own, &n_list);
-}
-
 // ---- candidate-record form (the whole-path call) ------------------------------------
 // The Hessian pass leaves, per 64 x 8 x 16 tile, the positions of its candidate voxels
 // (the foreground of the CCL) in the tile.  Labeling then touches only candidates: P (a
@@ -512,12 +223,13 @@ __global__ __launch_bounds__(NTR) void k_cand_local(const float* __restrict__ ca
                                                    unsigned long long* __restrict__ fgbits,
                                                    uint32_t* __restrict__ P,
                                                    uint32_t* __restrict__ rlab, int W, int H,
-                                                   int D, int nbx, int nby, int links) {
+                                                   int D, int nbx, int nby, int links,
+                                                   int rec_words) {
   const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
   const uint32_t n = cand_cnt[tile];
   __shared__ unsigned long long bits[RROWS];  // the tile's mask rows (word = ly + RTY lz)
   __shared__ uint16_t lab[RTV];              // parents, valid at candidate positions
-  const float* rec = cand + (size_t)tile * kCandTile * 7;
+  const float* rec = cand + (size_t)tile * kCandTile * rec_words;
   TileRecs R;
   R.load(rec, n);
   if (threadIdx.x < RROWS) bits[threadIdx.x] = 0ull;
@@ -579,13 +291,13 @@ __global__ __launch_bounds__(NTR) void k_cand_merge(const float* __restrict__ ca
                                                    const unsigned long long* __restrict__ fgbits,
                                                    uint32_t* P, const uint32_t* __restrict__ rlab,
                                                    int W, int H, int D, int nbx, int nby,
-                                                   int generic, Offsets offs) {
+                                                   int generic, Offsets offs, int rec_words) {
   const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
   const uint32_t n = cand_cnt[tile];
   if (n == 0) return;
   const CandTile T = cand_tile(tile, W, H, D, nbx, nby);
   const uint32_t Wu = (uint32_t)W, WH = Wu * (uint32_t)H;
-  const float* rec = cand + (size_t)tile * kCandTile * 7;
+  const float* rec = cand + (size_t)tile * kCandTile * rec_words;
   if (generic) {  // width or height 1: every candidate, the distinct positive offsets
     const i64 N = (i64)WH * D;
 #pragma unroll 1
@@ -707,13 +419,13 @@ __global__ __launch_bounds__(NTR) void k_cand_flatten(const float* __restrict__ 
                                                      unsigned long long* __restrict__ roots,
                                                      unsigned long long* __restrict__ multi,
                                                      int dense, int W, int H, int D, int nbx,
-                                                     int nby) {
+                                                     int nby, int rec_words) {
   const unsigned tile = blockIdx.x;
   const uint32_t n = cand_cnt[tile];
   if (n == 0) return;
   const CandTile T = cand_tile(tile, W, H, D, nbx, nby);
   const uint32_t Wu = (uint32_t)W, WH = Wu * (uint32_t)H;
-  const float* rec = cand + (size_t)tile * kCandTile * 7;
+  const float* rec = cand + (size_t)tile * kCandTile * rec_words;
   uint32_t* rl = rlab + (size_t)tile * kCandTile;
   TileRecs R;
   R.load(rec, n);
@@ -754,6 +466,43 @@ __global__ __launch_bounds__(NTR) void k_cand_flatten(const float* __restrict__ 
     one(i, __float_as_uint(rec[i]), rl[i]);
 }
 
+// Candidate positions of a byte mask, in the record layout of the Hessian pass (per
+// Hessian tile a count and the tile positions, one reservation per wave and tile row, in
+// lane order): the labeling of a given mask then runs the same kernels as the whole path.
+// Voxel 0 is background whatever its mask value (its label is 0).
+__global__ __launch_bounds__(256) void k_mask_records(const uint8_t* __restrict__ pd,
+                                                      float* __restrict__ pos,
+                                                      uint32_t* __restrict__ cand_cnt, int W,
+                                                      int H, int D, int nbx, int nby) {
+  const unsigned tile = blockIdx.x;
+  const CandTile T = cand_tile(tile, W, H, D, nbx, nby);
+  __shared__ uint32_t s_cnt;
+  if (threadIdx.x == 0) s_cnt = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t WH = (uint32_t)W * (uint32_t)H;
+  float* out = pos + (size_t)tile * kCandTile;
+  for (int row = wave; row < RROWS; row += 4) {  // uniform over the wave
+    const int ly = row % RTY, lz = row / RTY;
+    bool on = false;
+    if (lane < T.ex && ly < T.ey && lz < T.ez) {
+      const uint32_t idx = (uint32_t)(T.z0 + lz) * WH + (uint32_t)(T.y0 + ly) * (uint32_t)W +
+                           (uint32_t)(T.x0 + lane);
+      on = idx != 0 && pd[idx] != 0;
+    }
+    const unsigned long long bal = __ballot(on);
+    if (!bal) continue;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(&s_cnt, (uint32_t)__popcll(bal));
+    base = __shfl(base, 0, 64);
+    if (on)
+      out[base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] =
+          __uint_as_float((uint32_t)(lane + RTX * (ly + RTY * lz)));
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) cand_cnt[tile] = s_cnt;
+}
+
 // The distinct positive linear offsets dz W H + dy W + dx of the 27-neighbourhood
 // (src/3D/CCL3D.cu:43-48 keeps every offset whose target lies in [0, N)).
 Offsets positive_offsets(int W, int H) {
@@ -778,41 +527,20 @@ Offsets positive_offsets(int W, int H) {
 
 }  // namespace
 
-hipError_t launch_ccl(const uint8_t* pd, uint32_t* labels, int W, int H, int D, hipStream_t s,
-                      bool flatten) {
-  const i64 WH = (i64)W * H, N = WH * D;
-  const Offsets offs = positive_offsets(W, H);
-  const i64 nb = std::min<i64>((N + NT - 1) / NT, 1 << 20);
-  hipError_t e;
-  if (W >= 2 && H >= 2) {
-    const int nbx = (W + CX - 1) / CX, nby = (H + CY - 1) / CY, nbz = (D + CZ - 1) / CZ;
-    const i64 nt = (i64)nbx * nby * nbz;
-    hipLaunchKernelGGL(k_ccl_local, dim3((unsigned)nt), dim3(NTC), 0, s, pd, labels, W, H, D, nbx,
-                       nby);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    mark("ccl/local", s);
-    hipLaunchKernelGGL(k_ccl_merge, dim3((unsigned)nt), dim3(NTM), 0, s, labels, W, H, D, nbx, nby);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    mark("ccl/merge", s);
-  } else {
-    hipLaunchKernelGGL(k_ccl_init, dim3((unsigned)nb), dim3(NT), 0, s, pd, labels, N);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    mark("ccl/init", s);
-    hipLaunchKernelGGL(k_ccl_union, dim3((unsigned)nb), dim3(NT), 0, s, pd, labels, N, offs);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    mark("ccl/union", s);
-  }
-  if (!flatten) return hipSuccess;
-  hipLaunchKernelGGL(k_ccl_flatten, dim3((unsigned)nb), dim3(NT), 0, s, labels, N);
-  mark("ccl/flatten", s);
+hipError_t launch_mask_records(const uint8_t* pd, float* pos, uint32_t* cand_cnt, int W, int H,
+                               int D, hipStream_t s) {
+  const int nbx = (W + RTX - 1) / RTX, nby = (H + RTY - 1) / RTY;
+  hipLaunchKernelGGL(k_mask_records, dim3((unsigned)hessian_tiles(W, H, D)), dim3(256), 0, s, pd,
+                     pos, cand_cnt, W, H, D, nbx, nby);
+  mark("ccl/mask_records", s);
   return hipGetLastError();
 }
 
 hipError_t launch_ccl_candidates(const float* cand, const uint32_t* cand_cnt, i64 ntiles,
                                  unsigned long long* fgbits, uint32_t* P, uint32_t* rlab,
                                  unsigned long long* roots, unsigned long long* multi,
-                                 i64 root_words, bool dense, int W, int H, int D,
-                                 hipStream_t s) {
+                                 i64 root_words, bool dense, int W, int H, int D, hipStream_t s,
+                                 int rec_words) {
   const int nbx = (W + RTX - 1) / RTX, nby = (H + RTY - 1) / RTY;
   hipError_t e;
   if (dense) {  // background of the materialized label volume
@@ -822,18 +550,18 @@ hipError_t launch_ccl_candidates(const float* cand, const uint32_t* cand_cnt, i6
   // width, height >= 2; otherwise every link goes through the generic merge
   const int generic = (W < 2 || H < 2) ? 1 : 0;
   hipLaunchKernelGGL(k_cand_local, dim3((unsigned)ntiles), dim3(NTR), 0, s, cand, cand_cnt, fgbits,
-                     P, rlab, W, H, D, nbx, nby, 1 - generic);
+                     P, rlab, W, H, D, nbx, nby, 1 - generic, rec_words);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark("ccl/cand_local", s);
   hipLaunchKernelGGL(k_cand_merge, dim3((unsigned)ntiles), dim3(NTR), 0, s, cand, cand_cnt,
                      (const unsigned long long*)fgbits, P, rlab, W, H, D, nbx, nby, generic,
-                     positive_offsets(W, H));
+                     positive_offsets(W, H), rec_words);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   mark("ccl/cand_merge", s);
   if ((e = launch_fill_u64((u64*)roots, root_words, 0ull, s)) != hipSuccess) return e;
   if ((e = launch_fill_u64((u64*)multi, root_words, 0ull, s)) != hipSuccess) return e;
   hipLaunchKernelGGL(k_cand_flatten, dim3((unsigned)ntiles), dim3(NTR), 0, s, cand, cand_cnt, P,
-                     rlab, roots, multi, dense ? 1 : 0, W, H, D, nbx, nby);
+                     rlab, roots, multi, dense ? 1 : 0, W, H, D, nbx, nby, rec_words);
   mark("ccl/cand_flatten", s);
   return hipGetLastError();
 }

@@ -196,17 +196,11 @@ template <typename TD, typename TH>
 // kCandTile * 7 words.  Voxel 0 never has a record (its label is 0 whatever its mask).
 hipError_t launch_hessian(const TD* dog, TH* hess, uint8_t* pd, int W, int H, int D,
                           hipStream_t s, float* cand = nullptr, uint32_t* cand_cnt = nullptr);
-// CCL: labels out; union-find in place
-// flatten = false leaves non-root voxels pointing at an ancestor of their root; the region
-// numbering (launch_region_ids with flatten = true) then finishes the labels in its pass.
-hipError_t launch_ccl(const uint8_t* pd, uint32_t* labels, int W, int H, int D, hipStream_t s,
-                      bool flatten = true);
-// region numbering: fills rid[root] for every root, block counts, and *d_R = R
-// Root counts per block of 4096 voxels and their positions (masks: 64 words per block),
-// then *d_R = 1 + number of roots.
-hipError_t launch_region_ids(uint32_t* labels, uint32_t* rid, uint32_t* block_cnt,
+// Region numbering of a label volume: root counts per block of 4096 voxels and their
+// positions (masks: 64 words per block), then *d_R = 1 + number of roots.
+hipError_t launch_region_ids(const uint32_t* labels, uint32_t* block_cnt,
                              unsigned long long* masks, uint32_t* d_R, i64 N, hipStream_t s,
-                             i64* nblocks_out, bool flatten = false);
+                             i64* nblocks_out);
 // R is the host's region count, or, when d_R is non-null, read on the device as
 // min(*d_R, cap) (graph-captured batches, where the host never sees R); ids >= cap are skipped.
 hipError_t launch_region_assign(const unsigned long long* masks, const uint32_t* block_off,
@@ -254,11 +248,17 @@ hipError_t launch_fill_u32(uint32_t* p, i64 n, uint32_t value, hipStream_t s);
 // in linear order, and multi (same size, cleared here) the root bit of every component
 // with voxels outside the kAccGY x kAccGZ tile group of its root.  dense: P becomes the
 // full label volume (0 for the background).
+// rec_words: 32-bit words per record slot block (7: the Hessian pass's records; 1: the
+// positions of launch_mask_records).
 hipError_t launch_ccl_candidates(const float* cand, const uint32_t* cand_cnt, i64 ntiles,
                                  unsigned long long* fgbits, uint32_t* P, uint32_t* rlab,
                                  unsigned long long* roots, unsigned long long* multi,
-                                 i64 root_words, bool dense, int W, int H, int D,
-                                 hipStream_t s);
+                                 i64 root_words, bool dense, int W, int H, int D, hipStream_t s,
+                                 int rec_words = 7);
+// The candidate positions of a byte mask (pd != 0, voxel 0 excluded) in the Hessian pass's
+// record layout with one word per slot (pos: hessian_tiles() * kCandTile words).
+hipError_t launch_mask_records(const uint8_t* pd, float* pos, uint32_t* cand_cnt, int W, int H,
+                               int D, hipStream_t s);
 // Roots per numbering block from a root bit mask in linear order, then the exclusive scan
 // of the counts; *d_R = 1 + number of roots.
 hipError_t launch_region_count_masks(const unsigned long long* masks, uint32_t* block_cnt,

@@ -42,11 +42,9 @@ __device__ __forceinline__ int block_sum_int(int v, int* sh) {
 
 // Per block of BLK voxels: the number of roots (L[i] == i, i > 0) and their positions as
 // a bitmask (masks[64 b + 4 k + wave] bit lane <-> voxel b*BLK + k*NT + 64 wave + lane,
-// so bit order is linear order).  FLATTEN: also points every other foreground voxel at its
-// root (labels leave the union-find with parents that are only ancestors).  A root is its
-// own parent before and after, so counting and flattening can share the pass.
-template <bool FLATTEN>
-__global__ __launch_bounds__(NT) void k_count_roots(uint32_t* L, i64 N, uint32_t* __restrict__ cnt,
+// so bit order is linear order).
+__global__ __launch_bounds__(NT) void k_count_roots(const uint32_t* __restrict__ L, i64 N,
+                                                    uint32_t* __restrict__ cnt,
                                                     unsigned long long* __restrict__ masks) {
   __shared__ int sh[4];
   const i64 base = (i64)blockIdx.x * BLK;
@@ -65,10 +63,6 @@ __global__ __launch_bounds__(NT) void k_count_roots(uint32_t* L, i64 N, uint32_t
     const unsigned long long m = __ballot(root);
     if (lane == 0) masks[(size_t)blockIdx.x * (BLK / 64) + k * (NT / 64) + wave] = m;
     c += root ? 1 : 0;
-    if (FLATTEN && !root && i > 0 && i < N && v[k] != 0u) {
-      const uint32_t r = find_root_readonly(L, v[k]);
-      if (r != v[k]) L[i] = r;
-    }
   }
   const int t = block_sum_int(c, sh);
   if (threadIdx.x == 0) cnt[blockIdx.x] = (uint32_t)t;
@@ -521,22 +515,16 @@ __global__ __launch_bounds__(NT) void k_mask_counts(const unsigned long long* __
 
 }  // namespace
 
-hipError_t launch_region_ids(uint32_t* labels, uint32_t* rid, uint32_t* block_cnt,
+hipError_t launch_region_ids(const uint32_t* labels, uint32_t* block_cnt,
                              unsigned long long* masks, uint32_t* d_R, i64 N, hipStream_t s,
-                             i64* nblocks_out, bool flatten) {
-  (void)rid;
+                             i64* nblocks_out) {
   const i64 nb = (N + BLK - 1) / BLK;
   *nblocks_out = nb;
-  if (flatten) {
-    hipLaunchKernelGGL(k_count_roots<true>, dim3((unsigned)nb), dim3(NT), 0, s, labels, N,
-                       block_cnt, masks);
-  } else {
-    hipLaunchKernelGGL(k_count_roots<false>, dim3((unsigned)nb), dim3(NT), 0, s, labels, N,
-                       block_cnt, masks);
-  }
+  hipLaunchKernelGGL(k_count_roots, dim3((unsigned)nb), dim3(NT), 0, s, labels, N, block_cnt,
+                     masks);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  mark(flatten ? "regions/flatten_count_roots" : "regions/count_roots", s);
+  mark("regions/count_roots", s);
   hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, block_cnt, nb, d_R);
   mark("regions/scan", s);
   return hipGetLastError();

@@ -103,8 +103,11 @@ def test_hessian_bitexact(hd, shape):
                                      ((33, 1, 7), 0.5), ((9, 64, 65), 0.45), ((2, 3, 1), 0.9),
                                      ((9, 17, 40), 0.35), ((17, 9, 33), 0.4), ((3, 2, 2), 0.7),
                                      ((20, 3, 70), 0.4), ((8, 16, 64), 0.3), ((11, 2, 5), 0.6),
-                                     ((70, 24, 96), 0.3), ((41, 19, 132), 0.2)])
+                                     ((70, 24, 96), 0.3), ((41, 19, 132), 0.2),
+                                     ((32, 16, 128), 0.9), ((20, 9, 200), 0.6)])
 def test_ccl_exact(hd, shape, p):
+    # hdog_ccl3d labels the mask's candidates with the whole path's kernels; the dense
+    # cases put more candidates in a tile than its threads hold in registers
     for seed in range(3):
         mask = (np.random.default_rng(seed).random(shape) < p).astype(np.uint8)
         got = to_np(hd.CCL(mask))
