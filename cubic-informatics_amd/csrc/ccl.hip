@@ -247,24 +247,36 @@ __global__ __launch_bounds__(NTR) void k_cand_local(const float* __restrict__ ca
     R.each_rolled(rec, n, [&](uint32_t, uint32_t pv) __attribute__((always_inline)) {
       const int v = (int)pv;
       const int vx = v & (RTX - 1), vy = (v / RTX) & (RTY - 1), vz = v / (RTX * RTY);
-      uint32_t fg = 0u;  // bit q: forward neighbour q is a candidate inside the clipped tile
+      // bit q: forward neighbour q is a candidate inside the clipped tile (branch-free)
+      const bool xl = vx > 0, xr = vx + 1 < T.ex, yl = vy > 0, yr = vy + 1 < T.ey,
+                 zr = vz + 1 < T.ez;
+      uint32_t fg = 0u;
 #pragma unroll
       for (int q = 0; q < 13; ++q) {
-        const int nx = vx + fdx(q), ny = vy + fdy(q), nz = vz + fdz(q);
-        if (nx < 0 || nx >= T.ex || ny < 0 || ny >= T.ey || nz >= T.ez) continue;
-        if ((bits[ny + RTY * nz] >> nx) & 1ull) fg |= 1u << q;
+        const bool in = (fdx(q) < 0 ? xl : fdx(q) > 0 ? xr : true) &&
+                        (fdy(q) < 0 ? yl : fdy(q) > 0 ? yr : true) && (fdz(q) > 0 ? zr : true);
+        const int row = in ? (vy + fdy(q)) + RTY * (vz + fdz(q)) : 0;
+        const uint32_t b = (uint32_t)(bits[row] >> ((vx + fdx(q)) & (RTX - 1))) & 1u;
+        fg |= (in ? b : 0u) << q;
       }
-      uint32_t root = (uint32_t)v;  // a representative of v's set, refreshed by each link
-      uint32_t known = 0u;          // neighbours already in v's set
+      // the links: neighbours not adjacent to one already known to be in v's set
+      uint32_t known = 0u, link = 0u;
 #pragma unroll
       for (int j = 0; j < 13; ++j) {
         const int q = fwd_order(j);
-        if (!((fg >> q) & 1u)) continue;
-        const bool skip = (fwd_adj(q) & known) != 0u;
-        known |= 1u << q;
-        if (skip) continue;
-        const int u = (vx + fdx(q)) + RTX * ((vy + fdy(q)) + RTY * (vz + fdz(q)));
-        if (lab[u] != root) root = lds16_unite(lab, root, (uint32_t)u);
+        const uint32_t b = (fg >> q) & 1u;
+        link |= ((fwd_adj(q) & known) == 0u ? b : 0u) << q;
+        known |= b << q;
+      }
+      uint32_t root = (uint32_t)v;  // a representative of v's set, refreshed by each link
+      while (link) {  // one copy of the union code for every link
+        const int q = __builtin_ctz(link);
+        link &= link - 1u;
+        // tile-local offset of forward neighbour q: dz 512 + dy 64 + dx
+        const int off = q < 9 ? RTX * RTY + RTX * (q / 3 - 1) + (q % 3 - 1)
+                              : (q < 12 ? RTX + (q - 10) : 1);
+        const uint32_t u = (uint32_t)(v + off);
+        if (lab[u] != root) root = lds16_unite(lab, root, u);
       }
     });
   __syncthreads();
