@@ -332,6 +332,9 @@ __global__ __launch_bounds__(NTR) void k_cand_merge(const float* __restrict__ ca
   // face and the ring of every lower plane) are compacted into an LDS list first, so the
   // neighbour work below runs on full waves
   constexpr int RIM_MAX = RTX * RTY + (RTZ - 1) * (2 * RTX + 2 * (RTY - 2));
+  // the mask words of the whole tiling (gridDim.x tiles)
+  const __amdgpu_buffer_rsrc_t fg_rsrc =
+      buf_rsrc(fgbits, (uint32_t)gridDim.x * (uint32_t)(RROWS * sizeof(unsigned long long)));
   __shared__ uint16_t rim_v[RIM_MAX];
   __shared__ uint32_t rim_me[RIM_MAX];
   __shared__ uint32_t n_rim;
@@ -378,45 +381,59 @@ __global__ __launch_bounds__(NTR) void k_cand_merge(const float* __restrict__ ca
     const int lx = v & (RTX - 1), ly = (v / RTX) & (RTY - 1), lz = v / (RTX * RTY);
     const int x = T.x0 + lx, y = T.y0 + ly, z = T.z0 + lz;
     const uint32_t me = rim_me[j];
-    // candidate neighbours inside the clipped tile (linked to v by kernel A) and outside
-    uint32_t in_fg = 0u, out_fg = 0u, uidx[13];
+    // candidate neighbours inside the clipped tile (linked to v by kernel A) and outside;
+    // no branches: outside words come through a buffer descriptor (an offset past its end,
+    // for a neighbour that does not exist or lies inside, reads 0), so all are in flight
+    uint32_t in_fg = 0u, uidx[13];
+    unsigned long long ow[13];
 #pragma unroll
     for (int q = 0; q < 13; ++q) {
       const int dx = fdx(q), dy = fdy(q), dz = fdz(q);
-      uidx[q] = 0u;
       const int nx = lx + dx, ny = ly + dy, nz = lz + dz;
-      if (nx >= 0 && nx < T.ex && ny >= 0 && ny < T.ey && nz < T.ez) {
-        if ((tb[ny + RTY * nz] >> nx) & 1ull) in_fg |= 1u << q;
-        continue;
-      }
+      const bool inside = nx >= 0 && nx < T.ex && ny >= 0 && ny < T.ey && nz < T.ez;
+      const int row = inside ? ny + RTY * nz : 0;
+      in_fg |= (inside ? (uint32_t)(tb[row] >> (nx & (RTX - 1))) & 1u : 0u) << q;
       // the neighbour at linear offset dz WH + dy W + dx (wrapping across x and y like the
       // reference; one carry per axis for width, height >= 2); it exists below N
       int ux = x + dx, uy = y + dy, uz = z + dz;
-      if (ux < 0) { ux += W; --uy; } else if (ux >= W) { ux -= W; ++uy; }
-      if (uy < 0) { uy += H; --uz; } else if (uy >= H) { uy -= H; ++uz; }
-      if (uz >= D) continue;
-      if ((fgbits[mask_word((uint32_t)ux, (uint32_t)uy, (uint32_t)uz, nbx, nby)] >> (ux & 63)) &
-          1ull) {
-        out_fg |= 1u << q;
-        uidx[q] = (uint32_t)uz * WH + (uint32_t)uy * Wu + (uint32_t)ux;
-      }
+      if (dx < 0) { const bool c = ux < 0; ux = c ? ux + W : ux; uy = c ? uy - 1 : uy; }
+      if (dx > 0) { const bool c = ux >= W; ux = c ? ux - W : ux; uy = c ? uy + 1 : uy; }
+      if (dy < 0 || dx < 0) { const bool c = uy < 0; uy = c ? uy + H : uy; uz = c ? uz - 1 : uz; }
+      if (dy > 0 || dx > 0) { const bool c = uy >= H; uy = c ? uy - H : uy; uz = c ? uz + 1 : uz; }
+      const bool exists = !inside && uz < D;
+      const uint32_t word =
+          (uint32_t)mask_word((uint32_t)ux, (uint32_t)uy, (uint32_t)uz, nbx, nby);
+      ow[q] = __builtin_bit_cast(unsigned long long, __builtin_amdgcn_raw_buffer_load_b64(
+                                     fg_rsrc, exists ? (int)(word * 8u) : -16, 0, 0)) >>
+              (ux & 63);
+      uidx[q] = exists ? (uint32_t)uz * WH + (uint32_t)uy * Wu + (uint32_t)ux : 0u;
     }
+    uint32_t out_fg = 0u;
+#pragma unroll
+    for (int q = 0; q < 13; ++q) out_fg |= ((uint32_t)ow[q] & 1u) << q;
     // outside neighbours not adjacent to one already in v's set (those inside included)
     uint32_t known = in_fg, link = 0u;
 #pragma unroll
     for (int k = 0; k < 13; ++k) {
       const int q = fwd_order(k);
-      if (!((out_fg >> q) & 1u)) continue;
-      if (!(fwd_adj(q) & known)) link |= 1u << q;
-      known |= 1u << q;
+      const uint32_t b = (out_fg >> q) & 1u;
+      link |= ((fwd_adj(q) & known) == 0u ? b : 0u) << q;
+      known |= b << q;
     }
-    uint32_t lj[13];
-#pragma unroll
-    for (int q = 0; q < 13; ++q) lj[q] = ((link >> q) & 1u) ? P[uidx[q]] : 0u;
-    uint32_t seen1 = 0u, seen2 = 0u;
+    uint32_t lj[13];  // the linked neighbours' labels, loaded together (P[0] for the rest)
 #pragma unroll
     for (int q = 0; q < 13; ++q) {
-      const uint32_t l = lj[q];
+      const bool on = (link >> q) & 1u;
+      const uint32_t l = P[on ? uidx[q] : 0u];
+      lj[q] = on ? l : 0u;
+    }
+    uint32_t seen1 = 0u, seen2 = 0u;
+    while (link) {  // one copy of the union code for every link
+      const int q = __builtin_ctz(link);
+      link &= link - 1u;
+      uint32_t l = 0u;
+#pragma unroll
+      for (int k = 0; k < 13; ++k) l = k == q ? lj[k] : l;
       if (!l || l == me || l == seen1 || l == seen2) continue;
       unite(P, me, l);
       seen2 = seen1;
