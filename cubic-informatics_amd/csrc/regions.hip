@@ -170,16 +170,17 @@ constexpr uint32_t EMPTY = 0xFFFFFFFFu;
 
 // Partial sums of one region, then one set of global atomics per region and work-group.
 struct RunSums {
-  uint32_t cnt, sx, sy, sz, isum, imax, nkey;
+  uint32_t cnt, isum, imax, nkey;
+  unsigned long long sx, sy, sz;
   double h[6], ns;
 };
 
 __device__ __forceinline__ void flush_global(RegionAcc* a, const RunSums& v, int is_ave,
                                              bool has_img) {
   atomicAdd(&a->cnt, v.cnt);
-  atomicAdd(&a->sx, (unsigned long long)v.sx);
-  atomicAdd(&a->sy, (unsigned long long)v.sy);
-  atomicAdd(&a->sz, (unsigned long long)v.sz);
+  atomicAdd(&a->sx, v.sx);
+  atomicAdd(&a->sy, v.sy);
+  atomicAdd(&a->sz, v.sz);
 #pragma unroll
   for (int k = 0; k < 6; ++k) unsafeAtomicAdd(&a->h[k], v.h[k]);
   if (is_ave)
@@ -212,20 +213,31 @@ __device__ __forceinline__ void store_final(RegionAcc* a, const RunSums& v) {
 // atomics, so a region pays one set per work-group that touches it instead of one per
 // voxel; a voxel that finds the table full goes straight to the global atomics.  A slot
 // whose region the caller marks `owned` (no other work-group holds any of its voxels) is
-// stored instead, unless some voxel of the work-group found the table full.  The u32
-// partial sums cannot overflow for the <= 2^15 voxels a work-group folds in (2^31 / 2^16).
+// stored instead, unless some voxel of the work-group found the table full.
+// Coordinates are summed relative to the work-group's origin (ox, oy, oz), with the count
+// and the x and y sums packed in one 64-bit word (count: bits 0-15, x: 16-39, y: 40-63)
+// and the z sums in another: two LDS atomics instead of four.  For the < 2^16 voxels of a
+// work-group whose offsets stay below 256 in x and y the fields cannot carry into each
+// other.
 template <int S>
 struct LdsTable {
   uint32_t key[S];
-  uint32_t cnt[S], sx[S], sy[S], sz[S], is[S], im[S], nk[S], own[S];
+  unsigned long long cxy[S];
+  uint32_t dz[S], is[S], im[S], nk[S], own[S];
   double h[6][S], ns[S];
-  uint32_t spilled;
+  uint32_t spilled, ox, oy, oz;
 
-  __device__ void init() {
-    if (threadIdx.x == 0) spilled = 0u;
+  __device__ void init(uint32_t x0, uint32_t y0, uint32_t z0) {
+    if (threadIdx.x == 0) {
+      spilled = 0u;
+      ox = x0;
+      oy = y0;
+      oz = z0;
+    }
     for (int t = threadIdx.x; t < S; t += blockDim.x) {
       key[t] = EMPTY;
-      cnt[t] = sx[t] = sy[t] = sz[t] = is[t] = im[t] = nk[t] = own[t] = 0u;
+      cxy[t] = 0ull;
+      dz[t] = is[t] = im[t] = nk[t] = own[t] = 0u;
 #pragma unroll
       for (int k = 0; k < 6; ++k) h[k][t] = 0.0;
       ns[t] = 0.0;
@@ -255,10 +267,9 @@ struct LdsTable {
       return;
     }
     if (owned) own[slot] = 1u;  // the same value from every voxel of the region
-    atomicAdd(&cnt[slot], 1u);
-    atomicAdd(&sx[slot], x);
-    atomicAdd(&sy[slot], y);
-    atomicAdd(&sz[slot], z);
+    atomicAdd(&cxy[slot], 1ull | ((unsigned long long)(x - ox) << 16) |
+                              ((unsigned long long)(y - oy) << 40));
+    atomicAdd(&dz[slot], z - oz);
 #pragma unroll
     for (int k = 0; k < 6; ++k) unsafeAtomicAdd(&h[k][slot], (double)hv[k]);
     if (is_ave)
@@ -276,7 +287,11 @@ struct LdsTable {
       const uint32_t rr = key[t];
       if (rr == EMPTY) continue;
       RunSums v;
-      v.cnt = cnt[t]; v.sx = sx[t]; v.sy = sy[t]; v.sz = sz[t];
+      const unsigned long long p = cxy[t];
+      v.cnt = (uint32_t)(p & 0xFFFFull);
+      v.sx = ((p >> 16) & 0xFFFFFFull) + (unsigned long long)v.cnt * ox;
+      v.sy = (p >> 40) + (unsigned long long)v.cnt * oy;
+      v.sz = (unsigned long long)dz[t] + (unsigned long long)v.cnt * oz;
       v.isum = is[t]; v.imax = im[t]; v.nkey = nk[t]; v.ns = ns[t];
 #pragma unroll
       for (int k = 0; k < 6; ++k) v.h[k] = h[k][t];
@@ -303,13 +318,13 @@ __global__ __launch_bounds__(NT) void k_accum(const uint32_t* __restrict__ L,
                                               uint32_t cap, int W, int H, int D, i64 N, int nbx,
                                               int nby, int is_ave) {
   __shared__ LdsTable<128> tab;
-  tab.init();
   unsigned b = xcd_tile(blockIdx.x, gridDim.x);
   const int bx = b % nbx; b /= nbx;
   const int by = b % nby; b /= nby;
   const int x = bx * 64 + (threadIdx.x & 63);
   const int y = by * BY + (threadIdx.x >> 6);
   const int zb = (int)b * BZ;
+  tab.init((uint32_t)(bx * 64), (uint32_t)(by * BY), (uint32_t)zb);
   const i64 WH = (i64)W * H;
   const bool has_img = img != nullptr;
   uint32_t l[BZ], r[BZ];
@@ -344,7 +359,9 @@ __global__ __launch_bounds__(NT) void k_accum(const uint32_t* __restrict__ L,
 // kAccGY x kAccGZ Hessian tiles folds their records (voxel index + six elements, SoA) into
 // the table, gathering the label, normalized and raw intensity of each candidate voxel; no
 // pass over the non-candidate voxels.
-static_assert(kAccGY * kAccGZ * kCandTile <= (1 << 15), "u32 partial sums of x, y, z");
+static_assert(kAccGY * kAccGZ * kCandTile < (1 << 16) && kHessTX <= 256 &&
+                  kAccGY * kHessTY <= 256,
+              "packed table sums: count below 2^16, x and y offsets below 256");
 template <typename TN>
 __global__ __launch_bounds__(NT) void k_accum_cand(const float* __restrict__ cand,
                                                    const uint32_t* __restrict__ cand_cnt,
@@ -376,7 +393,7 @@ __global__ __launch_bounds__(NT) void k_accum_cand(const float* __restrict__ can
   }
   if (any == 0) return;  // uniform over the work-group
   __shared__ LdsTable<256> tab;
-  tab.init();
+  tab.init(bx * kHessTX, gy * kAccGY * kHessTY, gz * kAccGZ * kHessTZ);
   const bool has_img = img != nullptr;
   __syncthreads();
 #pragma unroll
