@@ -170,8 +170,8 @@ constexpr uint32_t EMPTY = 0xFFFFFFFFu;
 
 // Partial sums of one region, then one set of global atomics per region and work-group.
 struct RunSums {
-  uint32_t cnt, isum, imax, nkey;
-  unsigned long long sx, sy, sz;
+  uint32_t cnt, imax, nkey;
+  unsigned long long sx, sy, sz, isum;
   double h[6], ns;
 };
 
@@ -188,7 +188,7 @@ __device__ __forceinline__ void flush_global(RegionAcc* a, const RunSums& v, int
   else
     atomicMax(&a->nmax_key, v.nkey);
   if (has_img) {
-    atomicAdd(&a->isum, (unsigned long long)v.isum);
+    atomicAdd(&a->isum, v.isum);
     atomicMax(&a->imax, v.imax);
   }
 }
@@ -216,14 +216,14 @@ __device__ __forceinline__ void store_final(RegionAcc* a, const RunSums& v) {
 // stored instead, unless some voxel of the work-group found the table full.
 // Coordinates are summed relative to the work-group's origin (ox, oy, oz), with the count
 // and the x and y sums packed in one 64-bit word (count: bits 0-15, x: 16-39, y: 40-63)
-// and the z sums in another: two LDS atomics instead of four.  For the < 2^16 voxels of a
-// work-group whose offsets stay below 256 in x and y the fields cannot carry into each
-// other.
+// and the z sums with the raw intensity sums in another (z: bits 0-23, intensity: 24-63):
+// two LDS atomics instead of five.  For the < 2^16 voxels of a work-group whose offsets
+// stay below 256 in x, y and z the fields cannot carry into each other.
 template <int S>
 struct LdsTable {
   uint32_t key[S];
-  unsigned long long cxy[S];
-  uint32_t dz[S], is[S], im[S], nk[S], own[S];
+  unsigned long long cxy[S], zis[S];
+  uint32_t im[S], nk[S], own[S];
   double h[6][S], ns[S];
   uint32_t spilled, ox, oy, oz;
 
@@ -236,8 +236,8 @@ struct LdsTable {
     }
     for (int t = threadIdx.x; t < S; t += blockDim.x) {
       key[t] = EMPTY;
-      cxy[t] = 0ull;
-      dz[t] = is[t] = im[t] = nk[t] = own[t] = 0u;
+      cxy[t] = zis[t] = 0ull;
+      im[t] = nk[t] = own[t] = 0u;
 #pragma unroll
       for (int k = 0; k < 6; ++k) h[k][t] = 0.0;
       ns[t] = 0.0;
@@ -269,17 +269,14 @@ struct LdsTable {
     if (owned) own[slot] = 1u;  // the same value from every voxel of the region
     atomicAdd(&cxy[slot], 1ull | ((unsigned long long)(x - ox) << 16) |
                               ((unsigned long long)(y - oy) << 40));
-    atomicAdd(&dz[slot], z - oz);
+    atomicAdd(&zis[slot], (unsigned long long)(z - oz) | ((unsigned long long)iv << 24));
 #pragma unroll
     for (int k = 0; k < 6; ++k) unsafeAtomicAdd(&h[k][slot], (double)hv[k]);
     if (is_ave)
       unsafeAtomicAdd(&ns[slot], (double)nv);
     else
       atomicMax(&nk[slot], fkey(nv));
-    if (has_img) {
-      atomicAdd(&is[slot], iv);
-      atomicMax(&im[slot], iv);
-    }
+    if (has_img) atomicMax(&im[slot], iv);
   }
 
   __device__ void flush(RegionAcc* acc, int is_ave, bool has_img) {
@@ -291,8 +288,10 @@ struct LdsTable {
       v.cnt = (uint32_t)(p & 0xFFFFull);
       v.sx = ((p >> 16) & 0xFFFFFFull) + (unsigned long long)v.cnt * ox;
       v.sy = (p >> 40) + (unsigned long long)v.cnt * oy;
-      v.sz = (unsigned long long)dz[t] + (unsigned long long)v.cnt * oz;
-      v.isum = is[t]; v.imax = im[t]; v.nkey = nk[t]; v.ns = ns[t];
+      const unsigned long long q = zis[t];
+      v.sz = (q & 0xFFFFFFull) + (unsigned long long)v.cnt * oz;
+      v.isum = q >> 24;
+      v.imax = im[t]; v.nkey = nk[t]; v.ns = ns[t];
 #pragma unroll
       for (int k = 0; k < 6; ++k) v.h[k] = h[k][t];
       if (own[t] && !spilled)
@@ -360,8 +359,8 @@ __global__ __launch_bounds__(NT) void k_accum(const uint32_t* __restrict__ L,
 // the table, gathering the label, normalized and raw intensity of each candidate voxel; no
 // pass over the non-candidate voxels.
 static_assert(kAccGY * kAccGZ * kCandTile < (1 << 16) && kHessTX <= 256 &&
-                  kAccGY * kHessTY <= 256,
-              "packed table sums: count below 2^16, x and y offsets below 256");
+                  kAccGY * kHessTY <= 256 && kAccGZ * kHessTZ <= 256,
+              "packed table sums: count below 2^16, offsets below 256");
 template <typename TN>
 __global__ __launch_bounds__(NT) void k_accum_cand(const float* __restrict__ cand,
                                                    const uint32_t* __restrict__ cand_cnt,
