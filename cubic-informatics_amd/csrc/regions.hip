@@ -250,7 +250,10 @@ struct LdsTable {
     uint32_t slot = (r * 2654435761u) >> (32 - __builtin_ctz(S));
     bool placed = false;
     for (int probes = 0; probes < S; ++probes) {
-      const uint32_t k0 = atomicCAS(&key[slot], EMPTY, r);
+      // a key, once set, never changes: a plain read finds the region's slot, and only an
+      // empty slot needs the compare-and-swap
+      uint32_t k0 = __atomic_load_n(&key[slot], __ATOMIC_RELAXED);
+      if (k0 == EMPTY) k0 = atomicCAS(&key[slot], EMPTY, r);
       if (k0 == EMPTY || k0 == r) { placed = true; break; }
       slot = (slot + 1) & (S - 1);
     }
