@@ -164,7 +164,7 @@ int regional_features_t(hdog_ctx* ctx, hipStream_t s, uint32_t* labels, const TH
   const i64 N = (i64)W * H * D;
   HIPCHK(ensure(ctx->rid, (size_t)N * 4));
   const i64 nb = (N + kRootBlock - 1) / kRootBlock;
-  HIPCHK(ensure(ctx->bcnt, (size_t)nb * 4));
+  HIPCHK(ensure(ctx->bcnt, (size_t)block_count_words(N) * 4));
   HIPCHK(ensure(ctx->rmask, (size_t)nb * (kRootBlock / 8)));
   HIPCHK(ensure(ctx->dR, 64));
   i64 nbo = 0;
@@ -916,7 +916,7 @@ int hdog_batch_plan(hdog_ctx* ctx, const hdog_params* p, const hdog_tile* tiles,
   } need[] = {{&B->tmp1, (size_t)maxN * 8}, {&B->tmp2, (size_t)maxN * 8},
               {&B->norm, maxN * es}, {&B->dog, maxN * ed},
               {&B->labels, (size_t)maxN * 4}, {&B->rid, (size_t)maxN * 4},
-              {&B->bcnt, (size_t)((maxN + kRootBlock - 1) / kRootBlock) * 4},
+              {&B->bcnt, (size_t)block_count_words(maxN) * 4},
               {&B->rmask, (size_t)((maxN + kRootBlock - 1) / kRootBlock) * (kRootBlock / 8)},
               {&B->rmulti, (size_t)((maxN + kRootBlock - 1) / kRootBlock) * (kRootBlock / 8)},
               {&B->dR, 64},

@@ -16,6 +16,13 @@ constexpr int kMaxGaussRadius = 40;            // 2r+1 <= 81 taps per axis
 constexpr int kMaxTaps = 2 * kMaxGaussRadius + 1;
 constexpr int kMaxNormRadius = 200;
 constexpr int kRootBlock = 4096;  // voxels per root-numbering block (64 mask words)
+// per-block root counts of a volume of N voxels, followed (at a 64-word boundary) by the
+// scan's chunk sums, one per 4096 blocks
+inline i64 scan_chunk_offset(i64 nb) { return (nb + 63) / 64 * 64; }
+inline i64 block_count_words(i64 N) {
+  const i64 nb = (N + kRootBlock - 1) / kRootBlock;
+  return scan_chunk_offset(nb) + (nb + 4095) / 4096;
+}
 // word 3 of a raw buffer descriptor on gfx9-family GPUs (32-bit data format; no swizzle)
 constexpr int kBufferWord3 = 0x00020000;
 // Hessian tiles: 64 x 8 outputs per work-group over 16 slices.  Candidate records are kept
@@ -196,6 +203,9 @@ template <typename TD, typename TH>
 // kCandTile * 7 words.  Voxel 0 never has a record (its label is 0 whatever its mask).
 hipError_t launch_hessian(const TD* dog, TH* hess, uint8_t* pd, int W, int H, int D,
                           hipStream_t s, float* cand = nullptr, uint32_t* cand_cnt = nullptr);
+// Exclusive scan of the nb per-block counts in place (block_cnt: block_count_words() words),
+// *d_R = 1 + their total.
+hipError_t launch_scan_counts(uint32_t* block_cnt, i64 nb, uint32_t* d_R, hipStream_t s);
 // Region numbering of a label volume: root counts per block of 4096 voxels and their
 // positions (masks: 64 words per block), then *d_R = 1 + number of roots.
 hipError_t launch_region_ids(const uint32_t* labels, uint32_t* block_cnt,

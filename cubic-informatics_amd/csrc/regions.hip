@@ -68,57 +68,76 @@ __global__ __launch_bounds__(NT) void k_count_roots(const uint32_t* __restrict__
   if (threadIdx.x == 0) cnt[blockIdx.x] = (uint32_t)t;
 }
 
-// Exclusive scan of cnt[0..nb) in place by one work-group of 1024 threads, 4096 entries
-// per step (4 consecutive per thread, coalesced), the next step's entries loaded before
-// the current step is scanned; *d_R = 1 + total.  Totals stay below 2^32 (N < 2^32).
-__global__ __launch_bounds__(1024) void k_scan_counts(uint32_t* cnt, i64 nb, uint32_t* d_R) {
-  constexpr int V = 4, TILE = 1024 * V;
-  __shared__ uint32_t wsum[16];
+// Exclusive scan of cnt[0..nb) in place, *d_R = 1 + total (totals stay below 2^32 as
+// N < 2^32), in three short kernels: the sums of chunks of 4096 counts (one work-group of
+// 1024 threads, 4 consecutive counts each), the exclusive scan of the <= 256 chunk sums
+// (one work-group), and every chunk's scan from its offset.
+constexpr int SCAN_V = 4, SCAN_CH = 1024 * SCAN_V;
+
+// inclusive scan of x over the 1024 threads of the work-group; *total = the sum
+__device__ __forceinline__ uint32_t block_scan_1024(uint32_t x, uint32_t* wsum,
+                                                    uint32_t* total) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  uint32_t carry = 0;
-  uint32_t cur[V], nxt[V];
-  auto load = [&](i64 base, uint32_t* dst) {
-    const i64 i0 = base + (i64)threadIdx.x * V;
 #pragma unroll
-    for (int j = 0; j < V; ++j) dst[j] = (i0 + j < nb) ? cnt[i0 + j] : 0u;
-  };
-  load(0, cur);
-  for (i64 base = 0; base < nb; base += TILE) {
-    if (base + TILE < nb) load(base + TILE, nxt);
-    uint32_t sum = 0;
-#pragma unroll
-    for (int j = 0; j < V; ++j) sum += cur[j];
-    uint32_t x = sum;  // inclusive scan within the wave
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const uint32_t y = __shfl_up(x, o, 64);
-      if (lane >= o) x += y;
-    }
-    if (lane == 63) wsum[wave] = x;
-    __syncthreads();
-    if (wave == 0) {
-      uint32_t w = lane < 16 ? wsum[lane] : 0u;
-#pragma unroll
-      for (int o = 1; o < 16; o <<= 1) {
-        const uint32_t y = __shfl_up(w, o, 64);
-        if (lane >= o) w += y;
-      }
-      if (lane < 16) wsum[lane] = w;
-    }
-    __syncthreads();
-    uint32_t run = carry + (x - sum) + (wave ? wsum[wave - 1] : 0u);
-    const i64 i0 = base + (i64)threadIdx.x * V;
-#pragma unroll
-    for (int j = 0; j < V; ++j) {
-      if (i0 + j < nb) cnt[i0 + j] = run;
-      run += cur[j];
-    }
-    carry += wsum[15];
-    __syncthreads();  // wsum is rewritten by the next step
-#pragma unroll
-    for (int j = 0; j < V; ++j) cur[j] = nxt[j];
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
   }
-  if (threadIdx.x == 0) *d_R = 1u + carry;
+  if (lane == 63) wsum[wave] = x;
+  __syncthreads();
+  if (wave == 0) {
+    uint32_t w = lane < 16 ? wsum[lane] : 0u;
+#pragma unroll
+    for (int o = 1; o < 16; o <<= 1) {
+      const uint32_t y = __shfl_up(w, o, 64);
+      if (lane >= o) w += y;
+    }
+    if (lane < 16) wsum[lane] = w;
+  }
+  __syncthreads();
+  *total = wsum[15];
+  return x + (wave ? wsum[wave - 1] : 0u);
+}
+
+__global__ __launch_bounds__(1024) void k_chunk_sums(const uint32_t* __restrict__ cnt, i64 nb,
+                                                     uint32_t* __restrict__ csum) {
+  __shared__ uint32_t wsum[16];
+  const i64 i0 = (i64)blockIdx.x * SCAN_CH + (i64)threadIdx.x * SCAN_V;
+  uint32_t v = 0;
+#pragma unroll
+  for (int j = 0; j < SCAN_V; ++j) v += (i0 + j < nb) ? cnt[i0 + j] : 0u;
+  uint32_t total;
+  (void)block_scan_1024(v, wsum, &total);
+  if (threadIdx.x == 0) csum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_chunk_offsets(uint32_t* csum, int nch,
+                                                        uint32_t* d_R) {
+  __shared__ uint32_t wsum[16];
+  const uint32_t v = (int)threadIdx.x < nch ? csum[threadIdx.x] : 0u;
+  uint32_t total;
+  const uint32_t incl = block_scan_1024(v, wsum, &total);
+  if ((int)threadIdx.x < nch) csum[threadIdx.x] = incl - v;
+  if (threadIdx.x == 0) *d_R = 1u + total;
+}
+
+__global__ __launch_bounds__(1024) void k_chunk_scan(uint32_t* cnt, i64 nb,
+                                                     const uint32_t* __restrict__ coff) {
+  __shared__ uint32_t wsum[16];
+  const i64 i0 = (i64)blockIdx.x * SCAN_CH + (i64)threadIdx.x * SCAN_V;
+  uint32_t cur[SCAN_V], sum = 0;
+#pragma unroll
+  for (int j = 0; j < SCAN_V; ++j) {
+    cur[j] = (i0 + j < nb) ? cnt[i0 + j] : 0u;
+    sum += cur[j];
+  }
+  uint32_t total;
+  uint32_t run = coff[blockIdx.x] + block_scan_1024(sum, wsum, &total) - sum;
+#pragma unroll
+  for (int j = 0; j < SCAN_V; ++j) {
+    if (i0 + j < nb) cnt[i0 + j] = run;
+    run += cur[j];
+  }
 }
 
 // rid[root] = 1 + rank of the root in linear order; acc[rid].root = root.  One wave per
@@ -534,6 +553,19 @@ __global__ __launch_bounds__(NT) void k_mask_counts(const unsigned long long* __
 
 }  // namespace
 
+// block_cnt holds block_count_words(N) words: the nb counts, then the chunk sums
+hipError_t launch_scan_counts(uint32_t* block_cnt, i64 nb, uint32_t* d_R, hipStream_t s) {
+  const int nch = (int)((nb + SCAN_CH - 1) / SCAN_CH);
+  if (nch > 1024) return hipErrorInvalidValue;  // N < 2^32: at most 256 chunks
+  uint32_t* csum = block_cnt + scan_chunk_offset(nb);
+  hipLaunchKernelGGL(k_chunk_sums, dim3((unsigned)nch), dim3(1024), 0, s, block_cnt, nb, csum);
+  hipLaunchKernelGGL(k_chunk_offsets, dim3(1), dim3(1024), 0, s, csum, nch, d_R);
+  hipLaunchKernelGGL(k_chunk_scan, dim3((unsigned)nch), dim3(1024), 0, s, block_cnt, nb,
+                     (const uint32_t*)csum);
+  mark("regions/scan", s);
+  return hipGetLastError();
+}
+
 hipError_t launch_region_ids(const uint32_t* labels, uint32_t* block_cnt,
                              unsigned long long* masks, uint32_t* d_R, i64 N, hipStream_t s,
                              i64* nblocks_out) {
@@ -544,9 +576,7 @@ hipError_t launch_region_ids(const uint32_t* labels, uint32_t* block_cnt,
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   mark("regions/count_roots", s);
-  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, block_cnt, nb, d_R);
-  mark("regions/scan", s);
-  return hipGetLastError();
+  return launch_scan_counts(block_cnt, nb, d_R, s);
 }
 
 hipError_t launch_region_assign(const unsigned long long* masks, const uint32_t* block_off,
@@ -645,9 +675,7 @@ hipError_t launch_region_count_masks(const unsigned long long* masks, uint32_t* 
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   mark("regions/count_root_bits", s);
-  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, s, block_cnt, nb, d_R);
-  mark("regions/scan", s);
-  return hipGetLastError();
+  return launch_scan_counts(block_cnt, nb, d_R, s);
 }
 
 hipError_t launch_fill_u64(u64* p, i64 n, u64 value, hipStream_t s) {
