@@ -197,7 +197,7 @@ def c3_tiles(n_tiles, seed=3):
 def run_batched(hd, forms, n_tiles, steps):
     import torch
     from hdog import synth
-    cfg = synth.CONFIGS["c1"]
+    cfg = synth.WORKLOADS["c1"]
     shapes = c3_tiles(n_tiles)
     imgs = []
     t0 = time.perf_counter()
@@ -306,7 +306,7 @@ def main():
         torch.cuda.empty_cache()
 
     hd = HDoG(0)
-    cfg = synth.CONFIGS["c1"]
+    cfg = synth.WORKLOADS["c1"]
     shape = (args.depth, cfg["shape"][1], cfg["shape"][2])
     N = shape[0] * shape[1] * shape[2]
     n_blobs = int(round(cfg["n_blobs"] * args.depth / cfg["shape"][0]))

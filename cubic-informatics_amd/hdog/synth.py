@@ -43,8 +43,8 @@ def noise_normal(seed, start, count):
     return np.sqrt(-2.0 * np.log(u1)) * np.cos(2.0 * np.pi * u2)
 
 
-# Configurations: shape (z, y, x), blob count, noise model and blob model.
-CONFIGS = {
+# Synthetic workloads: shape (z, y, x), blob count, noise model and blob model.
+WORKLOADS = {
     # 128x256x256, noise N(200, 20^2), 2000 isotropic blobs sigma U[1.5,3.5], A U[500,3000]
     "c0": dict(shape=(128, 256, 256), n_blobs=2000, seed=0, mean0=200.0, mean1=200.0,
                sd=20.0, sigma=(1.5, 3.5), zfactor=1.0, amp=(500.0, 3000.0)),
@@ -93,7 +93,7 @@ def render(shape, blobs, seed, mean0, mean1, sd):
 
 def make_volume(name=None, shape=None, n_blobs=None, seed=None, **over):
     """Build a named configuration, optionally overriding its shape/blob count/seed."""
-    cfg = dict(CONFIGS[name or "c0"])
+    cfg = dict(WORKLOADS[name or "c0"])
     cfg.update(over)
     if shape is not None:
         cfg["shape"] = tuple(shape)

@@ -44,7 +44,7 @@ def main():
         import numpy as np
         import torch
         from hdog import HDoG, synth
-        cfg = synth.CONFIGS["c1"]
+        cfg = synth.WORKLOADS["c1"]
         nb = int(shape[0] * shape[1] * shape[2] / 2e4)
         blobs = synth.blob_table(shape, nb, cfg["seed"], cfg["sigma"], cfg["zfactor"], cfg["amp"])
         hd = HDoG(0)
