@@ -101,19 +101,16 @@ __global__ __launch_bounds__(NT, 5) void k_hessian(const TD* __restrict__ dog, T
   const uint32_t Wu = (uint32_t)W, WH = Wu * (uint32_t)H;
   auto inplane = [&](int gx, int gy) { return gx >= 0 && gx < W && gy >= 0 && gy < H; };
 
-  // slice-invariant addressing of this thread
-  constexpr bool kRaw = !std::is_same<TD, float>::value || BAND;  // buffer-load fetch
+  // slice-invariant addressing of this thread: the byte offset in the slice; positions
+  // outside the plane read past the slice descriptor's record count
   uint32_t fo[NPF];
-  bool fv[NPF];
 #pragma unroll
   for (int k = 0; k < NPF; ++k) {
     const int e = threadIdx.x + k * NT, lx = e % DX, ly = e / DX;
     const int gx = x0 - 2 + lx, gy = y0 - 2 + ly;
-    fv[k] = e < DY * DX && inplane(gx, gy);
-    // FP32: element index in the slice; 16-bit: byte offset, and positions outside the
-    // plane read past the descriptor's record count
+    const bool fv = e < DY * DX && inplane(gx, gy);
     const uint32_t i = (uint32_t)gy * Wu + (uint32_t)gx;
-    fo[k] = kRaw ? (fv[k] ? i * (uint32_t)sizeof(TD) : 0xFFFFFFF0u) : (fv[k] ? i : 0u);
+    fo[k] = fv ? i * (uint32_t)sizeof(TD) : 0xFFFFFFF0u;
   }
   int hxi[NHX], hyi[NHY], hzi[NHZ];
   bool hxe[NHX], hxv[NHX], hye[NHY], hyv[NHY], hzv[NHZ];
@@ -147,27 +144,19 @@ __global__ __launch_bounds__(NT, 5) void k_hessian(const TD* __restrict__ dog, T
     po[k] = ov[k] ? (uint32_t)gy * Wu + (uint32_t)gx : 0u;
   }
 
-  // dog slices go global -> registers one step ahead -> LDS ring.  FP32: loads under
-  // the position's mask.  16-bit storage: raw buffer loads through a descriptor of the
-  // slice (a position outside the plane has an offset past the record count, a slice
-  // outside the volume has no records, so both read zero), converted when the slice is
-  // written to LDS; converting at the load made every load wait where it was issued.
-  // (Measured at 2160 x 2560 x 512: each form is fastest with its own fetch.)
+  // dog slices go global -> registers one step ahead -> LDS ring, as raw buffer loads
+  // through a descriptor of the slice (a position outside the plane has an offset past
+  // the record count, a slice outside the volume has no records, so both read zero) with
+  // no branch around them; 16-bit storage is converted when the slice is written to LDS
+  // (converting at the load made every load wait where it was issued).
   typename StoredBits<TD>::type pf[NPF];
   const uint32_t slice_bytes = WH * (uint32_t)sizeof(TD);
   auto fetch = [&](int sl) {
     const bool zin = sl >= 0 && sl < D;
-    if constexpr (kRaw) {
-      const __amdgpu_buffer_rsrc_t r =
-          buf_rsrc(dog + (size_t)(zin ? sl : 0) * WH, zin ? slice_bytes : 0u);
+    const __amdgpu_buffer_rsrc_t r =
+        buf_rsrc(dog + (size_t)(zin ? sl : 0) * WH, zin ? slice_bytes : 0u);
 #pragma unroll
-      for (int k = 0; k < NPF; ++k) pf[k] = buf_load<TD>(r, fo[k]);
-    } else {
-      const uint32_t base = zin ? (uint32_t)sl * WH : 0u;
-#pragma unroll
-      for (int k = 0; k < NPF; ++k)
-        pf[k] = (zin && fv[k]) ? __float_as_uint(ld(dog, (i64)(base + fo[k]))) : 0u;
-    }
+    for (int k = 0; k < NPF; ++k) pf[k] = buf_load<TD>(r, fo[k]);
   };
   auto put = [&](float* d) {
 #pragma unroll
