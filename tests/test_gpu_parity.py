@@ -121,6 +121,32 @@ def test_ccl_large_component_and_wrap(hd):
     np.testing.assert_array_equal(got, oracle.ccl3d(mask))
 
 
+@pytest.mark.parametrize("shape", [(20, 17, 130), (1, 1, 1), (33, 9, 64)])
+def test_ccl_empty_single_and_full_masks(hd, shape):
+    # no candidate at all; one candidate (at the last voxel, and at voxel 0, which stays
+    # background); every voxel a candidate (one component, voxel 0 excluded)
+    empty = np.zeros(shape, np.uint8)
+    np.testing.assert_array_equal(to_np(hd.CCL(empty)), np.zeros(shape, np.uint32))
+    one = empty.copy()
+    one.reshape(-1)[-1] = 1
+    np.testing.assert_array_equal(to_np(hd.CCL(one)), oracle.ccl3d(one))
+    zero = empty.copy()
+    zero.reshape(-1)[0] = 1
+    np.testing.assert_array_equal(to_np(hd.CCL(zero)), np.zeros(shape, np.uint32))
+    full = np.ones(shape, np.uint8)
+    np.testing.assert_array_equal(to_np(hd.CCL(full)), oracle.ccl3d(full))
+
+
+def test_pipeline_without_candidates(hd):
+    # a constant stack: no negative-definite voxel, so only the background region
+    vol = np.full((24, 40, 72), 500, np.uint16)
+    o = oracle.substack(vol, C0P)
+    R, b, g = run_gpu(hd, vol, C0P)
+    assert R == len(o["regions"]["label"]) == 1
+    np.testing.assert_array_equal(to_np(b["pd"]), o["pd"])
+    assert not to_np(b["labels"]).any()
+
+
 @pytest.mark.parametrize("ave", [False, True])
 def test_regions_from_oracle_labels(hd, ave):
     vol = synth.make_volume("c0", shape=(32, 64, 72), n_blobs=60, seed=3)
