@@ -258,6 +258,20 @@ __global__ __launch_bounds__(64) void k_gauss_xy_walk(const TS* __restrict__ src
     qa[k] = ld(src, rb + xac);
     qb[k] = ld(src, rb + xbc);
   };
+  // the distinct weights in vector registers for the whole walk: in scalar registers the
+  // unrolled walk ran out of them and reloaded the weights row after row, and every such
+  // scalar load shares its wait counter with the LDS reads
+  float v1[R1 + 1], v2[R2 + 1];
+#pragma unroll
+  for (int i = 0; i <= R1; ++i) {
+    v1[i] = w1[R1 + i];
+    asm volatile("" : "+v"(v1[i]));
+  }
+#pragma unroll
+  for (int i = 0; i <= R2; ++i) {
+    v2[i] = w2[R2 + i];
+    asm volatile("" : "+v"(v2[i]));
+  }
 #pragma unroll
   for (int k = 0; k < PF; ++k) fetch(k, k);
   // x outputs of row y, whose slot is k; fetches row y + PF into the freed slot.  Past
@@ -273,9 +287,9 @@ __global__ __launch_bounds__(64) void k_gauss_xy_walk(const TS* __restrict__ src
     const float* t = ln + l + HX;
     float s1 = 0.0f, s2 = 0.0f;
 #pragma unroll
-    for (int j = -R1; j <= R1; ++j) s1 = fmaf(w1[R1 + (j < 0 ? -j : j)], t[j], s1);
+    for (int j = -R1; j <= R1; ++j) s1 = fmaf(v1[j < 0 ? -j : j], t[j], s1);
 #pragma unroll
-    for (int j = -R2; j <= R2; ++j) s2 = fmaf(w2[R2 + (j < 0 ? -j : j)], t[j], s2);
+    for (int j = -R2; j <= R2; ++j) s2 = fmaf(v2[j < 0 ? -j : j], t[j], s2);
     // pin the row's outputs here: otherwise the compiler sinks the FMA chains towards
     // their uses and keeps many rows' LDS windows live at once
     asm volatile("" : "+v"(s1), "+v"(s2));
@@ -298,9 +312,9 @@ __global__ __launch_bounds__(64) void k_gauss_xy_walk(const TS* __restrict__ src
       if (c >= H) return;
       float s1 = 0.0f, s2 = 0.0f;
 #pragma unroll
-      for (int j = -R1; j <= R1; ++j) s1 = fmaf(w1[R1 + (j < 0 ? -j : j)], ring[(q + j + R) % WB].x, s1);
+      for (int j = -R1; j <= R1; ++j) s1 = fmaf(v1[j < 0 ? -j : j], ring[(q + j + R) % WB].x, s1);
 #pragma unroll
-      for (int j = -R2; j <= R2; ++j) s2 = fmaf(w2[R2 + (j < 0 ? -j : j)], ring[(q + j + R) % WB].y, s2);
+      for (int j = -R2; j <= R2; ++j) s2 = fmaf(v2[j < 0 ? -j : j], ring[(q + j + R) % WB].y, s2);
       if (out) dcol[(i64)c * W] = make_float2(s1, s2);
       ring[q] = xrow(c + R + 1, (q + R + 1) % PF);  // row c + R + 1 replaces row c - R
     }
