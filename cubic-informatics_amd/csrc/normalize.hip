@@ -31,6 +31,24 @@ __device__ __forceinline__ uint32_t pkmin(uint32_t a, uint32_t b) {
   return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(us2, a),
                                                                 __builtin_bit_cast(us2, b)));
 }
+
+// min of w[3 .. 2R] (the window middle shared by four window minima) in four independent
+// chains: consecutive dependent packed 16-bit ops need a wait state between them on
+// gfx950, interleaved chains do not (min is exact, so the order is free)
+template <int R>
+__device__ __forceinline__ uint32_t mid_min(const uint32_t* w) {
+  if constexpr (2 * R - 2 < 4) {  // w[3 .. 2R]: a short chain
+    uint32_t m = w[3];
+#pragma unroll
+    for (int t = 4; t <= 2 * R; ++t) m = pkmin(m, w[t]);
+    return m;
+  } else {
+    uint32_t m[4] = {w[3], w[4], w[5], w[6]};
+#pragma unroll
+    for (int t = 7; t <= 2 * R; ++t) m[(t - 3) & 3] = pkmin(m[(t - 3) & 3], w[t]);
+    return pkmin(pkmin(m[0], m[1]), pkmin(m[2], m[3]));
+  }
+}
 __device__ __forceinline__ uint32_t pack(uint32_t v) { return v | ((65535u - v) << 16); }
 
 // van Herk prefix (P) / suffix (in place over A) minima over blocks of w along one LDS
@@ -138,9 +156,7 @@ __global__ __launch_bounds__(NT) void k_minmax_x_box(const uint32_t* __restrict_
         const uint4 q = *reinterpret_cast<const uint4*>(A + j0 + 4 * v);
         w[4 * v] = q.x; w[4 * v + 1] = q.y; w[4 * v + 2] = q.z; w[4 * v + 3] = q.w;
       }
-      uint32_t mid = w[3];
-#pragma unroll
-      for (int t = 4; t <= 2 * R; ++t) mid = pkmin(mid, w[t]);
+      const uint32_t mid = mid_min<R>(w);
       const uint32_t l2 = w[2], l1 = pkmin(w[1], l2), l0 = pkmin(w[0], l1);
       const uint32_t r1 = w[2 * R + 1], r2 = pkmin(r1, w[2 * R + 2]), r3 = pkmin(r2, w[2 * R + 3]);
       o[0] = pkmin(mid, l0);
@@ -441,9 +457,7 @@ __global__ __launch_bounds__(64) void k_norm_xy_walk(const uint32_t* __restrict_
         const uint4 q = *reinterpret_cast<const uint4*>(A + 4 * l + 4 * v);
         w[4 * v] = q.x; w[4 * v + 1] = q.y; w[4 * v + 2] = q.z; w[4 * v + 3] = q.w;
       }
-      uint32_t mid = w[3];
-#pragma unroll
-      for (int t = 4; t <= 2 * R; ++t) mid = pkmin(mid, w[t]);
+      const uint32_t mid = mid_min<R>(w);
       const uint32_t l2 = w[2], l1 = pkmin(w[1], l2), l0 = pkmin(w[0], l1);
       const uint32_t r1 = w[2 * R + 1], r2 = pkmin(r1, w[2 * R + 2]), r3 = pkmin(r2, w[2 * R + 3]);
       *reinterpret_cast<uint4*>(B + 4 * l) =
