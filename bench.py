@@ -67,7 +67,7 @@ KERNEL_BYTES = {
     "ccl/mask_records": lambda sn, sd, sh, cf: 1 + 4 * cf,
     "regions/count_roots": lambda sn, sd, sh, cf: 4,
     "regions/scan": lambda sn, sd, sh, cf: 0,
-    "regions/zero": lambda sn, sd, sh, cf: 0, "regions/assign_ids": lambda sn, sd, sh, cf: 4,
+    "regions/assign_ids": lambda sn, sd, sh, cf: 4,
     "regions/accum": lambda sn, sd, sh, cf: 4 + 6 * sh + sn,
     # per candidate: its record, label, region id, normalized and raw intensity
     "regions/accum_candidates": lambda sn, sd, sh, cf: cf * (28 + 4 + 4 + sn + 2),

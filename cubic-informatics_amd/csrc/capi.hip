@@ -181,8 +181,8 @@ int regional_features_t(hdog_ctx* ctx, hipStream_t s, uint32_t* labels, const TH
   int rc = ensure_regions(ctx, R);
   if (rc) return rc;
   RegionAcc* acc = (RegionAcc*)ctx->acc.p;
-  HIPCHK(launch_region_assign(masks, (const uint32_t*)ctx->bcnt.p, (uint32_t*)ctx->rid.p, acc, R, R,
-                              nullptr, N, s));
+  HIPCHK(launch_region_assign(masks, (const uint32_t*)ctx->bcnt.p, (uint32_t*)ctx->rid.p, acc, R, N,
+                              s));
   if (cand.rec)
     HIPCHK(launch_region_accum_cand<TN>(cand.rec, cand.cnt, cand.ntiles, cand.rlab, cand.multi,
                                         (const uint32_t*)ctx->rid.p, norm, img, acc, R, W, H,
@@ -379,7 +379,7 @@ hipError_t batch_tile(hdog_ctx* ctx, BatchState* B, int t, hipStream_t s) {
     return e;
   RegionAcc* acc = (RegionAcc*)B->acc.p;
   if ((e = launch_region_assign(masks, (const uint32_t*)B->bcnt.p, (uint32_t*)B->rid.p, acc,
-                                B->acc_cap, B->acc_cap, dR, N, s)) != hipSuccess)
+                                B->acc_cap, N, s)) != hipSuccess)
     return e;
   if ((e = launch_region_accum_cand<TN>((const float*)B->cand.p, (const uint32_t*)B->cand_cnt.p,
                                         ntiles, (const uint32_t*)B->cand_lab.p,

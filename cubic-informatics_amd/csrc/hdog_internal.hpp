@@ -213,11 +213,11 @@ hipError_t launch_scan_counts(uint32_t* block_cnt, i64 nb, uint32_t* d_R, hipStr
 hipError_t launch_region_ids(const uint32_t* labels, uint32_t* block_cnt,
                              unsigned long long* masks, uint32_t* d_R, i64 N, hipStream_t s,
                              i64* nblocks_out);
-// R is the host's region count, or, when d_R is non-null, read on the device as
-// min(*d_R, cap) (graph-captured batches, where the host never sees R); ids >= cap are skipped.
+// Region ids from the root bits and the scanned block counts: rid at every root, and a
+// cleared accumulator line (with the root) for every id below cap; ids >= cap are skipped
+// (graph-captured batches, where the host never sees the count, rely on that).
 hipError_t launch_region_assign(const unsigned long long* masks, const uint32_t* block_off,
-                                uint32_t* rid, RegionAcc* acc, i64 R, i64 cap,
-                                const uint32_t* d_R, i64 N, hipStream_t s);
+                                uint32_t* rid, RegionAcc* acc, i64 cap, i64 N, hipStream_t s);
 template <typename TH, typename TN>
 hipError_t launch_region_accum(const uint32_t* labels, const uint32_t* rid, const TH* hess,
                                const TN* norm, const uint16_t* img, RegionAcc* acc, i64 cap,

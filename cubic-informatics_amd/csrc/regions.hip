@@ -140,9 +140,11 @@ __global__ __launch_bounds__(1024) void k_chunk_scan(uint32_t* cnt, i64 nb,
   }
 }
 
-// rid[root] = 1 + rank of the root in linear order; acc[rid].root = root.  One wave per
-// block of BLK voxels: lane m takes root mask m of the block, a wave scan of the mask
-// popcounts gives each mask's rank offset, and only the (few) roots are visited.
+// rid[root] = 1 + rank of the root in linear order; acc[rid] = zero sums with .root =
+// root (a whole 128-byte line per region, which also clears the accumulator; block 0
+// clears acc[0], the background's).  One wave per block of BLK voxels: lane m takes root
+// mask m of the block, a wave scan of the mask popcounts gives each mask's rank offset,
+// and only the (few) roots are visited.
 __global__ __launch_bounds__(64) void k_assign_ids(const unsigned long long* __restrict__ masks,
                                                    const uint32_t* __restrict__ off,
                                                    uint32_t* __restrict__ rid,
@@ -159,22 +161,19 @@ __global__ __launch_bounds__(64) void k_assign_ids(const unsigned long long* __r
   }
   uint32_t r = 1 + off[blockIdx.x] + (incl - n);
   const i64 base = (i64)blockIdx.x * BLK + 64 * lane;
+  if (blockIdx.x == 0 && lane == 0 && cap > 0) acc[0] = RegionAcc{};
   while (m) {
     const int bit = __builtin_ctzll(m);
     m &= m - 1;
     const i64 i = base + bit;
     rid[i] = r;
-    if ((i64)r < cap) acc[r].root = (uint32_t)i;
+    if ((i64)r < cap) {
+      RegionAcc a{};
+      a.root = (uint32_t)i;
+      acc[r] = a;
+    }
     ++r;
   }
-}
-
-__global__ __launch_bounds__(NT) void k_zero_acc(RegionAcc* acc, i64 R, const uint32_t* d_R,
-                                                  i64 cap) {
-  if (d_R) R = std::min<i64>((i64)*d_R, cap);
-  const i64 nwords = R * (i64)(sizeof(RegionAcc) / 8);
-  unsigned long long* p = (unsigned long long*)acc;
-  for (i64 i = (i64)blockIdx.x * NT + threadIdx.x; i < nwords; i += (i64)gridDim.x * NT) p[i] = 0;
 }
 
 __device__ __forceinline__ uint32_t fkey(float f) {
@@ -580,16 +579,8 @@ hipError_t launch_region_ids(const uint32_t* labels, uint32_t* block_cnt,
 }
 
 hipError_t launch_region_assign(const unsigned long long* masks, const uint32_t* block_off,
-                                uint32_t* rid, RegionAcc* acc, i64 R, i64 cap,
-                                const uint32_t* d_R, i64 N, hipStream_t s) {
-  const i64 words = (d_R ? cap : std::min(R, cap)) * (i64)(sizeof(RegionAcc) / 8);
-  const i64 zb = std::min<i64>((words + NT - 1) / NT, 1 << 16);
+                                uint32_t* rid, RegionAcc* acc, i64 cap, i64 N, hipStream_t s) {
   mark(nullptr, s);  // the host waited for the region count: start a new interval
-  hipLaunchKernelGGL(k_zero_acc, dim3((unsigned)std::max<i64>(zb, 1)), dim3(NT), 0, s, acc, R, d_R,
-                     cap);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  mark("regions/zero", s);
   const i64 nb = (N + BLK - 1) / BLK;
   hipLaunchKernelGGL(k_assign_ids, dim3((unsigned)nb), dim3(64), 0, s, masks, block_off, rid, acc,
                      cap);
