@@ -350,7 +350,7 @@ __global__ __launch_bounds__(NTR) void k_cand_merge(const float* __restrict__ ca
     if (!bal) return;
     uint32_t base = 0;
     if (lane == 0) base = atomicAdd(&n_rim, (uint32_t)__popcll(bal));
-    base = __shfl(base, 0, 64);
+    base = __builtin_amdgcn_readfirstlane(base);  // lane 0 (the whole wave is here)
     if (on) {
       const uint32_t slot = base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
       rim_v[slot] = (uint16_t)v;
@@ -523,7 +523,7 @@ __global__ __launch_bounds__(256) void k_mask_records(const uint8_t* __restrict_
     if (!bal) continue;
     uint32_t base = 0;
     if (lane == 0) base = atomicAdd(&s_cnt, (uint32_t)__popcll(bal));
-    base = __shfl(base, 0, 64);
+    base = __builtin_amdgcn_readfirstlane(base);  // lane 0 (the whole wave is here)
     if (on)
       out[base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] =
           __uint_as_float((uint32_t)(lane + RTX * (ly + RTY * lz)));

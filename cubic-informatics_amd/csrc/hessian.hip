@@ -290,7 +290,7 @@ __global__ __launch_bounds__(NT, 5) void k_hessian(const TD* __restrict__ dog, T
         if (bal) {
           uint32_t base = 0;
           if (lane == 0) base = atomicAdd(&s_cnt, (uint32_t)__popcll(bal));
-          base = __shfl(base, 0, 64);
+          base = __builtin_amdgcn_readfirstlane(base);  // lane 0 (the whole wave is here)
           if (is_cand) {
             const uint32_t slot = base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
             rec[slot] = __uint_as_float(cidx);
