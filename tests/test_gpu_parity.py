@@ -161,6 +161,23 @@ def test_regions_from_oracle_labels(hd, ave):
     assert_regions_match(g, o["regions"], o["hess"])
 
 
+def test_region_numbering_across_scan_chunks(hd):
+    # 48 x 1024 x 1024 voxels: 12288 numbering blocks, so the block-count scan runs over
+    # three chunks of 4096; single-voxel components at random positions
+    D, H, W = 48, 1024, 1024
+    rng = np.random.default_rng(5)
+    idx = np.unique(rng.integers(1, D * H * W, 40000))
+    labels = np.zeros(D * H * W, np.uint32)
+    labels[idx] = idx.astype(np.uint32)
+    labels = labels.reshape(D, H, W)
+    hess = np.zeros((6, D, H, W), np.float32)
+    norm = np.zeros((D, H, W), np.float32)
+    g = hd.RegionalFeatures(labels, hess, norm)
+    assert len(g["label"]) == len(idx) + 1
+    np.testing.assert_array_equal(g["label"][1:], idx.astype(np.uint32))
+    np.testing.assert_array_equal(g["count"][1:], np.ones(len(idx)))
+
+
 def test_region_size_wraps_modulo_65536(hd):
     D, H, W = 4, 160, 120
     labels = np.zeros((D, H, W), np.uint32)
