@@ -132,13 +132,10 @@ static_assert(fwd_order(2) == 0 && fwd_order(5) == 3 && fwd_order(6) == 5 && fwd
                   fwd_order(10) == 9 && fwd_order(11) == 11 && fwd_order(12) == 12,
               "every forward offset once");
 
-// Kernel 1.  With 256 threads on a 32 x 8 x 8 tile, thread t owns the column (t % 32,
-// t / 32) of the tile and k = 0..7 walks its z; every index is 32-bit (N < 2^32).  The
-// foreground voxels are compacted into an LDS list (wave ballots), so the union phase runs
-// one dense pass over the few foreground voxels instead of eight divergent passes over all.
-// ---- candidate-record form (the whole-path call) ------------------------------------
-// The Hessian pass leaves, per 64 x 8 x 16 tile, the positions of its candidate voxels
-// (the foreground of the CCL) in the tile.  Labeling then touches only candidates: P (a
+// ---- labeling of candidate records ---------------------------------------------------
+// The Hessian pass (or launch_mask_records, for a given mask) leaves, per 64 x 8 x 16
+// tile, the positions of its candidate voxels (the foreground of the CCL) in the tile;
+// every index is 32-bit (N < 2^32).  Labeling then touches only candidates: P (a
 // volume-sized parent array) is written and read at candidate voxels only, and the
 // per-record labels go to rlab (slot-aligned with the records).
 //   kernel A: per tile, the tile's candidate bit mask (one word per row, stored tile-major
@@ -149,7 +146,8 @@ static_assert(fwd_order(2) == 0 && fwd_order(5) == 3 && fwd_order(6) == 5 && fwd
 //             outside the clipped tile (by linear offset, wrapping like the reference) with
 //             the global union-find on P;
 //   kernel C: every record's root (the component minimum), root bits for the region
-//             numbering, and with `dense` the final label written back into P.
+//             numbering, the roots of regions that reach beyond one region-pass
+//             work-group, and with `dense` the final label written back into P.
 // 256 threads and 16-bit LDS parents (17 KB of LDS per tile): eight tiles share a CU.
 constexpr int RTX = kHessTX, RTY = kHessTY, RTZ = kHessTZ, RTV = RTX * RTY * RTZ, NTR = 256;
 constexpr int RROWS = RTY * RTZ;  // mask words per tile
