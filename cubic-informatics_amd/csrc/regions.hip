@@ -9,11 +9,13 @@
 //
 // MI355X design: the labels already are the component minima, so a region's id is the
 // rank of its root among all roots in linear order -- a block-count / scan / assign
-// pass that preserves the reference's ascending order.  Values are then reduced straight
-// into one 128-byte accumulator line per region with hardware atomics: u64 for counts
-// and coordinate / intensity sums (exact), FP64 for Hessian and intensity-float sums,
-// ordered-integer atomicMax for the float max.  A finalize kernel turns the sums into
-// the reference's output arrays.
+// pass that preserves the reference's ascending order.  Each work-group folds its voxels
+// into an LDS table of partial sums (packed integer sums, FP64 Hessian and
+// intensity-float sums, ordered-integer max for the float max); each table entry then
+// ends in the region's 128-byte accumulator line, stored when the work-group holds the
+// whole region, else added with hardware atomics (u64 counts and coordinate / intensity
+// sums, exact; FP64 sums).  A finalize kernel turns the sums into the reference's output
+// arrays.
 #include "hdog_internal.hpp"
 
 namespace hdog {
