@@ -149,33 +149,42 @@ __global__ __launch_bounds__(NT, 5) void k_hessian(const TD* __restrict__ dog, T
   // the record count, a slice outside the volume has no records, so both read zero) with
   // no branch around them; 16-bit storage is converted when the slice is written to LDS
   // (converting at the load made every load wait where it was issued).
-  typename StoredBits<TD>::type pf[NPF];
+  // a queue of three register slices: dog(sl) waits in the slot of its LDS ring slot, so
+  // it is loaded three steps before it enters LDS
+  typename StoredBits<TD>::type pf[3][NPF];
   const uint32_t slice_bytes = WH * (uint32_t)sizeof(TD);
-  auto fetch = [&](int sl) {
+  auto fetch = [&](int sl, auto q_c) {
+    constexpr int q = decltype(q_c)::value;
     const bool zin = sl >= 0 && sl < D;
     const __amdgpu_buffer_rsrc_t r =
         buf_rsrc(dog + (size_t)(zin ? sl : 0) * WH, zin ? slice_bytes : 0u);
 #pragma unroll
-    for (int k = 0; k < NPF; ++k) pf[k] = buf_load<TD>(r, fo[k]);
+    for (int k = 0; k < NPF; ++k) pf[q][k] = buf_load<TD>(r, fo[k]);
   };
-  auto put = [&](float* d) {
+  auto put = [&](float* d, auto q_c) {
+    constexpr int q = decltype(q_c)::value;
 #pragma unroll
     for (int k = 0; k < NPF; ++k) {
       const int e = threadIdx.x + k * NT;
-      if (e < DY * DX) d[e] = from_bits<TD>(pf[k]);
+      if (e < DY * DX) d[e] = from_bits<TD>(pf[q][k]);
     }
   };
+  using I0 = std::integral_constant<int, 0>;
+  using I1 = std::integral_constant<int, 1>;
+  using I2 = std::integral_constant<int, 2>;
   // Both rings rotate with period 3, so the walk is unrolled by three steps whose slots
   // are compile-time constants: every LDS access is then a per-thread offset plus an
   // immediate.  Slots at a step: dog(s-2), dog(s-1), dog(s) in DP, DC, DN; H(s-3), H(s-2)
   // in DP, DC and H(s-1) written into DN.
-  fetch(z0 - 2);
-  put(sD[0]);
-  fetch(z0 - 1);
-  put(sD[1]);
-  fetch(z0);
-  put(sD[2]);
-  fetch(z0 + 1);
+  fetch(z0 - 2, I0());
+  fetch(z0 - 1, I1());
+  fetch(z0, I2());
+  put(sD[0], I0());
+  put(sD[1], I1());
+  put(sD[2], I2());
+  fetch(z0 + 1, I0());
+  fetch(z0 + 2, I1());
+  fetch(z0 + 3, I2());
   __syncthreads();
   auto step = [&](int s, auto dp_c, auto dc_c, auto dn_c) {
     constexpr int dp = decltype(dp_c)::value, dc = decltype(dc_c)::value,
@@ -300,14 +309,12 @@ __global__ __launch_bounds__(NT, 5) void k_hessian(const TD* __restrict__ dog, T
         }
       }
     }
-    // 3. dog(s+1) into the slot of dog(s-2) (last read above), loads of dog(s+2) issued
-    put(sD[dp]);
-    fetch(s + 2);
+    // 3. dog(s+1) into the slot of dog(s-2) (last read above); its register slot takes
+    // the loads of dog(s+4)
+    put(sD[dp], decltype(dp_c)());
+    fetch(s + 4, decltype(dp_c)());
     __syncthreads();
   };
-  using I0 = std::integral_constant<int, 0>;
-  using I1 = std::integral_constant<int, 1>;
-  using I2 = std::integral_constant<int, 2>;
   for (int s = z0; s <= z1 + 1; s += 3) {  // the step count is uniform over the work-group
     step(s, I0(), I1(), I2());
     if (s + 1 > z1 + 1) break;
