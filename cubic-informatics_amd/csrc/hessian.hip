@@ -155,7 +155,9 @@ __global__ __launch_bounds__(NT, 5) void k_hessian(const TD* __restrict__ dog, T
   const uint32_t slice_bytes = WH * (uint32_t)sizeof(TD);
   auto fetch = [&](int sl, auto q_c) {
     constexpr int q = decltype(q_c)::value;
-    const bool zin = sl >= 0 && sl < D;
+    // slices past the volume, and past z1 + 1 (the last one this column reads), have no
+    // records: their loads return zero without touching memory
+    const bool zin = sl >= 0 && sl < D && sl <= z1 + 1;
     const __amdgpu_buffer_rsrc_t r =
         buf_rsrc(dog + (size_t)(zin ? sl : 0) * WH, zin ? slice_bytes : 0u);
 #pragma unroll
