@@ -56,7 +56,7 @@ struct hdog_ctx {
   hipStream_t copy_stream = nullptr;
   hipEvent_t up_done[2] = {nullptr, nullptr};   // upload into the slot complete
   hipEvent_t run_done[2] = {nullptr, nullptr};  // last run reading the slot complete
-  int up_depth[2] = {0, 0};
+  int up_shape[2][3] = {{0, 0, 0}, {0, 0, 0}};  // (W, H, D) of each slot's upload
 };
 
 thread_local hdog::Prof* hdog::tl_prof = nullptr;
@@ -1071,7 +1071,9 @@ int hdog_substack_upload(hdog_ctx* ctx, const hdog_params* p, const uint16_t* h_
     HIPCHK(hipMemsetAsync(d + plane * depth_valid, 0, (size_t)(N - plane * depth_valid) * 2,
                           ctx->copy_stream));
   HIPCHK(hipEventRecord(ctx->up_done[slot], ctx->copy_stream));
-  ctx->up_depth[slot] = p->depth;
+  ctx->up_shape[slot][0] = p->width;
+  ctx->up_shape[slot][1] = p->height;
+  ctx->up_shape[slot][2] = p->depth;
   return HDOG_OK;
 }
 
@@ -1080,8 +1082,14 @@ int hdog_substack_run(hdog_ctx* ctx, void* stream, const hdog_params* p, int slo
   if (!ctx) return HDOG_ERR_ARG;
   int rc = check_params(ctx, p);
   if (rc) return rc;
-  if (slot < 0 || slot > 1 || !ctx->up_done[slot] || ctx->up_depth[slot] != p->depth)
-    return fail(ctx, HDOG_ERR_STATE, "slot %d holds no upload of this shape", slot);
+  if (slot < 0 || slot > 1 || !ctx->up_done[slot])
+    return fail(ctx, HDOG_ERR_STATE, "slot %d holds no upload", slot);
+  const int* up = ctx->up_shape[slot];
+  if (up[0] != p->width || up[1] != p->height || up[2] != p->depth ||
+      ctx->up_img[slot].bytes < (size_t)p->width * p->height * p->depth * 2)
+    return fail(ctx, HDOG_ERR_STATE,
+                "slot %d holds an upload of %d x %d x %d, not the %d x %d x %d of these params",
+                slot, up[0], up[1], up[2], p->width, p->height, p->depth);
   HIPCHK(hipSetDevice(ctx->device));
   hipStream_t s = (hipStream_t)stream;
   HIPCHK(hipStreamWaitEvent(s, ctx->up_done[slot], 0));

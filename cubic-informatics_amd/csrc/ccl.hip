@@ -23,21 +23,45 @@ struct Offsets {
   int n;
 };
 
-// Parents only ever decrease, so every chain ends at a root.  Plain (cacheable) loads are
-// enough: a stale parent is still an ancestor, two voxels that ever shared an ancestor are
-// in the same set, and every link goes through an atomicMin whose returned value is
-// current, so each retry of unite strictly lowers `a` and the loop ends.  Path halving
-// (x's parent replaced by its grandparent) keeps the trees shallow; the plain store only
-// ever replaces a parent by one of its ancestors, so no union is lost.
+// Every access to the parent array P while other work-groups update it (the merge and the
+// flatten) is an atomic at agent scope: a relaxed load or store here, the minimum in
+// unite.  Plain accesses beside the atomics would be a data race, and on gfx950 they are
+// issued without the device-scope coherence bit, so they may be served by, or left in, a
+// cache that is not kept coherent with the other XCDs' for the rest of the kernel; the
+// relaxed atomic loads and stores carry that bit (sc1) and cost one coherent access each.
+__device__ __forceinline__ uint32_t ld_parent(const uint32_t* L, uint32_t x) {
+  return __hip_atomic_load(L + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_parent(uint32_t* L, uint32_t x, uint32_t v) {
+  __hip_atomic_store(L + x, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Parents only ever decrease, so every chain ends at a root.  A parent read while others
+// link is at worst out of date, and then still an ancestor: two voxels that ever shared an
+// ancestor are in the same set, and every link goes through an atomicMin whose returned
+// value is current, so each retry of unite strictly lowers `a` and the loop ends.  Path
+// halving (x's parent replaced by its grandparent) keeps the trees shallow; the store only
+// ever replaces a parent by one of its ancestors, so no union is lost (a store that lands
+// after another thread's minimum on the same non-root entry is repaired by that thread's
+// retry, which unites the old parent with its new one).
 // Only the merge may halve paths: in the flatten kernel a halving store could overwrite
 // another voxel's final root with an intermediate ancestor.
 __device__ __forceinline__ uint32_t find_root(uint32_t* L, uint32_t x) {
-  uint32_t p = L[x];
+  uint32_t p = ld_parent(L, x);
   while (p != x) {
-    const uint32_t gp = L[p];
-    if (gp != p) L[x] = gp;
+    const uint32_t gp = ld_parent(L, p);
+    if (gp != p) st_parent(L, x, gp);
     x = p;
     p = gp;
+  }
+  return x;
+}
+// the root of x while other threads only replace parents by roots (the flatten)
+__device__ __forceinline__ uint32_t find_root_flat(const uint32_t* L, uint32_t x) {
+  uint32_t p = ld_parent(L, x);
+  while (p != x) {
+    x = p;
+    p = ld_parent(L, x);
   }
   return x;
 }
@@ -58,10 +82,15 @@ __device__ uint32_t unite(uint32_t* L, uint32_t a, uint32_t b) {
 // The tile-local union-find, in LDS on 16-bit parents (tile positions below 2^16): half
 // the LDS of 32-bit parents, so twice the work-groups per CU.  LDS has no 16-bit atomic
 // minimum; the link is a 32-bit compare-and-swap on the word holding the entry, which
-// fails and retries when either half changed.  Path-halving stores are plain 16-bit
-// stores (a parent is only ever replaced by one of its ancestors, as above).
+// fails and retries when either half changed.  Path-halving stores are 16-bit stores (a
+// parent is only ever replaced by one of its ancestors, as above).  While the unions run,
+// every access to the array is a relaxed work-group-scope atomic (one ds_read_u16 /
+// ds_write_b16 each), so the compiler keeps each load and store where it is written.
+__device__ __forceinline__ uint32_t lab_ld(const uint16_t* lab, uint32_t x) {
+  return __hip_atomic_load(lab + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
 __device__ __forceinline__ uint32_t lds16_find(const uint16_t* lab, uint32_t x) {
-  uint32_t p = lab[x];
+  uint32_t p = lab[x];  // after the unions (read-only)
   while (p != x) {
     x = p;
     p = lab[x];
@@ -69,10 +98,11 @@ __device__ __forceinline__ uint32_t lds16_find(const uint16_t* lab, uint32_t x) 
   return x;
 }
 __device__ __forceinline__ uint32_t lds16_find_halve(uint16_t* lab, uint32_t x) {
-  uint32_t p = lab[x];
+  uint32_t p = lab_ld(lab, x);
   while (p != x) {
-    const uint32_t gp = lab[p];
-    if (gp != p) lab[x] = (uint16_t)gp;
+    const uint32_t gp = lab_ld(lab, p);
+    if (gp != p)
+      __hip_atomic_store(lab + x, (uint16_t)gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     x = p;
     p = gp;
   }
@@ -82,7 +112,7 @@ __device__ __forceinline__ uint32_t lds16_find_halve(uint16_t* lab, uint32_t x) 
 __device__ __forceinline__ uint32_t lds16_min(uint16_t* lab, uint32_t a, uint32_t b) {
   uint32_t* w = reinterpret_cast<uint32_t*>(lab + (a & ~1u));
   const int sh = (int)(a & 1u) * 16;
-  uint32_t cur = *w;
+  uint32_t cur = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   while (true) {
     const uint32_t old = (cur >> sh) & 0xFFFFu;
     if (old <= b) return old;
@@ -226,7 +256,9 @@ __global__ __launch_bounds__(NTR) void k_cand_local(const float* __restrict__ ca
   const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
   const uint32_t n = cand_cnt[tile];
   __shared__ unsigned long long bits[RROWS];  // the tile's mask rows (word = ly + RTY lz)
-  __shared__ uint16_t lab[RTV];              // parents, valid at candidate positions
+  // parents, valid at candidate positions; linked by 32-bit compare-and-swap on the word
+  // holding two of them, so the array starts on a 4-byte boundary
+  __shared__ alignas(4) uint16_t lab[RTV];
   const float* rec = cand + (size_t)tile * kCandTile * rec_words;
   TileRecs R;
   R.load(rec, n);
@@ -274,7 +306,7 @@ __global__ __launch_bounds__(NTR) void k_cand_local(const float* __restrict__ ca
         const int off = q < 9 ? RTX * RTY + RTX * (q / 3 - 1) + (q % 3 - 1)
                               : (q < 12 ? RTX + (q - 10) : 1);
         const uint32_t u = (uint32_t)(v + off);
-        if (lab[u] != root) root = lds16_unite(lab, root, u);
+        if (lab_ld(lab, u) != root) root = lds16_unite(lab, root, u);
       }
     });
   __syncthreads();
@@ -422,7 +454,7 @@ __global__ __launch_bounds__(NTR) void k_cand_merge(const float* __restrict__ ca
 #pragma unroll
     for (int q = 0; q < 13; ++q) {
       const bool on = (link >> q) & 1u;
-      const uint32_t l = P[on ? uidx[q] : 0u];
+      const uint32_t l = ld_parent(P, on ? uidx[q] : 0u);
       lj[q] = on ? l : 0u;
     }
     uint32_t seen1 = 0u, seen2 = 0u;
@@ -468,9 +500,9 @@ __global__ __launch_bounds__(NTR) void k_cand_flatten(const float* __restrict__ 
                          (v & (RTX - 1));
     // any value of P met on the way is an ancestor, also where another thread has already
     // written a root
-    const uint32_t r = find_root_readonly(P, l);
+    const uint32_t r = find_root_flat(P, l);
     rl[i] = r;
-    if (dense && P[idx] != r) P[idx] = r;
+    if (dense && ld_parent(P, idx) != r) st_parent(P, idx, r);
     if (r == idx) {
       atomicOr(roots + (r >> 6), 1ull << (r & 63));
     } else if (l == idx) {

@@ -165,18 +165,6 @@ __device__ __forceinline__ float cdiff_half(float hi, float lo) {
   return s;
 }
 
-// Root of x in a union-find forest stored as parent indices (parents only decrease, a
-// root is its own parent).  Read-only: safe while other threads replace parents by
-// ancestors.
-__device__ __forceinline__ uint32_t find_root_readonly(const uint32_t* L, uint32_t x) {
-  uint32_t p = L[x];
-  while (p != x) {
-    x = p;
-    p = L[x];
-  }
-  return x;
-}
-
 // v rounded to the storage type T and widened back
 template <typename T> __device__ __forceinline__ float round_to(float v) {
   const T t = cvt<T>(v);

@@ -398,6 +398,11 @@ def test_split_upload_and_run_match_the_oracle(hd):
                                p["sigma_xy1"], p["sigma_z1"], p["sigma_xy2"], p["sigma_z2"], 2.0)
         assert C.hdog_substack_run(hd._h, hd._stream(), ctypes.byref(other), 0, None) == \
             C.HDOG_ERR_STATE
+        # so is one of the same depth over a larger plane (it would read past the upload)
+        wider = hd.make_params((24, 48, 128), p["rxy"], p["rz"], p["min_intensity"],
+                               p["sigma_xy1"], p["sigma_z1"], p["sigma_xy2"], p["sigma_z2"], 2.0)
+        assert C.hdog_substack_run(hd._h, hd._stream(), ctypes.byref(wider), 0, None) == \
+            C.HDOG_ERR_STATE
     finally:
         for ptr, _ in bufs:
             C.hdog_host_free(hd._h, ptr)
