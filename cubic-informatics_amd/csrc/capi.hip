@@ -4,6 +4,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "hdog_internal.hpp"
@@ -97,6 +98,57 @@ void release(Buf& b) {
   if (b.p) (void)hipFree(b.p);
   b.p = nullptr;
   b.bytes = 0;
+}
+
+// ---- the no-allocation labeling (hdog_ccl3d_noalloc) ----------------------------------
+// One fixed scratch array per GPU, sized for volumes of at most kNoAllocVoxels voxels once
+// padded to whole Hessian tiles (2^23: a 128 x 256 x 256 stack exactly).  Its layout per
+// call, every part 256-byte aligned: candidate positions and record labels (4 bytes per
+// padded voxel each), per-tile counts, the candidate bit mask (one word per tile row), and
+// the root and region-spanning bitmaps.  The array's name is only a host-side stand-in;
+// kernels get the address hipGetSymbolAddress returns on each device (noalloc_addr).
+constexpr i64 kNoAllocVoxels = (i64)1 << 23;
+constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+struct NoAllocLayout {
+  size_t cand, lab, cnt, fg, roots, multi, total;
+  i64 ntiles, root_words;
+};
+constexpr NoAllocLayout noalloc_layout(i64 ntiles, i64 N) {
+  NoAllocLayout L{};
+  const i64 nb = (N + kRootBlock - 1) / kRootBlock;
+  L.ntiles = ntiles;
+  L.root_words = nb * (kRootBlock / 64);
+  L.cand = 0;
+  L.lab = L.cand + align256((size_t)ntiles * kCandTile * 4);
+  L.cnt = L.lab + align256((size_t)ntiles * kCandTile * 4);
+  L.fg = L.cnt + align256((size_t)ntiles * 4);
+  L.roots = L.fg + align256((size_t)ntiles * kHessTY * kHessTZ * 8);
+  L.multi = L.roots + align256((size_t)L.root_words * 8);
+  L.total = L.multi + align256((size_t)L.root_words * 8);
+  return L;
+}
+constexpr size_t kNoAllocScratchBytes =
+    noalloc_layout(kNoAllocVoxels / kCandTile, kNoAllocVoxels).total;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kMaxDevices = 64;
+void* g_noalloc_addr[kMaxDevices] = {};
+std::mutex g_noalloc_mu;
+
+// the device address of `scratch` on `device` (the current device), looked up once
+hipError_t noalloc_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (g_noalloc_addr[device]) return hipSuccess;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_noalloc_addr[device] = ptr;
+  return hipSuccess;
+}
+void* noalloc_addr(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  return g_noalloc_addr[device];
 }
 
 int check_shape(hdog_ctx* ctx, int W, int H, int D) {
@@ -435,6 +487,8 @@ int hdog_create(hdog_ctx** out, int device) {
   if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return HDOG_ERR_HIP;
   if (device < 0 || device >= n) return HDOG_ERR_ARG;
   if (hipSetDevice(device) != hipSuccess) return HDOG_ERR_HIP;
+  // the no-allocation labeling's scratch address on this device (set-up, once per device)
+  if (noalloc_init(device) != hipSuccess) return HDOG_ERR_HIP;
   hdog_ctx* c = new hdog_ctx();
   c->device = device;
   if (hipHostMalloc((void**)&c->h_R, 64, hipHostMallocDefault) != hipSuccess) {
@@ -633,6 +687,36 @@ int hdog_ccl3d(hdog_ctx* ctx, void* stream, const uint8_t* d_pd, uint32_t* d_lab
                                (uint32_t*)ctx->cand_lab.p, (unsigned long long*)ctx->rmask.p,
                                (unsigned long long*)ctx->rmulti.p, nb * (kRootBlock / 64), true,
                                width, height, depth, s, 1));
+  return HDOG_OK;
+}
+
+int hdog_ccl3d_noalloc(hdog_ctx* ctx, void* stream, const uint8_t* d_pd, uint32_t* d_labels,
+                       int width, int height, int depth) {
+  if (!ctx) return HDOG_ERR_ARG;
+  int rc = check_shape(ctx, width, height, depth);
+  if (rc) return rc;
+  if (!d_pd || !d_labels) return fail(ctx, HDOG_ERR_ARG, "NULL device pointer");
+  const i64 N = (i64)width * height * depth, ntiles = hessian_tiles(width, height, depth);
+  if (ntiles * kCandTile > kNoAllocVoxels)
+    return fail(ctx, HDOG_ERR_ARG,
+                "hdog_ccl3d_noalloc: %d x %d x %d pads to %lld voxels of Hessian tiles, more "
+                "than its fixed scratch holds (%lld)", width, height, depth,
+                (long long)(ntiles * kCandTile), (long long)kNoAllocVoxels);
+  HIPCHK(hipSetDevice(ctx->device));
+  unsigned char* base = (unsigned char*)noalloc_addr(ctx->device);
+  if (!base) return fail(ctx, HDOG_ERR_STATE, "no scratch address for device %d", ctx->device);
+  const NoAllocLayout L = noalloc_layout(ntiles, N);
+  const hipStream_t s = (hipStream_t)stream;
+  // every part is written before it is read: the counts and positions by the mask pass, the
+  // mask rows and record labels by the tile-local labeling, the bitmaps and the label
+  // volume cleared by fill kernels inside launch_ccl_candidates
+  HIPCHK(launch_mask_records(d_pd, (float*)(base + L.cand), (uint32_t*)(base + L.cnt), width,
+                             height, depth, s));
+  HIPCHK(launch_ccl_candidates((const float*)(base + L.cand), (const uint32_t*)(base + L.cnt),
+                               ntiles, (unsigned long long*)(base + L.fg), d_labels,
+                               (uint32_t*)(base + L.lab), (unsigned long long*)(base + L.roots),
+                               (unsigned long long*)(base + L.multi), L.root_words, true, width,
+                               height, depth, s, 1));
   return HDOG_OK;
 }
 

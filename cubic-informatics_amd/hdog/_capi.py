@@ -70,6 +70,7 @@ hdog_init_gaussian3d = _sig("hdog_init_gaussian3d", [_vp, _f, _f, _f, _f, _d, ct
 hdog_dog3d = _sig("hdog_dog3d", [_vp, _vp, _vp, _vp, _i, _i, _i, _f])
 hdog_hessian_pd = _sig("hdog_hessian_pd", [_vp, _vp, _vp, _vp, _vp, _i, _i, _i])
 hdog_ccl3d = _sig("hdog_ccl3d", [_vp, _vp, _vp, _vp, _i, _i, _i])
+hdog_ccl3d_noalloc = _sig("hdog_ccl3d_noalloc", [_vp, _vp, _vp, _vp, _i, _i, _i])
 hdog_regional_features = _sig("hdog_regional_features", [_vp, _vp, _vp, _vp, _vp, _vp, _i, _i, _i,
                                                          _i, ctypes.POINTER(_i64)])
 hdog_regions_download = _sig("hdog_regions_download", [_vp, _vp, ctypes.POINTER(Regions)])
@@ -106,7 +107,7 @@ hdog_soft_backward = _sig("hdog_soft_backward", [_vp, _vp, _vp, _vp, _vp, _vp, _
 
 EXPORTED = ["hdog_create", "hdog_destroy", "hdog_last_error", "hdog_release_scratch",
             "hdog_normalize3d", "hdog_normalize3d_ed", "hdog_init_gaussian3d", "hdog_dog3d", "hdog_hessian_pd",
-            "hdog_ccl3d", "hdog_regional_features", "hdog_regions_download",
+            "hdog_ccl3d", "hdog_ccl3d_noalloc", "hdog_regional_features", "hdog_regions_download",
             "hdog_process_substack", "hdog_run_device", "hdog_synth_volume",
             "hdog_set_profiling", "hdog_stage_times", "hdog_soft_forward", "hdog_soft_backward",
             "hdog_batch_plan", "hdog_batch_run", "hdog_batch_tile_info", "hdog_batch_download",

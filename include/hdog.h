@@ -130,6 +130,17 @@ int hdog_hessian_pd(hdog_ctx* ctx, void* stream, const float* d_dog, float* d_he
 int hdog_ccl3d(hdog_ctx* ctx, void* stream, const uint8_t* d_pd, uint32_t* d_labels,
                int width, int height, int depth);
 
+/* hdog_ccl3d without any allocation: no hipMalloc, hipFree or allocating library call, so
+ * it can be called where allocation is not allowed (under stream capture into a HIP graph,
+ * in tight serving loops, in harnesses that forbid it).  It has no counterpart in the
+ * reference.  Its scratch is one fixed device array per GPU, held until the process ends,
+ * sized for volumes that pad to at most 2^23 voxels of 64 x 8 x 16 Hessian tiles (about
+ * 70 MB); larger volumes are refused with HDOG_ERR_ARG.  Every call on a GPU shares that
+ * array, so calls that overlap on one GPU (other streams, other contexts, other threads)
+ * race and must be serialized by the caller.  Same results as hdog_ccl3d. */
+int hdog_ccl3d_noalloc(hdog_ctx* ctx, void* stream, const uint8_t* d_pd, uint32_t* d_labels,
+                       int width, int height, int depth);
+
 /* Per-region features of a labelled substack: Hessian sums, eigen-features, max (or
  * sum) of normalized intensity, size, centroid, plus exact intensity sum / max.
  * d_img may be NULL (intensity outputs then read 0).  Results stay on the device in
