@@ -20,8 +20,11 @@ kernel (event-timed in a profiled pass after the timed one) against the 8 TB/s H
 peak; `pipeline_roofline` is the whole step against the 80 B/voxel (46 B/voxel for
 FP16/BF16 storage, 50 with an FP32 DoG) stage-boundary traffic model, and beside it
 the model of what the step actually moves: the Hessian handed to the region pass as
-28-byte records of the candidate voxels only (about 32 B/voxel at 6.6 % candidates).  `cpu_baseline` times the CPU oracle
-(OpenMP, all host threads) on one 32-slice substack of the same stack.
+28-byte records of the candidate voxels only (about 32 B/voxel at 6.6 % candidates).
+`cpu_baseline` times the reference's CPU detector (a scipy counterpart of
+script/HDoG_cpu.py) on a 32 x 540 x 640 crop of the same stack, median of 3, on all host
+cores (filter stages on slabs in worker processes), with its single-process time and the
+C++ oracle's beside it.
 
 `batched_form` is the many-stack form: 40 stacks of z ~ U{64..640} and y, x each from
 {512, 1024, 1536, 2160} (seeded), the same noise model and parameters at one blob per
@@ -166,22 +169,55 @@ def run_form(hd, img, shape, precision, steps, warmup, keep_pd=False):
                 pd=pd, cand=cand)
 
 
-def cpu_baseline(img, depth):
-    """CPU oracle (OpenMP over all host threads) on one `depth`-slice substack."""
+def cpu_baseline(img, depth, height, width, runs=3):
+    """The reference's CPU detector (a from-scratch scipy counterpart of
+    script/HDoG_cpu.py, oracle/hdog_cpu_scipy.py) on a centred depth x height x width crop of
+    the same stack, median of `runs`: in one process (scipy.ndimage is single-threaded), and
+    with the filter stages on slabs in one worker process per host core.  Beside it the C++
+    oracle (the GPU path's semantics, OpenMP over the host cores) on the same crop.  `value`
+    is the all-cores scipy run."""
+    import statistics
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import hdog_cpu_scipy
     import oracle
-    z0 = max(0, img.shape[0] // 2 - depth // 2)
-    sub = img[z0:z0 + depth].cpu().numpy()
+    D, H, W = img.shape
+    z0, y0, x0 = max(0, D // 2 - depth // 2), max(0, H // 2 - height // 2), max(0, W // 2 - width // 2)
+    sub = img[z0:z0 + depth, y0:y0 + height, x0:x0 + width].cpu().numpy()
+    cores = len(os.sched_getaffinity(0))
+    det = hdog_cpu_scipy.HDoGCPU((C1["sigma_z1"], C1["sigma_xy1"], C1["sigma_xy1"]),
+                                 (C1["sigma_z2"], C1["sigma_xy2"], C1["sigma_xy2"]), C1["rxy"],
+                                 C1["rz"], C1["min_intensity"], C1["gamma"], C1["truncate"])
     p = dict(min_intensity=C1["min_intensity"], rxy=C1["rxy"], rz=C1["rz"],
              sigma_xy1=C1["sigma_xy1"], sigma_z1=C1["sigma_z1"], sigma_xy2=C1["sigma_xy2"],
              sigma_z2=C1["sigma_z2"], truncate=C1["truncate"], gamma=C1["gamma"])
-    t0 = time.perf_counter()
-    o = oracle.substack(sub, p)
-    t = time.perf_counter() - t0
-    cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
-    return dict(value=sub.size / t / 1e9, unit="Gvoxel/s", cores=cores, kind="port",
-                sample=f"{depth}x{sub.shape[1]}x{sub.shape[2]} substack (z {z0}..{z0 + depth - 1}) "
-                       f"of the same stack, whole path, {t:.1f} s, {len(o['regions']['label'])} regions")
+
+    def median_time(f):
+        ts, out = [], None
+        for _ in range(runs):
+            t0 = time.perf_counter()
+            out = f()
+            ts.append(time.perf_counter() - t0)
+        return statistics.median(ts), out
+
+    t_single, reg = median_time(lambda: det.run(sub))
+    t_slabs, reg_s = median_time(lambda: det.run_slabs(sub, cores))
+    t_oracle, o = median_time(lambda: oracle.substack(sub, p))
+    n = sub.size
+    desc = (f"{depth}x{sub.shape[1]}x{sub.shape[2]} crop (z {z0}, y {y0}, x {x0}) of the same "
+            f"stack, whole path, median of {runs}")
+    log(f"CPU baseline on the {desc}: scipy 1 process {t_single:.2f} s, scipy {cores} processes "
+        f"{t_slabs:.2f} s, oracle {cores} threads {t_oracle:.2f} s")
+    return dict(value=n / t_slabs / 1e9, unit="Gvoxel/s", cores=cores, kind="port",
+                sample=desc + f"; scipy counterpart of script/HDoG_cpu.py, filter stages on "
+                              f"{cores} slabs in {cores} processes, {reg_s['n']} regions",
+                variants={
+                    "scipy_single_process": dict(value=n / t_single / 1e9, cores=1,
+                                                 seconds=t_single, regions=int(reg["n"])),
+                    "scipy_slabs": dict(value=n / t_slabs / 1e9, cores=cores, seconds=t_slabs,
+                                        regions=int(reg_s["n"])),
+                    "oracle_openmp": dict(value=n / t_oracle / 1e9, cores=cores,
+                                          seconds=t_oracle,
+                                          regions=len(o["regions"]["label"]) - 1)})
 
 
 def c3_tiles(n_tiles, seed=3):
@@ -276,7 +312,9 @@ def main():
     ap.add_argument("--forms", default="fp32,fp16,fp16_dog32",
                     help="storage forms to time; the first one gives `value`")
     ap.add_argument("--depth", type=int, default=512, help="stack depth (z) of the workload")
-    ap.add_argument("--cpu-depth", type=int, default=32, help="substack depth of the CPU sample; 0 skips")
+    ap.add_argument("--cpu-depth", type=int, default=32, help="depth of the CPU sample; 0 skips")
+    ap.add_argument("--cpu-crop", default="540x640",
+                    help="height x width of the CPU sample (a centred crop of the stack)")
     ap.add_argument("--batch-forms", default="fp32,fp16",
                     help="storage forms of the batched many-stack run; empty skips it")
     ap.add_argument("--batch-stacks", type=int, default=40)
@@ -375,8 +413,8 @@ def main():
         "wall_s_timed": r["wall"],
     }
     if args.cpu_depth > 0:
-        log(f"CPU oracle on a {args.cpu_depth}-slice substack ...")
-        out["cpu_baseline"] = cpu_baseline(img, args.cpu_depth)
+        ch, cw = (int(v) for v in args.cpu_crop.split("x"))
+        out["cpu_baseline"] = cpu_baseline(img, args.cpu_depth, ch, cw)
     else:
         out["cpu_baseline"] = None
     bforms = [f for f in args.batch_forms.split(",") if f]
