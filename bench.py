@@ -22,7 +22,7 @@ FP16/BF16 storage, 50 with an FP32 DoG) stage-boundary traffic model, and beside
 the model of what the step actually moves: the Hessian handed to the region pass as
 28-byte records of the candidate voxels only (about 32 B/voxel at 6.6 % candidates).
 `cpu_baseline` times the reference's CPU detector (a scipy counterpart of
-script/HDoG_cpu.py) on a 32 x 540 x 640 crop of the same stack, median of 3, on all host
+script/HDoG_cpu.py) on a 32 x 1080 x 1280 crop of the same stack, median of 3, on all host
 cores (filter stages on slabs in worker processes), with its single-process time and the
 C++ oracle's beside it.
 
@@ -84,7 +84,7 @@ FORM_SIZES = {"fp32": (4, 4, 4), "fp16": (2, 2, 2), "bf16": (2, 2, 2),
 def pipeline_bytes(form):
     """Stage-boundary model: img 2 + norm (write, read by DoG, read by regions) + dog (write,
     read by Hessian) + 6 Hessian elements (write, read by regions) + mask 2 + labels 8."""
-    sn, sd, sh = FORM_SIZES[form]
+    sn, sd, sh = FORM_SIZES[split_form(form)[0]]
     return 2 + 3 * sn + 2 * sd + 12 * sh + 2 + 8
 
 
@@ -123,11 +123,19 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_form(hd, img, shape, precision, steps, warmup, keep_pd=False):
+def split_form(spec):
+    """'fp16' -> ('fp16', 0.0); 'fp16/1e-3' -> ('fp16', 1e-3): storage form and FP64
+    re-evaluation band of the leading minors (0: the library default)."""
+    name, _, band = spec.partition("/")
+    return name, float(band) if band else 0.0
+
+
+def run_form(hd, img, shape, spec, steps, warmup, keep_pd=False):
     import torch
+    precision, band = split_form(spec)
     params = hd.make_params(shape, C1["rxy"], C1["rz"], C1["min_intensity"], C1["sigma_xy1"],
                             C1["sigma_z1"], C1["sigma_xy2"], C1["sigma_z2"], C1["truncate"],
-                            C1["gamma"], False, precision)
+                            C1["gamma"], False, precision, minor_band=band)
     bufs = hd.alloc_intermediates(shape, precision)
     # the step's product is the region arrays: the Hessian volumes, the label volume and
     # the byte mask are not requested, so the library does not materialize them (labeling
@@ -167,6 +175,46 @@ def run_form(hd, img, shape, precision, steps, warmup, keep_pd=False):
     torch.cuda.empty_cache()
     return dict(R=R, wall=wall, dev_ms=dev_ms, kernels={k: sum(v) / len(v) for k, v in kms.items()},
                 pd=pd, cand=cand)
+
+
+def precision_errors(hd, img, forms, depth):
+    """Per-stage error, candidate Jaccard and region-feature error of each storage form
+    against the FP32 path on a full-plane `depth`-slice substack (the middle of the stack),
+    every intermediate materialized (hdog.precision)."""
+    import torch
+    from hdog import precision
+    z0 = max(0, img.shape[0] // 2 - depth // 2)
+    sub = img[z0:z0 + depth].contiguous()
+    shape = tuple(sub.shape)
+
+    def run(form):
+        params = hd.make_params(shape, C1["rxy"], C1["rz"], C1["min_intensity"],
+                                C1["sigma_xy1"], C1["sigma_z1"], C1["sigma_xy2"], C1["sigma_z2"],
+                                C1["truncate"], C1["gamma"], False, form)
+        bufs = hd.alloc_intermediates(shape, form)
+        hd.run_device(sub, params, bufs)
+        g = hd.download_regions()
+        reg = dict(label=g["label"], count=g["count"], structureness=g["structureness"],
+                   blobness=g["blobness"], norm=g["norm"])
+        del bufs["labels"]
+        return bufs, reg
+
+    ref, ref_reg = run("fp32")
+    out = {}
+    for f in forms:
+        b, reg = run(f)
+        out[f] = dict(substack=f"{depth}x{shape[1]}x{shape[2]} (z {z0}..{z0 + depth - 1})",
+                      stages=precision.stage_errors(b["norm"], b["dog"], b["hess"], ref),
+                      candidate_jaccard=precision.jaccard(b["pd"], ref["pd"]),
+                      regions=precision.region_errors(reg, ref_reg))
+        log(f"[{f}] vs fp32 on the substack: Jaccard {out[f]['candidate_jaccard']:.5f}, norm "
+            f"max_rel {out[f]['stages']['norm']['max_rel']:.2e}, dog max_rel "
+            f"{out[f]['stages']['dog']['max_rel']:.2e}")
+        del b
+        torch.cuda.empty_cache()
+    del ref, sub
+    torch.cuda.empty_cache()
+    return out
 
 
 def cpu_baseline(img, depth, height, width, runs=3):
@@ -309,11 +357,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--forms", default="fp32,fp16,fp16_dog32",
-                    help="storage forms to time; the first one gives `value`")
+    ap.add_argument("--forms", default="fp32,fp16,bf16,fp16_dog32,fp16/1e-3",
+                    help="storage forms to time (FORM or FORM/BAND, BAND the FP64 re-evaluation "
+                         "band of the minors); the first one gives `value`")
     ap.add_argument("--depth", type=int, default=512, help="stack depth (z) of the workload")
+    ap.add_argument("--error-depth", type=int, default=32,
+                    help="depth of the full-plane substack on which the storage forms' per-stage "
+                         "errors against FP32 are measured; 0 skips")
     ap.add_argument("--cpu-depth", type=int, default=32, help="depth of the CPU sample; 0 skips")
-    ap.add_argument("--cpu-crop", default="540x640",
+    ap.add_argument("--cpu-crop", default="1080x1280",
                     help="height x width of the CPU sample (a centred crop of the stack)")
     ap.add_argument("--batch-forms", default="fp32,fp16",
                     help="storage forms of the batched many-stack run; empty skips it")
@@ -371,6 +423,10 @@ def main():
     achieved = kb * N / (kernels[dom] * 1e-3) / 1e9
     pipe_gbs = pipeline_bytes(main_f) * N / (ms_step * 1e-3) / 1e9
 
+    errs = {}
+    if len(forms) > 1 and main_f == "fp32" and args.error_depth > 0:
+        errs = precision_errors(hd, img, [f for f in forms[1:] if "/" not in f],
+                                args.error_depth)
     forms_out = {}
     for f in forms[1:]:
         rf = res[f]
@@ -384,6 +440,8 @@ def main():
                             num_regions=rf["R"],
                             pipeline_roofline_frac=pipeline_bytes(f) * N / (msf * 1e-3) / 1e9 / HBM_PEAK_GBS,
                             kernels_ms={k: round(v, 4) for k, v in rf["kernels"].items()})
+        if f in errs:
+            forms_out[f]["errors_vs_fp32"] = errs[f]
 
     out = {
         "metric": "Gvoxel/s HDoG pipeline, 2160x2560x512 u16; FP16 vs FP32 at fixed accuracy",

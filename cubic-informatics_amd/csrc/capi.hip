@@ -151,6 +151,11 @@ void* noalloc_addr(int device) {
   return g_noalloc_addr[device];
 }
 
+// the FP64 re-evaluation band of the reduced-precision forms (0 selects the default)
+float minor_band(const hdog_params* p) {
+  return p->minor_band > 0.0f ? p->minor_band : kDefaultMinorBand;
+}
+
 int check_shape(hdog_ctx* ctx, int W, int H, int D) {
   if (W < 1 || H < 1 || D < 1)
     return fail(ctx, HDOG_ERR_ARG, "invalid shape %dx%dx%d (width x height x depth)", W, H, D);
@@ -301,7 +306,7 @@ int run_stages(hdog_ctx* ctx, hipStream_t s, const hdog_params* p, const uint16_
   cand.rlab = (const uint32_t*)ctx->cand_lab.p;
   cand.multi = (const unsigned long long*)ctx->rmulti.p;
   HIPCHK(launch_hessian<TD, TH>(dog, hess, pd, W, H, D, s, (float*)ctx->cand.p,
-                                (uint32_t*)ctx->cand_cnt.p));
+                                (uint32_t*)ctx->cand_cnt.p, minor_band(p)));
   HIPCHK(launch_ccl_candidates(cand.rec, cand.cnt, cand.ntiles,
                                (unsigned long long*)ctx->fgbits.p, labels,
                                (uint32_t*)ctx->cand_lab.p, (unsigned long long*)ctx->rmask.p,
@@ -324,6 +329,8 @@ int check_params(hdog_ctx* ctx, const hdog_params* p) {
                 p->radius_z, kMaxNormRadius);
   if (p->precision < HDOG_FP32 || p->precision > HDOG_BF16_DOG32)
     return fail(ctx, HDOG_ERR_ARG, "unknown precision %d", p->precision);
+  if (!(p->minor_band >= 0.0f && p->minor_band < 0.5f))
+    return fail(ctx, HDOG_ERR_ARG, "minor_band %g outside [0, 0.5)", (double)p->minor_band);
   return HDOG_OK;
 }
 
@@ -411,7 +418,7 @@ hipError_t batch_tile(hdog_ctx* ctx, BatchState* B, int t, hipStream_t s) {
     return e;
   const i64 ntiles = hessian_tiles(W, H, D);
   if ((e = launch_hessian<TD, TH>(dog, hess, nullptr, W, H, D, s, (float*)B->cand.p,
-                                  (uint32_t*)B->cand_cnt.p)) != hipSuccess)
+                                  (uint32_t*)B->cand_cnt.p, minor_band(&B->p))) != hipSuccess)
     return e;
   i64 nbo = 0;
   uint32_t* dR = (uint32_t*)B->dR.p;

@@ -34,6 +34,9 @@ constexpr int kBufferWord3 = 0x00020000;
 #endif
 constexpr int kHessTX = 64, kHessTY = HDOG_HESS_TY, kHessTZ = 16;
 constexpr int kCandTile = kHessTX * kHessTY * kHessTZ;
+// reduced-precision forms: default FP64 re-evaluation band of the leading minors (relative
+// to the sum of the magnitudes of their terms)
+constexpr float kDefaultMinorBand = 1e-5f;
 inline i64 hessian_tiles(int W, int H, int D) {
   return (i64)((W + kHessTX - 1) / kHessTX) * ((H + kHessTY - 1) / kHessTY) *
          ((D + kHessTZ - 1) / kHessTZ);
@@ -191,8 +194,10 @@ template <typename TD, typename TH>
 // records: the voxel's position in the tile, lx + kHessTX (ly + kHessTY lz), then the six
 // elements rounded to TH, as 7 SoA words of kCandTile slots; cand holds hessian_tiles() *
 // kCandTile * 7 words.  Voxel 0 never has a record (its label is 0 whatever its mask).
+// band: the FP64 re-evaluation band of the leading minors when TD or TH is a 2-byte type.
 hipError_t launch_hessian(const TD* dog, TH* hess, uint8_t* pd, int W, int H, int D,
-                          hipStream_t s, float* cand = nullptr, uint32_t* cand_cnt = nullptr);
+                          hipStream_t s, float* cand = nullptr, uint32_t* cand_cnt = nullptr,
+                          float band = kDefaultMinorBand);
 // Exclusive scan of the nb per-block counts in place (block_cnt: block_count_words() words),
 // *d_R = 1 + their total.
 hipError_t launch_scan_counts(uint32_t* block_cnt, i64 nb, uint32_t* d_R, hipStream_t s);

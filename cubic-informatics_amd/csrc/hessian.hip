@@ -15,8 +15,8 @@
 //
 // Reduced-precision storage (FP16 / BF16): the elements are computed in FP32 from the
 // stored DoG and rounded only on the store; the candidate test uses the FP32 values, and
-// when a leading minor lies within 1e-5 of the sum of the magnitudes of its terms, the
-// minors are re-evaluated in FP64 from the same FP32 elements.
+// when a leading minor lies within a band (1e-5 by default) of the sum of the magnitudes
+// of its terms, the minors are re-evaluated in FP64 from the same FP32 elements.
 #include "hdog_internal.hpp"
 
 namespace hdog {
@@ -34,16 +34,15 @@ __device__ __forceinline__ uint8_t negdef32(float xx, float xy, float xz, float 
   return (xx < 0 && m2 > 0 && m3 < 0) ? 1 : 0;
 }
 
-// Reduced-precision forms: a leading minor within kBand of the sum of its terms'
+// Reduced-precision forms: a leading minor within `band` of the sum of its terms'
 // magnitudes has its sign re-decided in FP64 from the same FP32 elements.  The FP32
 // evaluation error of these minors is at most about 6 * 2^-24 (3.6e-7) of that sum (two
-// or three rounded products per term, four rounded additions), so a band of 1e-5 covers
-// every sign the FP32 rounding can flip, with a margin of about 28.  The FP32 minors and
-// the term magnitudes share their products with negdef32's evaluation (same order, same
-// roundings).
-constexpr float kBand = 1e-5f;
+// or three rounded products per term, four rounded additions), so the default band of
+// 1e-5 covers every sign the FP32 rounding can flip, with a margin of about 28.  The FP32
+// minors and the term magnitudes share their products with negdef32's evaluation (same
+// order, same roundings).
 __device__ __forceinline__ uint8_t negdef_band(float xx, float xy, float xz, float yy, float yz,
-                                               float zz) {
+                                               float zz, float band) {
   const float p1 = xx * yy, p2 = xy * xy;
   const float m2 = p1 - p2;
   const float t1 = p1 * zz, t2 = 2 * xy * yz * xz, t3 = xx * yz * yz, t4 = yy * xz * xz,
@@ -51,7 +50,7 @@ __device__ __forceinline__ uint8_t negdef_band(float xx, float xy, float xz, flo
   const float m3 = t1 + t2 - t3 - t4 - t5;
   const float s2 = fabsf(p1) + fabsf(p2);
   const float s3 = fabsf(t1) + fabsf(t2) + fabsf(t3) + fabsf(t4) + fabsf(t5);
-  if (fabsf(m2) < kBand * s2 || fabsf(m3) < kBand * s3) {
+  if (fabsf(m2) < band * s2 || fabsf(m3) < band * s3) {
     const double a = xx, b = xy, c = xz, d = yy, e = yz, f = zz;
     const double n2 = a * d - b * b;
     const double n3 = a * d * f + 2 * b * e * c - a * e * e - d * c * c - f * b * b;
@@ -77,7 +76,7 @@ template <typename TD, typename TS, bool BAND>
 __global__ __launch_bounds__(NT, 5) void k_hessian(const TD* __restrict__ dog, TS* __restrict__ hess,
                                                 uint8_t* __restrict__ pd, int W, int H, int D,
                                                 int nbx, int nby, float* __restrict__ cand,
-                                                uint32_t* __restrict__ cand_cnt) {
+                                                uint32_t* __restrict__ cand_cnt, float band) {
   constexpr int DX = TX + 4, DY = TY + 4;        // dog slice with halo 2
   constexpr int XX = TX + 2, XY = TY + 2;        // Hx slice with halo 1 in x and y
   constexpr int YX = TX, YY = TY + 2;            // Hy slice with halo 1 in y
@@ -285,7 +284,7 @@ __global__ __launch_bounds__(NT, 5) void k_hessian(const TD* __restrict__ dog, T
             hess[5 * N + idx] = cvt<TS>(zz);
           }
           const uint8_t m =
-              BAND ? negdef_band(xx, xy, xz, yy, yz, zz) : negdef32(xx, xy, xz, yy, yz, zz);
+              BAND ? negdef_band(xx, xy, xz, yy, yz, zz, band) : negdef32(xx, xy, xz, yy, yz, zz);
           if (pd) pd[idx] = m;
           // voxel 0 is background whatever its mask (its label is 0): no record
           is_cand = m != 0 && idx != 0 && rec != nullptr;
@@ -331,22 +330,22 @@ __global__ __launch_bounds__(NT, 5) void k_hessian(const TD* __restrict__ dog, T
 
 template <typename TD, typename TH>
 hipError_t launch_hessian(const TD* dog, TH* hess, uint8_t* pd, int W, int H, int D,
-                          hipStream_t s, float* cand, uint32_t* cand_cnt) {
+                          hipStream_t s, float* cand, uint32_t* cand_cnt, float band) {
   const int nbx = (W + TX - 1) / TX, nby = (H + TY - 1) / TY, nbz = (D + ZC - 1) / ZC;
   const i64 nblk = (i64)nbx * nby * nbz;
   // slices are read through buffer descriptors with 32-bit byte offsets
   if ((i64)W * H * (i64)sizeof(TD) >= 0xFFFFFFF0ll) return hipErrorInvalidValue;
   // reduced-precision storage anywhere on the path: FP64 re-evaluation band for the minors
-  constexpr bool band = !std::is_same<TD, float>::value || !std::is_same<TH, float>::value;
-  hipLaunchKernelGGL((k_hessian<TD, TH, band>), dim3((unsigned)nblk), dim3(NT), 0, s, dog, hess,
-                     pd, W, H, D, nbx, nby, cand, cand_cnt);
+  constexpr bool reduced = !std::is_same<TD, float>::value || !std::is_same<TH, float>::value;
+  hipLaunchKernelGGL((k_hessian<TD, TH, reduced>), dim3((unsigned)nblk), dim3(NT), 0, s, dog,
+                     hess, pd, W, H, D, nbx, nby, cand, cand_cnt, band);
   mark("hessian/hessian_pd", s);
   return hipGetLastError();
 }
 
 #define HDOG_INST_HESS(TD, TH)                                                          \
   template hipError_t launch_hessian<TD, TH>(const TD*, TH*, uint8_t*, int, int, int, \
-                                             hipStream_t, float*, uint32_t*);
+                                             hipStream_t, float*, uint32_t*, float);
 HDOG_INST_HESS(float, float)
 HDOG_INST_HESS(__half, __half)
 HDOG_INST_HESS(__hip_bfloat16, __hip_bfloat16)

@@ -234,7 +234,8 @@ class HDoG:
     # -- whole path ---------------------------------------------------------------
     @staticmethod
     def make_params(shape, radius_xy, radius_z, min_intensity, sigma_xy1, sigma_z1, sigma_xy2,
-                    sigma_z2, truncate=2.0, gamma=1.0, is_ave_mode=False, precision="fp32"):
+                    sigma_z2, truncate=2.0, gamma=1.0, is_ave_mode=False, precision="fp32",
+                    minor_band=0.0):
         D, H, W = shape
         p = C.Params()
         p.width, p.height, p.depth = W, H, D
@@ -245,6 +246,7 @@ class HDoG:
         p.gamma = float(gamma)
         p.is_ave_mode = int(bool(is_ave_mode))
         p.precision = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        p.minor_band = float(minor_band)
         return p
 
     def alloc_intermediates(self, shape, precision="fp32"):

@@ -41,7 +41,7 @@ class Params(ctypes.Structure):
     _fields_ = [("width", _i), ("height", _i), ("depth", _i), ("radius_xy", _i),
                 ("radius_z", _i), ("min_intensity", _f), ("sigma_xy1", _f), ("sigma_z1", _f),
                 ("sigma_xy2", _f), ("sigma_z2", _f), ("truncate", _d), ("gamma", _f),
-                ("is_ave_mode", _i), ("precision", _i)]
+                ("is_ave_mode", _i), ("precision", _i), ("minor_band", _f)]
 
 
 class Tile(ctypes.Structure):

@@ -76,6 +76,11 @@ typedef struct hdog_params {
   float gamma;               /* gamma_n */
   int is_ave_mode;           /* top-level is_ave_mode: sum instead of max of normalized */
   int precision;             /* enum hdog_precision */
+  /* Reduced-precision forms only: a leading minor whose magnitude is below minor_band times
+   * the sum of its terms' magnitudes has its sign re-decided in FP64.  0 selects 1e-5,
+   * which covers every sign the FP32 evaluation can flip; the precision-form config quotes
+   * 1e-3.  Ignored by HDOG_FP32. */
+  float minor_band;
 } hdog_params;
 
 /* ---- context --------------------------------------------------------------- */

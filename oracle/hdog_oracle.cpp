@@ -444,4 +444,161 @@ i64 oracle_substack(const uint16_t* img, int W, int H, int D,
                         out_norm, out_isum, out_imax);
 }
 
+// ---------------------------------------------------------------------------
+// FP64 restatement of the filter stages: the reference for the precision forms (FP16 /
+// BF16 storage), against which their per-stage error and candidate agreement are stated.
+// The same stage semantics as above in double throughout: box means of the (exact,
+// integer) min / max filters with double sums and 1/(2r+1) in double; the divide in
+// double; Gaussian weights exp(-0.5 i^2 / sigma^2) normalised in double (the recipe of
+// src/3D/DoG3DFilter.cu:372-392 without its float roundings); the separable Gaussian as
+// a double fma chain with zero padding; the DoG normalizer pow(sxy1, 2 gamma) *
+// pow(sz1, gamma) in double; central differences and leading minors in double.  Sigmas
+// and radii are those of the float parameters, so the supports match the FP32 path.
+
+static void box3d_f64(const uint16_t* in, double* out, int W, int H, int D, int rxy, int rz) {
+  Geo g(W, H, D);
+  std::vector<double> a(g.N), b(g.N);
+  for (i64 i = 0; i < g.N; ++i) a[i] = (double)in[i];
+  for (int axis = 0; axis < 3; ++axis) {
+    const int r = axis == 2 ? rz : rxy;
+    const double k = 1.0 / (double)(2 * r + 1);
+    const i64 L = g.len(axis), st = g.stride(axis), nl = g.nlines(axis);
+#pragma omp parallel for schedule(static)
+    for (i64 l = 0; l < nl; ++l) {
+      const i64 b0 = g.base(axis, l);
+      for (i64 c = 0; c < L; ++c) {
+        double val = 0.0;
+        for (int j = 0; j <= 2 * r; ++j) val += a[b0 + clampi(c - r + j, 0, L - 1) * st];
+        b[b0 + c * st] = val * k;
+      }
+    }
+    a.swap(b);
+  }
+  std::memcpy(out, a.data(), g.N * sizeof(double));
+}
+
+void oracle64_normalize3d(const uint16_t* img, double* norm, double min_intensity, int W, int H,
+                          int D, int rxy, int rz) {
+  Geo g(W, H, D);
+  if (rxy == 0 || rz == 0) {
+#pragma omp parallel for schedule(static)
+    for (i64 i = 0; i < g.N; ++i) {
+      const double v = (double)img[i];
+      norm[i] = (v >= min_intensity) ? v : 0.0;
+    }
+    return;
+  }
+  std::vector<uint16_t> t(g.N);
+  std::vector<double> E(g.N), Dl(g.N);
+  oracle_minmax3d(img, t.data(), W, H, D, rxy, rz, 1);
+  box3d_f64(t.data(), E.data(), W, H, D, rxy, rz);
+  oracle_minmax3d(img, t.data(), W, H, D, rxy, rz, 0);
+  box3d_f64(t.data(), Dl.data(), W, H, D, rxy, rz);
+#pragma omp parallel for schedule(static)
+  for (i64 i = 0; i < g.N; ++i) {
+    const double v = (double)img[i];
+    norm[i] = (v >= min_intensity) ? (v - E[i]) / (Dl[i] - E[i]) : 0.0;
+  }
+}
+
+int oracle64_gaussian_weights(float sigma, double truncate, double* w) {
+  const int lw = oracle_gaussian_radius(sigma, truncate);
+  const double s2 = (double)sigma * (double)sigma;
+  double sum = 1.0;
+  w[lw] = 1.0;
+  for (int i = 1; i <= lw; ++i) {
+    const double p = std::exp(-0.5 * i * i / s2);
+    w[lw - i] = p;
+    w[lw + i] = p;
+    sum += p * 2;
+  }
+  for (int i = 0; i < 2 * lw + 1; ++i) w[i] /= sum;
+  return lw;
+}
+
+void oracle64_gaussian3d(const double* in, double* out, int W, int H, int D, const double* wxy,
+                         int rxy, const double* wz, int rz) {
+  Geo g(W, H, D);
+  std::vector<double> a(in, in + g.N), b(g.N);
+  for (int axis = 0; axis < 3; ++axis) {
+    const int r = axis == 2 ? rz : rxy;
+    const double* w = axis == 2 ? wz : wxy;
+    const i64 L = g.len(axis), st = g.stride(axis), nl = g.nlines(axis);
+#pragma omp parallel for schedule(static)
+    for (i64 l = 0; l < nl; ++l) {
+      const i64 b0 = g.base(axis, l);
+      for (i64 c = 0; c < L; ++c) {
+        double sum = 0.0;
+        for (int j = -r; j <= r; ++j) {
+          const i64 p = c + j;
+          const double v = (p >= 0 && p < L) ? a[b0 + p * st] : 0.0;
+          sum = std::fma(w[r - j], v, sum);
+        }
+        b[b0 + c * st] = sum;
+      }
+    }
+    a.swap(b);
+  }
+  std::memcpy(out, a.data(), g.N * sizeof(double));
+}
+
+int oracle64_dog3d(const double* norm, double* dog, int W, int H, int D, float sxy1, float sz1,
+                   float sxy2, float sz2, double truncate, float gamma) {
+  Geo g(W, H, D);
+  std::vector<double> w0(2 * oracle_gaussian_radius(sxy1, truncate) + 1),
+      w1(2 * oracle_gaussian_radius(sz1, truncate) + 1),
+      w2(2 * oracle_gaussian_radius(sxy2, truncate) + 1),
+      w3(2 * oracle_gaussian_radius(sz2, truncate) + 1);
+  const int r0 = oracle64_gaussian_weights(sxy1, truncate, w0.data());
+  const int r1 = oracle64_gaussian_weights(sz1, truncate, w1.data());
+  const int r2 = oracle64_gaussian_weights(sxy2, truncate, w2.data());
+  const int r3 = oracle64_gaussian_weights(sz2, truncate, w3.data());
+  std::vector<double> g2(g.N);
+  oracle64_gaussian3d(norm, g2.data(), W, H, D, w2.data(), r2, w3.data(), r3);
+  oracle64_gaussian3d(norm, dog, W, H, D, w0.data(), r0, w1.data(), r1);
+  const double nz = std::pow((double)sxy1, (double)gamma * 2) * std::pow((double)sz1, (double)gamma);
+#pragma omp parallel for schedule(static)
+  for (i64 i = 0; i < g.N; ++i) dog[i] = (dog[i] - g2[i]) * nz;
+  return r3;
+}
+
+static void gradient_axis_f64(const double* in, double* out, const Geo& g, int axis) {
+  const i64 L = g.len(axis), st = g.stride(axis), nl = g.nlines(axis);
+#pragma omp parallel for schedule(static)
+  for (i64 l = 0; l < nl; ++l) {
+    const i64 b0 = g.base(axis, l);
+    for (i64 c = 0; c < L; ++c) {
+      const double hi = (c + 1 < L) ? in[b0 + (c + 1) * st] : 0.0;
+      const double lo = (c - 1 >= 0) ? in[b0 + (c - 1) * st] : 0.0;
+      out[b0 + c * st] = (hi - lo) * 0.5;
+    }
+  }
+}
+
+uint8_t oracle64_negdef(double xx, double xy, double xz, double yy, double yz, double zz) {
+  const double m2 = xx * yy - xy * xy;
+  const double m3 = xx * yy * zz + 2 * xy * yz * xz - xx * yz * yz - yy * xz * xz - zz * xy * xy;
+  return (xx < 0 && m2 > 0 && m3 < 0) ? 1 : 0;
+}
+
+void oracle64_hessian3d(const double* dog, double* hess, uint8_t* pd, int W, int H, int D) {
+  Geo g(W, H, D);
+  const i64 N = g.N;
+  double *Hxx = hess, *Hxy = hess + N, *Hxz = hess + 2 * N, *Hyy = hess + 3 * N,
+         *Hyz = hess + 4 * N, *Hzz = hess + 5 * N;
+  std::vector<double> t(N);
+  gradient_axis_f64(dog, t.data(), g, 0);
+  gradient_axis_f64(t.data(), Hxx, g, 0);
+  gradient_axis_f64(t.data(), Hxy, g, 1);
+  gradient_axis_f64(t.data(), Hxz, g, 2);
+  gradient_axis_f64(dog, t.data(), g, 1);
+  gradient_axis_f64(t.data(), Hyy, g, 1);
+  gradient_axis_f64(t.data(), Hyz, g, 2);
+  gradient_axis_f64(dog, t.data(), g, 2);
+  gradient_axis_f64(t.data(), Hzz, g, 2);
+#pragma omp parallel for schedule(static)
+  for (i64 i = 0; i < N; ++i)
+    pd[i] = oracle64_negdef(Hxx[i], Hxy[i], Hxz[i], Hyy[i], Hyz[i], Hzz[i]);
+}
+
 }  // extern "C"

@@ -22,6 +22,7 @@ _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 _u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
 _i64p = np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 _i = ctypes.c_int
 _f = ctypes.c_float
 _d = ctypes.c_double
@@ -62,6 +63,15 @@ def lib():
                                      _u32p, _u16p, _u64p, _u64p, _f32p, _f32p, _f32p,
                                      _f32p, _u64p, _u16p]
         L.oracle_regions.restype = _l
+        L.oracle64_normalize3d.argtypes = [_u16p, _f64p, _d, _i, _i, _i, _i, _i]
+        L.oracle64_gaussian_weights.argtypes = [_f, _d, _f64p]
+        L.oracle64_gaussian_weights.restype = _i
+        L.oracle64_gaussian3d.argtypes = [_f64p, _f64p, _i, _i, _i, _f64p, _i, _f64p, _i]
+        L.oracle64_dog3d.argtypes = [_f64p, _f64p, _i, _i, _i, _f, _f, _f, _f, _d, _f]
+        L.oracle64_dog3d.restype = _i
+        L.oracle64_negdef.argtypes = [_d] * 6
+        L.oracle64_negdef.restype = ctypes.c_uint8
+        L.oracle64_hessian3d.argtypes = [_f64p, _f64p, _u8p, _i, _i, _i]
         _lib = L
     return _lib
 
@@ -193,3 +203,85 @@ def substack(img, p):
     labels = ccl3d(pd)
     reg = regions(labels, hess, norm, img, p.get("is_ave_mode", False))
     return dict(norm=norm, dog=dog, hess=hess, pd=pd, labels=labels, regions=reg, rz2=rz2)
+
+
+# ---- FP64 restatement (the reference of the reduced-precision storage forms) ----------
+def normalize3d_f64(img, min_intensity, rxy, rz):
+    img = np.ascontiguousarray(img, np.uint16)
+    norm = np.empty(img.shape, np.float64)
+    lib().oracle64_normalize3d(img, norm, float(min_intensity), *_shape(img), rxy, rz)
+    return norm
+
+
+def gaussian_weights_f64(sigma, truncate=2.0):
+    r = gaussian_radius(sigma, truncate)
+    w = np.zeros(2 * r + 1, np.float64)
+    lib().oracle64_gaussian_weights(float(sigma), float(truncate), w)
+    return w
+
+
+def gaussian3d_f64(vol, wxy, wz):
+    vol = np.ascontiguousarray(vol, np.float64)
+    out = np.empty_like(vol)
+    wxy = np.ascontiguousarray(wxy, np.float64)
+    wz = np.ascontiguousarray(wz, np.float64)
+    lib().oracle64_gaussian3d(vol, out, *_shape(vol), wxy, len(wxy) // 2, wz, len(wz) // 2)
+    return out
+
+
+def dog3d_f64(norm, sigma_xy1, sigma_z1, sigma_xy2, sigma_z2, truncate=2.0, gamma=1.0):
+    norm = np.ascontiguousarray(norm, np.float64)
+    out = np.empty_like(norm)
+    rz2 = lib().oracle64_dog3d(norm, out, *_shape(norm), sigma_xy1, sigma_z1, sigma_xy2,
+                               sigma_z2, float(truncate), float(gamma))
+    return out, rz2
+
+
+def negdef_f64(xx, xy, xz, yy, yz, zz):
+    return lib().oracle64_negdef(xx, xy, xz, yy, yz, zz)
+
+
+def hessian3d_f64(dog):
+    dog = np.ascontiguousarray(dog, np.float64)
+    hess = np.empty((6,) + dog.shape, np.float64)
+    pd = np.empty(dog.shape, np.uint8)
+    lib().oracle64_hessian3d(dog, hess, pd, *_shape(dog))
+    return hess, pd
+
+
+def regions_f64(labels, hess, norm):
+    """Per-region features of an FP64 result, in float64 (numpy): label, voxel count,
+    centroid (sum / count), Hessian sums, structureness and blobness from the sorted
+    eigenvalues of the summed Hessian, max of norm.  Region 0 (background) excluded."""
+    flat = np.ascontiguousarray(labels).ravel()
+    on = np.nonzero(flat)[0]
+    lab = flat[on]
+    uniq, inv = np.unique(lab, return_inverse=True)
+    R = len(uniq)
+    cnt = np.bincount(inv, minlength=R).astype(np.float64)
+    D, H, W = labels.shape
+    x, y, z = on % W, (on // W) % H, on // (W * H)
+    cen = np.stack([np.bincount(inv, weights=c.astype(np.float64), minlength=R) / cnt
+                    for c in (x, y, z)], axis=1)
+    hs = np.stack([np.bincount(inv, weights=hess[k].ravel()[on], minlength=R) for k in range(6)],
+                  axis=1)
+    M = np.stack([hs[:, [0, 1, 2]], hs[:, [1, 3, 4]], hs[:, [2, 4, 5]]], axis=1)
+    ev = np.linalg.eigvalsh(M)
+    st = (ev ** 2).sum(axis=1)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        bl = ev[:, 2] ** 2 / (ev[:, 0] * ev[:, 1])
+    nmax = np.full(R, -np.inf)
+    np.maximum.at(nmax, inv, np.ascontiguousarray(norm).ravel()[on])
+    return dict(label=uniq.astype(np.uint32), count=cnt, centroid=cen, hsum=hs,
+                structureness=st, blobness=bl, norm=nmax)
+
+
+def substack_f64(img, p):
+    """Filter stages in FP64, then labels and region features of the FP64 candidates."""
+    norm = normalize3d_f64(img, p["min_intensity"], p["rxy"], p["rz"])
+    dog, _ = dog3d_f64(norm, p["sigma_xy1"], p["sigma_z1"], p["sigma_xy2"], p["sigma_z2"],
+                       p.get("truncate", 2.0), p.get("gamma", 1.0))
+    hess, pd = hessian3d_f64(dog)
+    labels = ccl3d(pd)
+    return dict(norm=norm, dog=dog, hess=hess, pd=pd, labels=labels,
+                regions=regions_f64(labels, hess, norm))

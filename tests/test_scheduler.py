@@ -126,9 +126,14 @@ def test_failed_run_gives_exit_1(tmp_path):
 
 
 @pytest.mark.gpu
-def test_fan_out_results_equal_a_direct_run(tmp_path):
+def test_fan_out_results_equal_oracle_records(tmp_path):
+    # every stack's result file holds the records the oracle gives for it: the substack
+    # loop, margins and record filter of src/3D/HDoG3D_main.cpp:294-315 and
+    # src/3D/utils.h:204-250 applied to the oracle's regions of each substack
+    import numpy as np
     from hdog import synth
-    from test_host_program import make_param
+    from test_host_program import DT_LOCAL, expected_records, make_param
+    import oracle
     exe = os.path.join(HOST, "HDoG3D_main")
     if not os.path.exists(exe):
         pytest.skip("host program not built")
@@ -140,24 +145,28 @@ def test_fan_out_results_equal_a_direct_run(tmp_path):
         os.makedirs(d)
         for z in range(L):
             vol[z].astype("<u2").tofile(os.path.join(d, "{:06d}.bin".format(z)))
-        vols[(y, x)] = d
+        vols[(y, x)] = vol
     base = json.load(open(make_param(tmp_path, [], depth=24, W=56, H=40)))
     base.update(src_basedir=src, dst_basedir=os.path.join(str(tmp_path), "fan"), gpus=[0])
     pfile = os.path.join(str(tmp_path), "fan.json")
     json.dump(base, open(pfile, "w"))
     assert hdog_gpu.main([pfile, "--exec", exe]) == 0
-    # direct run of the same stacks
-    direct = dict(base, stacks=[])
-    os.makedirs(os.path.join(str(tmp_path), "direct"))
-    for (y, x), d in vols.items():
-        direct["stacks"].append({
-            "src_paths": [os.path.join(d, f) for f in sorted(os.listdir(d))],
-            "dst_path": os.path.join(str(tmp_path), "direct", "{}_{}.bin".format(y, x))})
-    dfile = os.path.join(str(tmp_path), "direct.json")
-    json.dump(direct, open(dfile, "w"))
-    r = subprocess.run([exe, dfile], capture_output=True, text=True, timeout=300)
-    assert r.returncode == 0, r.stdout + r.stderr
-    for (y, x) in vols:
-        a = open(os.path.join(str(tmp_path), "fan", "{}_{}.bin".format(y, x)), "rb").read()
-        b = open(os.path.join(str(tmp_path), "direct", "{}_{}.bin".format(y, x)), "rb").read()
-        assert a == b and len(a) > 0
+    hp = base["HDoG_param"]
+    sd = hp["sigma_dog"]
+    p = dict(min_intensity=hp["min_intensity"], rxy=hp["radius_norm"]["xy_large"],
+             rz=hp["radius_norm"]["z_large"], sigma_xy1=sd["xy_small"], sigma_z1=sd["z_small"],
+             sigma_xy2=sd["xy_large"], sigma_z2=sd["z_large"], truncate=2.0, gamma=hp["gamma_n"])
+    ii = base["input_image_info"]
+    P = dict(depth=hp["depth"],
+             margin=oracle.gaussian_radius(sd["z_large"], 2.0) + hp["extra_depth_margin"],
+             left=ii["left_margin"], right=ii["right_margin"], top=ii["top_margin"],
+             bottom=ii["bottom_margin"], min_size=hp["min_size"], p=p)
+    for (y, x), vol in vols.items():
+        got = np.fromfile(os.path.join(str(tmp_path), "fan", "{}_{}.bin".format(y, x)),
+                          dtype=DT_LOCAL)
+        exp = expected_records(vol, P)
+        assert len(got) == len(exp) and len(exp) > 0, (y, x, len(got), len(exp))
+        for k in ("local_x", "local_y", "local_z", "intensity", "size", "padding"):
+            np.testing.assert_array_equal(got[k], exp[k], err_msg=k)
+        for k in ("structureness", "blobness"):
+            np.testing.assert_allclose(got[k], exp[k], rtol=1e-5, err_msg=k)
