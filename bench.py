@@ -346,6 +346,48 @@ def run_batched(hd, forms, n_tiles, steps):
     return out
 
 
+def run_soft(hd, steps):
+    """The differentiable form's step on its workload (128 x 512 x 512, config-0 noise,
+    12000 blobs): normalization -> DoG -> Hessian -> soft blob loss, forward and backward
+    (gradients for the input, both sigma triples and tau)."""
+    import torch
+    from hdog import soft, synth
+    cfg = synth.WORKLOADS["c4"]
+    shape = cfg["shape"]
+    blobs = synth.blob_table(shape, cfg["n_blobs"], cfg["seed"], cfg["sigma"], cfg["zfactor"],
+                             cfg["amp"])
+    v = hd.synth_volume(shape, blobs, cfg["seed"], cfg["mean0"], cfg["mean1"], cfg["sd"]).float()
+    t = torch.zeros(shape, dtype=torch.uint8, device=v.device)
+    c = torch.from_numpy(blobs[:, :3]).round().long().to(v.device)
+    t[c[:, 0], c[:, 1], c[:, 2]] = 1
+    model = soft.BlobResponseLoss(hd, [2.0, 2.0, 2.0], [3.2, 3.2, 3.2], 3e-8,
+                                  normalization=(0.0, 10, 10))
+    vg = v.requires_grad_()
+
+    def step():
+        model.zero_grad()
+        vg.grad = None
+        loss = model(vg, t)
+        loss.backward()
+        return loss
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    N = shape[0] * shape[1] * shape[2]
+    log(f"[differentiable] {shape}: forward + backward {ms:.1f} ms per step")
+    out = dict(shape_zyx=list(shape), ms_per_step=ms, gvoxel_per_s=N / (ms * 1e-3) / 1e9,
+               step="normalize -> DoG -> Hessian -> soft blob loss, forward and backward "
+                    "(input, sigma1, sigma2, tau gradients); wall clock")
+    del vg, v, t, model
+    torch.cuda.empty_cache()
+    return out
+
+
 def _with_shape(params, shape):
     p = type(params).from_buffer_copy(params)
     p.depth, p.height, p.width = shape
@@ -370,6 +412,8 @@ def main():
     ap.add_argument("--batch-forms", default="fp32,fp16",
                     help="storage forms of the batched many-stack run; empty skips it")
     ap.add_argument("--batch-stacks", type=int, default=40)
+    ap.add_argument("--soft-steps", type=int, default=3,
+                    help="timed steps of the differentiable form; 0 skips it")
     ap.add_argument("--batch-steps", type=int, default=2)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device-to-device copy of MIB MiB (three launches), "
@@ -475,6 +519,8 @@ def main():
         out["cpu_baseline"] = cpu_baseline(img, args.cpu_depth, ch, cw)
     else:
         out["cpu_baseline"] = None
+    if args.soft_steps > 0:
+        out["differentiable_form"] = run_soft(hd, args.soft_steps)
     bforms = [f for f in args.batch_forms.split(",") if f]
     if bforms:
         img = res = r = None

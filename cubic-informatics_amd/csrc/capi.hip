@@ -957,6 +957,48 @@ int hdog_soft_backward(hdog_ctx* ctx, void* stream, const float* d_v, const floa
   return HDOG_OK;
 }
 
+int soft_norm_check(hdog_ctx* ctx, int width, int height, int depth, int rxy, int rz) {
+  int rc = check_shape(ctx, width, height, depth);
+  if (rc) return rc;
+  if (rxy < 1 || rz < 1 || rxy > kMaxNormRadius || rz > kMaxNormRadius)
+    return fail(ctx, HDOG_ERR_ARG, "normalization radii (%d, %d) outside [1, %d]", rxy, rz,
+                kMaxNormRadius);
+  return HDOG_OK;
+}
+
+int hdog_soft_normalize_forward(hdog_ctx* ctx, void* stream, const float* d_v, float* d_norm,
+                                float* d_ed, uint32_t* d_arg, float min_intensity, int width,
+                                int height, int depth, int radius_xy, int radius_z) {
+  if (!ctx) return HDOG_ERR_ARG;
+  int rc = soft_norm_check(ctx, width, height, depth, radius_xy, radius_z);
+  if (rc) return rc;
+  if (!d_v || !d_norm || !d_ed || !d_arg) return fail(ctx, HDOG_ERR_ARG, "NULL device pointer");
+  HIPCHK(hipSetDevice(ctx->device));
+  const i64 N = (i64)width * height * depth;
+  HIPCHK(ensure(ctx->tmp1, (size_t)N * 32));
+  HIPCHK(launch_soft_normalize(d_v, d_norm, (float2*)d_ed, d_arg, ctx->tmp1.p, min_intensity,
+                               width, height, depth, radius_xy, radius_z, (hipStream_t)stream));
+  return HDOG_OK;
+}
+
+int hdog_soft_normalize_backward(hdog_ctx* ctx, void* stream, const float* d_v, const float* d_ed,
+                                 const uint32_t* d_arg, const float* d_grad_norm,
+                                 float* d_grad_v, float min_intensity, int width, int height,
+                                 int depth, int radius_xy, int radius_z) {
+  if (!ctx) return HDOG_ERR_ARG;
+  int rc = soft_norm_check(ctx, width, height, depth, radius_xy, radius_z);
+  if (rc) return rc;
+  if (!d_v || !d_ed || !d_arg || !d_grad_norm || !d_grad_v)
+    return fail(ctx, HDOG_ERR_ARG, "NULL device pointer");
+  HIPCHK(hipSetDevice(ctx->device));
+  const i64 N = (i64)width * height * depth;
+  HIPCHK(ensure(ctx->tmp1, (size_t)N * 16));
+  HIPCHK(launch_soft_normalize_adjoint(d_v, (const float2*)d_ed, d_arg, d_grad_norm, d_grad_v,
+                                       ctx->tmp1.p, min_intensity, width, height, depth,
+                                       radius_xy, radius_z, (hipStream_t)stream));
+  return HDOG_OK;
+}
+
 int hdog_batch_release(hdog_ctx* ctx) {
   if (!ctx) return HDOG_ERR_ARG;
   if (ctx->batch) {

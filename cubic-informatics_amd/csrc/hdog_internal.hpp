@@ -275,6 +275,18 @@ hipError_t launch_soft_grad_h(float* hess, const uint8_t* mask, const uint8_t* t
                               double inv_tau, double inv_count, double* sums, hipStream_t s);
 hipError_t launch_hessian_adjoint(const float* gh, float* P, float* out, int W, int H, int D,
                                   hipStream_t s);
+// Differentiable form, local-contrast normalization of a float volume (clamped min / max
+// and box of radius (rxy, rxy, rz)): norm, the (E, D) pairs and the window argmin / argmax
+// (N each) for the backward pass; scratch 8 N words.
+hipError_t launch_soft_normalize(const float* v, float* norm, float2* ed, uint32_t* arg,
+                                 void* scratch, float min_int, int W, int H, int D, int rxy,
+                                 int rz, hipStream_t s);
+// Its adjoint: gv = dL/dv from gn = dL/dnorm (min / max by their subgradient); scratch 4 N
+// words.
+hipError_t launch_soft_normalize_adjoint(const float* v, const float2* ed, const uint32_t* arg,
+                                         const float* gn, float* gv, void* scratch,
+                                         float min_int, int W, int H, int D, int rxy, int rz,
+                                         hipStream_t s);
 hipError_t launch_dot_pair(const float* a, const float2* pair, i64 N, double* out, hipStream_t s);
 hipError_t launch_dot(const float* a, const float* b, i64 N, double* out, hipStream_t s);
 hipError_t launch_synth(uint16_t* img, float* scratch, int W, int H, int D, u64 seed,

@@ -16,6 +16,20 @@
 //   dL/dv   = s * (G_s1 - G_s2) dL/ddog (a symmetric zero-padded Gaussian is self-adjoint)
 //   dL/dsigma: one separable pass with the weight derivative along one axis, dotted with
 //             dL/ddog, plus the normalizer term (gamma / sigma_a1) <dL/ddog, dog>.
+//
+// Local-contrast normalization in front of the DoG (src/3D/NormalizeFilter.cu:53-654 on a
+// float volume v): E = box(min3(v)), D = box(max3(v)) with clamp-to-edge windows of
+// radius (rxy, rxy, rz) for both the min / max and the box, n = v >= m ? (v - E)/(D - E)
+// : 0.  Its adjoint, given dL/dn:
+//   dL/dv  = [v >= m] dL/dn / (D - E)                          (direct term)
+//          + route(argmin, box^T(dL/dE)) + route(argmax, box^T(dL/dD))
+//   dL/dE  = [v >= m] dL/dn (v - D) / (D - E)^2,  dL/dD = -[v >= m] dL/dn (v - E) / (D - E)^2
+// box^T is the transpose of the clamped box mean (the clamped border positions collect
+// the weight of every window tap that clamps onto them); the min filter is taken by its
+// subgradient: each output's gradient goes to the voxel that attains the window minimum
+// (the lowest linear index among equal minima), likewise for the max.  The argmin / argmax
+// of the 3-D window are found by three separable passes over (value, index) pairs, which
+// is exact because the order "smaller value, then smaller index" is total.
 #include "hdog_internal.hpp"
 
 namespace hdog {
@@ -156,13 +170,212 @@ __global__ __launch_bounds__(NT) void k_dot(const float* __restrict__ a,
   if ((threadIdx.x & 63) == 0) unsafeAtomicAdd(out, s);
 }
 
+__global__ __launch_bounds__(NT) void k_pack_pair(const float* __restrict__ a,
+                                                  const float* __restrict__ b,
+                                                  float2* __restrict__ out, i64 N) {
+  for (i64 i = (i64)blockIdx.x * NT + threadIdx.x; i < N; i += (i64)gridDim.x * NT)
+    out[i] = make_float2(a[i], b[i]);
+}
+
 __global__ void k_zero_f64(double* p, int n) {
   if ((int)threadIdx.x < n) p[threadIdx.x] = 0.0;
 }
 
 unsigned grid_for(i64 N) { return (unsigned)std::min<i64>((N + NT - 1) / NT, 1 << 16); }
 
+struct Line {  // the line through voxel i along one axis
+  i64 base, stride;
+  int c, L;
+};
+__device__ __forceinline__ Line line_of(i64 i, int axis, int W, int H, int D) {
+  const i64 WH = (i64)W * H;
+  const int x = (int)(i % W), y = (int)((i / W) % H), z = (int)(i / WH);
+  Line l;
+  if (axis == 0) { l.c = x; l.L = W; l.stride = 1; }
+  else if (axis == 1) { l.c = y; l.L = H; l.stride = W; }
+  else { l.c = z; l.L = D; l.stride = WH; }
+  l.base = i - (i64)l.c * l.stride;
+  return l;
+}
+
+// one separable pass of the windowed (value, index) minimum and maximum, clamp-to-edge;
+// first pass: indices are the voxels' own (imn == NULL)
+__global__ __launch_bounds__(NT) void k_soft_argminmax(const float* __restrict__ vmn,
+                                                       const uint32_t* __restrict__ imn,
+                                                       const float* __restrict__ vmx,
+                                                       const uint32_t* __restrict__ imx,
+                                                       float* __restrict__ omn, uint32_t* __restrict__ oimn,
+                                                       float* __restrict__ omx, uint32_t* __restrict__ oimx,
+                                                       int W, int H, int D, int axis, int r) {
+  const i64 N = (i64)W * H * D;
+  for (i64 i = (i64)blockIdx.x * NT + threadIdx.x; i < N; i += (i64)gridDim.x * NT) {
+    const Line l = line_of(i, axis, W, H, D);
+    float bmn = INFINITY, bmx = -INFINITY;
+    uint32_t jmn = 0xFFFFFFFFu, jmx = 0xFFFFFFFFu;
+    for (int t = -r; t <= r; ++t) {
+      const i64 p = l.base + (i64)clampi(l.c + t, 0, l.L - 1) * l.stride;
+      const float a = vmn[p], b = vmx[p];
+      const uint32_t ja = imn ? imn[p] : (uint32_t)p, jb = imx ? imx[p] : (uint32_t)p;
+      if (a < bmn || (a == bmn && ja < jmn)) { bmn = a; jmn = ja; }
+      if (b > bmx || (b == bmx && jb < jmx)) { bmx = b; jmx = jb; }
+    }
+    omn[i] = bmn; oimn[i] = jmn;
+    omx[i] = bmx; oimx[i] = jmx;
+  }
+}
+
+// clamped box mean of the (E, D) pair along one axis (FP32 sums in window order)
+__global__ __launch_bounds__(NT) void k_soft_box(const float2* __restrict__ in,
+                                                 float2* __restrict__ out, int W, int H, int D,
+                                                 int axis, int r) {
+  const i64 N = (i64)W * H * D;
+  const float k = 1.0f / (float)(2 * r + 1);
+  for (i64 i = (i64)blockIdx.x * NT + threadIdx.x; i < N; i += (i64)gridDim.x * NT) {
+    const Line l = line_of(i, axis, W, H, D);
+    float se = 0.0f, sd = 0.0f;
+    for (int t = -r; t <= r; ++t) {
+      const float2 v = in[l.base + (i64)clampi(l.c + t, 0, l.L - 1) * l.stride];
+      se += v.x;
+      sd += v.y;
+    }
+    out[i] = make_float2(se * k, sd * k);
+  }
+}
+
+// transpose of the clamped box mean along one axis: position p receives every window tap
+// that lands on it, border positions also the taps that clamp onto them
+__global__ __launch_bounds__(NT) void k_soft_box_adj(const float2* __restrict__ in,
+                                                     float2* __restrict__ out, int W, int H,
+                                                     int D, int axis, int r) {
+  const i64 N = (i64)W * H * D;
+  const float k = 1.0f / (float)(2 * r + 1);
+  for (i64 i = (i64)blockIdx.x * NT + threadIdx.x; i < N; i += (i64)gridDim.x * NT) {
+    const Line l = line_of(i, axis, W, H, D);
+    const int p = l.c, L = l.L;
+    float se = 0.0f, sd = 0.0f;
+    for (int c = max(0, p - r); c <= min(L - 1, p + r); ++c) {
+      const float2 v = in[l.base + (i64)c * l.stride];
+      se += v.x;
+      sd += v.y;
+    }
+    if (p == 0)  // taps c - r + j < 0 of the windows of c < r clamp onto 0
+      for (int c = 0; c <= min(L - 1, r - 1); ++c) {
+        const float2 v = in[l.base + (i64)c * l.stride];
+        se += v.x * (float)(r - c);
+        sd += v.y * (float)(r - c);
+      }
+    if (p == L - 1)  // taps c - r + j > L - 1 of the windows of c > L - 1 - r clamp onto L - 1
+      for (int c = max(0, L - r); c <= L - 1; ++c) {
+        const float2 v = in[l.base + (i64)c * l.stride];
+        se += v.x * (float)(c + r - L + 1);
+        sd += v.y * (float)(c + r - L + 1);
+      }
+    out[i] = make_float2(se * k, sd * k);
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_soft_norm(const float* __restrict__ v,
+                                                  const float2* __restrict__ ed,
+                                                  float* __restrict__ norm, float min_int, i64 N) {
+  for (i64 i = (i64)blockIdx.x * NT + threadIdx.x; i < N; i += (i64)gridDim.x * NT) {
+    const float I = v[i];
+    const float2 e = ed[i];
+    norm[i] = I >= min_int ? (I - e.x) / (e.y - e.x) : 0.0f;
+  }
+}
+
+// direct term into gv; (dL/dE, dL/dD) into g_ed
+__global__ __launch_bounds__(NT) void k_soft_norm_adj(const float* __restrict__ v,
+                                                      const float2* __restrict__ ed,
+                                                      const float* __restrict__ gn,
+                                                      float* __restrict__ gv,
+                                                      float2* __restrict__ g_ed, float min_int,
+                                                      i64 N) {
+  for (i64 i = (i64)blockIdx.x * NT + threadIdx.x; i < N; i += (i64)gridDim.x * NT) {
+    const float I = v[i];
+    const float2 e = ed[i];
+    const float g = I >= min_int ? gn[i] : 0.0f;
+    const float inv = 1.0f / (e.y - e.x);
+    gv[i] = g * inv;
+    g_ed[i] = make_float2(g * (I - e.y) * inv * inv, -g * (I - e.x) * inv * inv);
+  }
+}
+
+// subgradient routing: gv[argmin(i)] += dL/dmin(i), gv[argmax(i)] += dL/dmax(i)
+__global__ __launch_bounds__(NT) void k_soft_route(const float2* __restrict__ g,
+                                                   const uint32_t* __restrict__ amn,
+                                                   const uint32_t* __restrict__ amx,
+                                                   float* __restrict__ gv, i64 N) {
+  for (i64 i = (i64)blockIdx.x * NT + threadIdx.x; i < N; i += (i64)gridDim.x * NT) {
+    const float2 q = g[i];
+    if (q.x != 0.0f) atomicAdd(gv + amn[i], q.x);
+    if (q.y != 0.0f) atomicAdd(gv + amx[i], q.y);
+  }
+}
+
 }  // namespace
+
+// norm, and for the backward pass ed (E, D pairs) and arg (argmin N, then argmax N);
+// scratch: 4 N words + 4 N words
+hipError_t launch_soft_normalize(const float* v, float* norm, float2* ed, uint32_t* arg,
+                                 void* scratch, float min_int, int W, int H, int D, int rxy,
+                                 int rz, hipStream_t s) {
+  const i64 N = (i64)W * H * D;
+  float* a = (float*)scratch;  // ping-pong (value, index) min / max planes
+  float* fmn[2] = {a, a + 4 * N};
+  const unsigned g = grid_for(N);
+  const int rad[3] = {rxy, rxy, rz};
+  for (int axis = 0; axis < 3; ++axis) {
+    float* src = fmn[(axis + 1) & 1];
+    float* dst = fmn[axis & 1];
+    const float* vmn = axis ? src : v;
+    const float* vmx = axis ? src + 2 * N : v;
+    const uint32_t* imn = axis ? (const uint32_t*)(src + N) : nullptr;
+    const uint32_t* imx = axis ? (const uint32_t*)(src + 3 * N) : nullptr;
+    hipLaunchKernelGGL(k_soft_argminmax, dim3(g), dim3(NT), 0, s, vmn, imn, vmx, imx, dst,
+                       (uint32_t*)(dst + N), dst + 2 * N, (uint32_t*)(dst + 3 * N), W, H, D,
+                       axis, rad[axis]);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  const float* res = fmn[0];  // axis 2 wrote plane set 0
+  hipError_t e = hipMemcpyAsync(arg, res + N, N * 4, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(arg + N, res + 3 * N, N * 4, hipMemcpyDeviceToDevice, s);
+  if (e != hipSuccess) return e;
+  // (min, max) pairs -> box along x, y, z -> (E, D)
+  float2* p0 = (float2*)fmn[1];
+  float2* p1 = ed;
+  hipLaunchKernelGGL(k_pack_pair, dim3(g), dim3(NT), 0, s, res, res + 2 * N, p0, N);
+  float2* bufs[2] = {p0, p1};
+  for (int axis = 0; axis < 3; ++axis) {
+    // x: p0 -> ed, y: ed -> p0, z: p0 -> ed
+    hipLaunchKernelGGL(k_soft_box, dim3(g), dim3(NT), 0, s, bufs[axis & 1], bufs[(axis + 1) & 1],
+                       W, H, D, axis, rad[axis]);
+  }
+  hipLaunchKernelGGL(k_soft_norm, dim3(g), dim3(NT), 0, s, v, ed, norm, min_int, N);
+  return hipGetLastError();
+}
+
+hipError_t launch_soft_normalize_adjoint(const float* v, const float2* ed, const uint32_t* arg,
+                                         const float* gn, float* gv, void* scratch,
+                                         float min_int, int W, int H, int D, int rxy, int rz,
+                                         hipStream_t s) {
+  const i64 N = (i64)W * H * D;
+  float2* q0 = (float2*)scratch;
+  float2* q1 = q0 + N;
+  const unsigned g = grid_for(N);
+  const int rad[3] = {rxy, rxy, rz};
+  hipLaunchKernelGGL(k_soft_norm_adj, dim3(g), dim3(NT), 0, s, v, ed, gn, gv, q0, min_int, N);
+  // the box was applied x, y, z: its transpose runs z, y, x
+  float2* bufs[2] = {q0, q1};
+  for (int k = 0; k < 3; ++k) {
+    const int axis = 2 - k;
+    hipLaunchKernelGGL(k_soft_box_adj, dim3(g), dim3(NT), 0, s, bufs[k & 1], bufs[(k + 1) & 1], W,
+                       H, D, axis, rad[axis]);
+  }
+  hipLaunchKernelGGL(k_soft_route, dim3(g), dim3(NT), 0, s, bufs[1], arg, arg + N, gv, N);
+  return hipGetLastError();
+}
 
 hipError_t launch_soft_loss(const float* hess, const uint8_t* mask, const uint8_t* target, i64 N,
                             double inv_tau, double* sums, hipStream_t s) {

@@ -93,6 +93,10 @@ hdog_batch_release = _sig("hdog_batch_release", [_vp])
 hdog_synth_volume = _sig("hdog_synth_volume", [_vp, _vp, _vp, _vp, _i, _i, _i, ctypes.c_uint64,
                                                _d, _d, _d, _vp, _i64])
 
+hdog_soft_normalize_forward = _sig("hdog_soft_normalize_forward", [_vp, _vp, _vp, _vp, _vp, _vp,
+                                                                   _f, _i, _i, _i, _i, _i])
+hdog_soft_normalize_backward = _sig("hdog_soft_normalize_backward", [_vp, _vp, _vp, _vp, _vp, _vp,
+                                                                     _vp, _f, _i, _i, _i, _i, _i])
 hdog_set_profiling = _sig("hdog_set_profiling", [_vp, _i])
 hdog_stage_times = _sig("hdog_stage_times", [_vp, ctypes.POINTER(_d), ctypes.POINTER(ctypes.c_char_p), _i])
 
@@ -110,6 +114,7 @@ EXPORTED = ["hdog_create", "hdog_destroy", "hdog_last_error", "hdog_release_scra
             "hdog_ccl3d", "hdog_ccl3d_noalloc", "hdog_regional_features", "hdog_regions_download",
             "hdog_process_substack", "hdog_run_device", "hdog_synth_volume",
             "hdog_set_profiling", "hdog_stage_times", "hdog_soft_forward", "hdog_soft_backward",
+            "hdog_soft_normalize_forward", "hdog_soft_normalize_backward",
             "hdog_batch_plan", "hdog_batch_run", "hdog_batch_tile_info", "hdog_batch_download",
             "hdog_batch_release", "hdog_host_alloc", "hdog_host_free", "hdog_substack_upload",
             "hdog_substack_run"]

@@ -4,6 +4,14 @@
     H    = Hessian(dog)  (the detector's central differences),   M = candidate mask of H
     loss = mean over M of BCE(sigmoid(-det(H) / tau), target)
 
+optionally behind the detector's local-contrast normalization of the input (normalize):
+
+    n = v >= m ? (v - E) / (D - E) : 0,  E = box(min3(v)), D = box(max3(v))
+
+whose backward pass takes the min / max filters by their subgradient (the gradient of a
+window's extreme goes to the voxel that attains it) and the clamped box means by their
+transposes (hdog_soft_normalize_forward / _backward).
+
 Forward and backward run the hand-written HIP kernels of libhdog.so
 (hdog_soft_forward / hdog_soft_backward): the DoG volume is checkpointed in the forward
 pass and H is recomputed from it in the backward pass.  The mask M is treated as a
@@ -85,10 +93,51 @@ class _SoftFn(torch.autograd.Function):
             None, None, None, None, None
 
 
-class BlobResponseLoss(torch.nn.Module):
-    """Loss module with learnable DoG sigmas (z, y, x) and threshold tau."""
+class _NormFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, v, hd, min_intensity, rxy, rz):
+        v = v.contiguous()
+        D, H, W = v.shape
+        norm = torch.empty_like(v)
+        ed = torch.empty((D, H, W, 2), dtype=torch.float32, device=v.device)
+        arg = torch.empty((2, D, H, W), dtype=torch.int32, device=v.device)  # uint32 indices
+        hd._check(C.hdog_soft_normalize_forward(hd._h, hd._stream(), ctypes.c_void_p(v.data_ptr()),
+                                                ctypes.c_void_p(norm.data_ptr()),
+                                                ctypes.c_void_p(ed.data_ptr()),
+                                                ctypes.c_void_p(arg.data_ptr()),
+                                                float(min_intensity), W, H, D, int(rxy), int(rz)))
+        ctx.save_for_backward(v, ed, arg)
+        ctx.hd, ctx.args = hd, (float(min_intensity), int(rxy), int(rz))
+        return norm
 
-    def __init__(self, hd, sigma1_zyx, sigma2_zyx, tau, truncate=4.0, gamma=1.0):
+    @staticmethod
+    def backward(ctx, grad_norm):
+        v, ed, arg = ctx.saved_tensors
+        hd = ctx.hd
+        m, rxy, rz = ctx.args
+        D, H, W = v.shape
+        gn = grad_norm.contiguous().float()
+        gv = torch.empty_like(v)
+        hd._check(C.hdog_soft_normalize_backward(hd._h, hd._stream(), ctypes.c_void_p(v.data_ptr()),
+                                                 ctypes.c_void_p(ed.data_ptr()),
+                                                 ctypes.c_void_p(arg.data_ptr()),
+                                                 ctypes.c_void_p(gn.data_ptr()),
+                                                 ctypes.c_void_p(gv.data_ptr()), m, W, H, D,
+                                                 rxy, rz))
+        return gv, None, None, None, None
+
+
+def normalize(hd, v, min_intensity, radius_xy, radius_z):
+    """Differentiable local-contrast normalization of a float32 (D, H, W) device volume."""
+    return _NormFn.apply(v, hd, min_intensity, radius_xy, radius_z)
+
+
+class BlobResponseLoss(torch.nn.Module):
+    """Loss module with learnable DoG sigmas (z, y, x) and threshold tau; with
+    `normalization=(min_intensity, radius_xy, radius_z)` the input is normalized first."""
+
+    def __init__(self, hd, sigma1_zyx, sigma2_zyx, tau, truncate=4.0, gamma=1.0,
+                 normalization=None):
         super().__init__()
         self.hd = hd
         self.sigma1 = torch.nn.Parameter(torch.tensor(sigma1_zyx, dtype=torch.float64))
@@ -97,8 +146,11 @@ class BlobResponseLoss(torch.nn.Module):
         self.rad1 = radii(sigma1_zyx, truncate)
         self.rad2 = radii(sigma2_zyx, truncate)
         self.gamma = gamma
+        self.normalization = normalization
 
     def forward(self, v, target):
+        if self.normalization is not None:
+            v = normalize(self.hd, v, *self.normalization)
         return _SoftFn.apply(v, self.sigma1, self.sigma2, self.tau, target, self.hd, self.rad1,
                              self.rad2, self.gamma)
 

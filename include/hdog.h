@@ -267,6 +267,23 @@ int hdog_soft_backward(hdog_ctx* ctx, void* stream, const float* d_v, const floa
                        int64_t count, double grad_out, float* d_grad_v, double grad_sigma1[3],
                        double grad_sigma2[3], double* grad_tau);
 
+/* The local-contrast normalization in front of the differentiable blob response, on a
+ * float volume: E = box(min3(v)), D = box(max3(v)) (clamp-to-edge windows and box of radius
+ * (radius_xy, radius_xy, radius_z)), norm = v >= min_intensity ? (v - E)/(D - E) : 0
+ * (src/3D/NormalizeFilter.cu:53-654 with a float input).  The forward pass writes norm and,
+ * for the backward pass, d_ed (2N floats: (E, D) pairs) and d_arg (2N: each voxel's window
+ * argmin, then argmax, the lowest linear index among equal extremes).  The backward pass
+ * writes d_grad_v = dL/dv from d_grad_norm = dL/dnorm: the min and max filters by their
+ * subgradient (gradient routed to the argmin / argmax), the box means by their clamped
+ * transposes.  No counterpart in the reference. */
+int hdog_soft_normalize_forward(hdog_ctx* ctx, void* stream, const float* d_v, float* d_norm,
+                                float* d_ed, uint32_t* d_arg, float min_intensity, int width,
+                                int height, int depth, int radius_xy, int radius_z);
+int hdog_soft_normalize_backward(hdog_ctx* ctx, void* stream, const float* d_v, const float* d_ed,
+                                 const uint32_t* d_arg, const float* d_grad_norm,
+                                 float* d_grad_v, float min_intensity, int width, int height,
+                                 int depth, int radius_xy, int radius_z);
+
 /* ---- synthetic input (benchmark / tests) ---------------------------------- */
 /* Renders noise + Gaussian blobs into d_img (depth x height x width uint16):
  * background mean0 + (mean1 - mean0) * z / (depth-1), sd `sd` (counter-based normal

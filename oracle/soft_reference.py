@@ -23,16 +23,52 @@ def weights(sigma, r):
 
 
 def sep_gauss(v, sig_zyx, rad_zyx):
-    """Separable zero-padded Gaussian of a (D, H, W) float64 tensor."""
-    x = v[None, None]
+    """Separable zero-padded Gaussian of a (D, H, W) float64 tensor: one 1-D convolution per
+    axis over the lines along it (the weights are symmetric, so correlation = convolution)."""
+    x = v
     for axis, (s, r) in enumerate(zip(sig_zyx, rad_zyx)):
-        w = weights(s, r)
-        shape = [1, 1, 1, 1, 1]
-        shape[2 + axis] = 2 * r + 1
-        pad = [0, 0, 0]
-        pad[axis] = r
-        x = F.conv3d(x, w.reshape(shape), padding=tuple(pad))
-    return x[0, 0]
+        w = weights(s, r).to(v.device)
+        xl = x.movedim(axis, -1)
+        shp = xl.shape
+        y = F.conv1d(xl.reshape(-1, 1, shp[-1]), w.reshape(1, 1, -1), padding=r)
+        x = y.reshape(shp).movedim(-1, axis)
+    return x
+
+
+def _clamp_pad(x, axis, r):
+    pad = [0, 0, 0, 0, 0, 0]
+    pad[2 * (2 - axis)] = r
+    pad[2 * (2 - axis) + 1] = r
+    return F.pad(x[None, None], pad, mode="replicate")
+
+
+def _window_max(x, axis, r):
+    k = [1, 1, 1]
+    k[axis] = 2 * r + 1
+    return F.max_pool3d(_clamp_pad(x, axis, r), kernel_size=k, stride=1)[0, 0]
+
+
+def _box(x, axis, r):
+    k = [1, 1, 1]
+    k[axis] = 2 * r + 1
+    return F.avg_pool3d(_clamp_pad(x, axis, r), kernel_size=k, stride=1)[0, 0]
+
+
+def normalize(v, min_intensity, rxy, rz):
+    """Local-contrast normalization of a float64 volume, src/3D/NormalizeFilter.cu:53-654:
+    clamp-to-edge min / max filters and box means of radius (rz, rxy, rxy) (z, y, x),
+    n = v >= m ? (v - E) / (D - E) : 0.  Autograd takes each window extreme's gradient to
+    the voxel that attains it."""
+    rad = (rz, rxy, rxy)
+    mn, mx = v, v
+    for axis in (2, 1, 0):
+        mn = -_window_max(-mn, axis, rad[axis])
+        mx = _window_max(mx, axis, rad[axis])
+    E, Dl = mn, mx
+    for axis in (2, 1, 0):
+        E = _box(E, axis, rad[axis])
+        Dl = _box(Dl, axis, rad[axis])
+    return torch.where(v >= min_intensity, (v - E) / (Dl - E), torch.zeros_like(v))
 
 
 def cdiff(f, axis):
@@ -60,9 +96,9 @@ def forward(v, sigma1, sigma2, tau, target, rad1, rad2, gamma=1.0, mask=None):
     if mask is None:
         mask = ((xx < 0) & (xx * yy - xy * xy > 0) & (det < 0)).detach()
     u = -det / tau
-    t = target.to(torch.float64)
+    t = target.to(torch.float64).to(det.device)
     terms = F.softplus(u) - t * u
-    m = mask.bool()
+    m = mask.bool().to(det.device)
     loss = terms[m].mean() if m.any() else terms.sum() * 0.0
     return loss, mask, dog
 

@@ -22,7 +22,8 @@ def test_more_than_one_gpu_is_refused():
 def test_bench_line_on_a_thin_stack():
     # a 16-slice stack of the benchmark's plane size, one timed step, no CPU sample
     r = subprocess.run([sys.executable, BENCH, "--depth", "16", "--steps", "1", "--warmup", "0",
-                        "--cpu-depth", "0", "--forms", "fp32", "--batch-forms", ""],
+                        "--cpu-depth", "0", "--forms", "fp32", "--batch-forms", "",
+                        "--soft-steps", "0"],
                        capture_output=True, text=True, timeout=600, cwd=ROOT)
     assert r.returncode == 0, r.stderr[-2000:]
     lines = [l for l in r.stdout.splitlines() if l.startswith("{")]

@@ -79,3 +79,24 @@ def test_autograd_matches_central_finite_differences(which):
         assert abs(fd - g) <= 1e-3 * max(1e-3, abs(g)), (which, k, fd, g)
         rich = (4 * fd2 - fd) / 3
         assert abs(rich - g) <= 1e-6 * max(1.0, abs(g)), (which, k, rich, g)
+
+
+def test_reference_normalization_is_the_detectors():
+    # on integer inputs the float64 restatement equals the FP32 oracle's normalization to
+    # FP32 rounding (same clamp-to-edge min / max and box windows, same formula)
+    import oracle
+    rng = np.random.default_rng(12)
+    img = rng.integers(0, 4000, (12, 30, 26), dtype=np.uint16)
+    want = oracle.normalize3d(img, 150, 4, 2).astype(np.float64)
+    got = ref.normalize(torch.from_numpy(img.astype(np.float64)), 150.0, 4, 2).numpy()
+    np.testing.assert_allclose(got, want, rtol=2e-6, atol=2e-7)
+
+
+def test_reference_normalization_gradient_is_the_subgradient():
+    # continuous values (no ties in any window): the normalization is differentiable and
+    # autograd's gradient, which routes each window extreme to its voxel, matches central
+    # finite differences
+    rng = np.random.default_rng(13)
+    v = torch.from_numpy(rng.random((5, 7, 6)) * 100).requires_grad_()
+    assert torch.autograd.gradcheck(lambda x: ref.normalize(x, 10.0, 2, 1), (v,), eps=1e-6,
+                                    atol=1e-7, rtol=1e-5)
