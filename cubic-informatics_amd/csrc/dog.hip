@@ -211,9 +211,9 @@ __global__ __launch_bounds__(NT) void k_gauss_walk(const float2* __restrict__ sr
       for (int j = -R2; j <= R2; ++j) s2 = fmaf(w2[R2 - j], ring[(q + j + R) % WB].y, s2);
       const i64 idx = base + (i64)c * sa;
       if (LAST)
-        dog[idx] = cvt<TS>((s1 - s2) * nz);
+        st_nt(dog + idx, (s1 - s2) * nz);
       else
-        dst[idx] = make_float2(s1, s2);
+        st_nt(dst + idx, make_float2(s1, s2));
       ring[q] = pf[q % PF];  // position c + R + 1 replaces position c - R
       pf[q % PF] = at(c + R + 1 + PF);
     }
@@ -233,7 +233,7 @@ __global__ __launch_bounds__(64) void k_gauss_xy_walk(const TS* __restrict__ src
                                                       float2* __restrict__ dst,
                                                       const float* __restrict__ w1,
                                                       const float* __restrict__ w2, int W, int H,
-                                                      int nxs) {
+                                                      int nxs, int nys, int rows) {
   constexpr int R = R1 > R2 ? R1 : R2;
   constexpr int WB = 2 * R + 1;
   // rows fetched ahead: a divisor of WB, so the queue slot of every row is static in
@@ -242,7 +242,11 @@ __global__ __launch_bounds__(64) void k_gauss_xy_walk(const TS* __restrict__ src
   constexpr int HX = 16, LW = 128;  // line positions 96 .. 127 stay unused (zero)
   static_assert(R <= HX, "x window exceeds the LDS line");
   __shared__ float line[2][LW];
-  const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+  // work-group = one 64-column strip of one plane over the rows [y0, y1) of one of the nys
+  // segments of the plane (rows: a multiple of WB)
+  const unsigned t = xcd_tile(blockIdx.x, gridDim.x);
+  const int y0 = (int)(t % nys) * rows, y1 = min(H, y0 + rows);
+  const unsigned tile = t / nys;
   const int l = threadIdx.x;
   const int x0 = (int)(tile % nxs) * 64;
   const i64 pbase = (i64)(tile / nxs) * H * W;
@@ -272,8 +276,15 @@ __global__ __launch_bounds__(64) void k_gauss_xy_walk(const TS* __restrict__ src
     v2[i] = w2[R2 + i];
     asm volatile("" : "+v"(v2[i]));
   }
+  // row y waits in queue slot y % PF (y0 is a multiple of WB, so of PF): every slot index
+  // below is a compile-time constant
+  if (y0 == 0) {
 #pragma unroll
-  for (int k = 0; k < PF; ++k) fetch(k, k);
+    for (int k = 0; k < PF; ++k) fetch(k, k);
+  } else {
+#pragma unroll
+    for (int k = 0; k < PF; ++k) fetch(y0 - R + k, ((k - R) % PF + PF) % PF);
+  }
   // x outputs of row y, whose slot is k; fetches row y + PF into the freed slot.  Past
   // the plane the line holds zeros, so the outputs are +0 as the y pass's zero padding
   // requires.
@@ -298,24 +309,29 @@ __global__ __launch_bounds__(64) void k_gauss_xy_walk(const TS* __restrict__ src
   const bool out = x0 + l < W;
   float2* dcol = dst + pbase + x0 + l;
   float2 ring[WB];  // row p lives in ring[(p + R) % WB]; rows < 0 are zero
+  if (y0 == 0) {
 #pragma unroll
-  for (int q = 0; q < R; ++q) ring[q] = make_float2(0.0f, 0.0f);
+    for (int q = 0; q < R; ++q) ring[q] = make_float2(0.0f, 0.0f);
 #pragma unroll
-  for (int q = R; q < WB; ++q) ring[q] = xrow(q - R, (q - R) % PF);
-  // The plane's end leaves the kernel (a wave-uniform return) instead of skipping the
+    for (int q = R; q < WB; ++q) ring[q] = xrow(q - R, (q - R) % PF);
+  } else {  // a later segment starts from the R rows above it
+#pragma unroll
+    for (int q = 0; q < WB; ++q) ring[q] = xrow(y0 - R + q, ((q - R) % PF + PF) % PF);
+  }
+  // The segment's end leaves the kernel (a wave-uniform return) instead of skipping the
   // remaining unrolled rows: no path that skips a row's loads merges back into the loop,
   // so the compiler's memory-wait counts keep the PF rows in flight.
-  for (int c0 = 0;; c0 += WB) {
+  for (int c0 = y0;; c0 += WB) {
 #pragma unroll
     for (int q = 0; q < WB; ++q) {
       const int c = c0 + q;
-      if (c >= H) return;
+      if (c >= y1) return;
       float s1 = 0.0f, s2 = 0.0f;
 #pragma unroll
       for (int j = -R1; j <= R1; ++j) s1 = fmaf(v1[j < 0 ? -j : j], ring[(q + j + R) % WB].x, s1);
 #pragma unroll
       for (int j = -R2; j <= R2; ++j) s2 = fmaf(v2[j < 0 ? -j : j], ring[(q + j + R) % WB].y, s2);
-      if (out) dcol[(i64)c * W] = make_float2(s1, s2);
+      if (out) st_nt(dcol + (i64)c * W, make_float2(s1, s2));
       ring[q] = xrow(c + R + 1, (q + R + 1) % PF);  // row c + R + 1 replaces row c - R
     }
   }
@@ -356,9 +372,11 @@ bool gauss_xy(const TS* src, float2* dst, const float* w1, const float* w2, int 
 #define X(A, B)                                                                              \
   if (r1 == A && r2 == B) {                                                                  \
     const int nxs = (W + 63) / 64;                                                           \
-    const i64 nblk = (i64)nxs * D;                                                           \
+    int nys, rows;                                                                           \
+    walk_segments(H, 2 * (A > B ? A : B) + 1, &nys, &rows);                                  \
+    const i64 nblk = (i64)nxs * D * nys;                                                     \
     hipLaunchKernelGGL((k_gauss_xy_walk<A, B, TS>), dim3((unsigned)nblk), dim3(64), 0, s,    \
-                       src, dst, w1, w2, W, H, nxs);                                         \
+                       src, dst, w1, w2, W, H, nxs, nys, rows);                              \
     *e = hipGetLastError();                                                                  \
     return true;                                                                             \
   }

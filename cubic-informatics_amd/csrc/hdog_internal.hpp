@@ -99,6 +99,20 @@ __host__ __device__ __forceinline__ int clampi(int v, int lo, int hi) {
   return v < lo ? lo : (v > hi ? hi : v);
 }
 
+// ---- streaming (nontemporal) stores of volumes written once, read by a later kernel ----
+__device__ __forceinline__ void st_nt(float* p, float v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void st_nt(uint32_t* p, uint32_t v) { __builtin_nontemporal_store(v, p); }
+__device__ __forceinline__ void st_nt(float2* p, float2 v) {
+  __builtin_nontemporal_store(v.x, &p->x);
+  __builtin_nontemporal_store(v.y, &p->y);
+}
+__device__ __forceinline__ void st_nt(float4* p, float4 v) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  const f4 t = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(t, reinterpret_cast<f4*>(p));
+}
+template <typename T> __device__ __forceinline__ void st_nt(T* p, float v) { *p = cvt<T>(v); }
+
 // ---- tile staging -----------------------------------------------------------------
 // Copies n elements global -> LDS with B independent loads in flight per lane before the
 // first LDS store (a plain strided loop waits for each load in turn).
@@ -152,6 +166,16 @@ inline void mark(const char* name, hipStream_t s) {
 // walks one contiguous range of tiles in order: tiles that share cache lines (halo rows,
 // 128-byte lines split between narrow tiles) then meet in the same L2.  A bijection of
 // [0, n) for any n.
+// Row walkers (one wave per plane strip) split tall planes into two row segments, each
+// starting with the 2R rows of context its window needs: a C1 plane strip as one wave left
+// the last round of waves part-empty (2.5 and 1.4 rounds of resident waves for the two
+// walkers).  Segment length: a multiple of the ring period wb.
+inline void walk_segments(int H, int wb, int* nys, int* rows) {
+  *nys = H >= 16 * wb ? 2 : 1;
+  const int per = (H + *nys - 1) / *nys;
+  *rows = (per + wb - 1) / wb * wb;
+}
+
 __device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned n) {
   constexpr unsigned X = 8;
   const unsigned q = n / X, r = n % X, j = b % X;

@@ -323,10 +323,10 @@ __global__ __launch_bounds__(NT) void k_minmax_walk(const void* __restrict__ src
 #pragma unroll
     for (int j = 0; j < WB; ++j) {
       if (j > 0) {
-        if (c0 + j < L) dst[base + (i64)(c0 + j) * sa] = pkmin(S[j], p);
+        if (c0 + j < L) st_nt(dst + base + (i64)(c0 + j) * sa, pkmin(S[j], p));
         p = pkmin(p, cur[j]);
       } else if (c0 < L) {
-        dst[base + (i64)c0 * sa] = S[0];
+        st_nt(dst + base + (i64)c0 * sa, S[0]);
       }
     }
     S[WB - 1] = cur[WB - 1];
@@ -383,9 +383,9 @@ __global__ __launch_bounds__(NT) void k_box_walk(const float2* __restrict__ src,
       const i64 idx = base + (i64)c * sa;
       if (NORM) {
         const float I = iv[q];
-        norm[idx] = cvt<TS>((I >= min_int) ? (I - ve) / (vd - ve) : 0.0f);
+        st_nt(norm + idx, (I >= min_int) ? (I - ve) / (vd - ve) : 0.0f);
       } else {
-        dst[idx] = make_float2(ve, vd);
+        st_nt(dst + idx, make_float2(ve, vd));
       }
       // slot of position c - R is reused for position c + R + 1
       ring[q] = pf[q % PF];
@@ -411,7 +411,7 @@ __global__ __launch_bounds__(NT) void k_box_walk(const float2* __restrict__ src,
 template <int R>
 __global__ __launch_bounds__(64) void k_norm_xy_walk(const uint32_t* __restrict__ src,
                                                      float4* __restrict__ dst, int W, int H,
-                                                     int nxs, float k) {
+                                                     int nxs, int nys, int rows, float k) {
   constexpr int WB = 2 * R + 1;
   constexpr int CW = 128;                    // output columns per wave
   constexpr int HX = (2 * R + 3) / 4 * 4;    // line halo: 16-byte aligned task reads
@@ -424,7 +424,11 @@ __global__ __launch_bounds__(64) void k_norm_xy_walk(const uint32_t* __restrict_
   static_assert(4 * NTASK + 2 * R + 4 <= LW, "task reads stay inside the line");
   __shared__ __align__(16) uint32_t A[LW];
   __shared__ __align__(16) uint32_t B[4 * NTASK];
-  const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+  // work-group = one strip of one plane over the rows [y0, y1) of one of the nys segments
+  // of the plane (rows: a multiple of WB)
+  const unsigned t = xcd_tile(blockIdx.x, gridDim.x);
+  const int y0 = (int)(t % nys) * rows, y1 = min(H, y0 + rows);
+  const unsigned tile = t / nys;
   const int l = threadIdx.x;
   const int x0 = (int)(tile % nxs) * CW;
   const uint32_t* plane = src + (i64)(tile / nxs) * H * W;
@@ -437,8 +441,14 @@ __global__ __launch_bounds__(64) void k_norm_xy_walk(const uint32_t* __restrict_
 #pragma unroll
     for (int j = 0; j < NLD; ++j) qv[s][j] = row[xl[j]];
   };
+  // row y waits in queue slot y % PF (y0 is a multiple of WB, so of PF)
+  if (y0 == 0) {
 #pragma unroll
-  for (int s = 0; s < PF; ++s) fetch(s, s);
+    for (int s = 0; s < PF; ++s) fetch(s, s);
+  } else {
+#pragma unroll
+    for (int s = 0; s < PF; ++s) fetch(y0 - R + s, ((s - R) % PF + PF) % PF);
+  }
   const bool edge = x0 - R < 0 || x0 + CW + R > W;  // wave-uniform
   const int blo = R - x0, bhi = W - 1 - x0 + R;     // B index of x = 0 and x = W - 1
   const uint32_t full = (uint32_t)WB * 65535u;
@@ -505,16 +515,21 @@ __global__ __launch_bounds__(64) void k_norm_xy_walk(const uint32_t* __restrict_
   float4* dq = dst + ((tile / nxs) * (i64)H * W + xc) / 2;  // valid when st2
   float2* dp = reinterpret_cast<float2*>(dst) + (tile / nxs) * (i64)H * W + xc;
   float4 ring[WB];  // row p lives in ring[(p + R) % WB]; rows clamp to [0, H)
-  ring[R] = xrow(0, 0);
+  if (y0 == 0) {
+    ring[R] = xrow(0, 0);
 #pragma unroll
-  for (int q = 0; q < R; ++q) ring[q] = ring[R];
+    for (int q = 0; q < R; ++q) ring[q] = ring[R];
 #pragma unroll
-  for (int q = R + 1; q < WB; ++q) ring[q] = xrow(min(q - R, H - 1), (q - R) % PF);
-  for (int c0 = 0;; c0 += WB) {
+    for (int q = R + 1; q < WB; ++q) ring[q] = xrow(min(q - R, H - 1), (q - R) % PF);
+  } else {  // a later segment starts from the R rows above it
+#pragma unroll
+    for (int q = 0; q < WB; ++q) ring[q] = xrow(y0 - R + q, ((q - R) % PF + PF) % PF);
+  }
+  for (int c0 = y0;; c0 += WB) {
 #pragma unroll
     for (int q = 0; q < WB; ++q) {
       const int c = c0 + q;
-      if (c >= H) return;
+      if (c >= y1) return;
       float e0 = 0.0f, d0 = 0.0f, e1 = 0.0f, d1 = 0.0f;
 #pragma unroll
       for (int j = 0; j <= 2 * R; ++j) {  // row c - R + j -> ring[(q + j) % WB]
@@ -526,7 +541,7 @@ __global__ __launch_bounds__(64) void k_norm_xy_walk(const uint32_t* __restrict_
       }
       const float4 o = make_float4(e0 * k, d0 * k, e1 * k, d1 * k);
       if (st2) {
-        dq[(i64)c * W / 2] = o;
+        st_nt(dq + (i64)c * W / 2, o);
       } else if (st1) {
         dp[(i64)c * W] = make_float2(o.x, o.y);
         if (xc + 1 < W) dp[(i64)c * W + 1] = make_float2(o.z, o.w);
@@ -673,9 +688,11 @@ hipError_t launch_normalize(const uint16_t* img, TS* norm, void* tmp1, void* tmp
     // then box along z fused with normalization
     constexpr int CW = 128;
     const int nxs = (W + CW - 1) / CW;
-    const i64 nblk = (i64)nxs * D;
+    int nys, rows;
+    walk_segments(H, 21, &nys, &rows);
+    const i64 nblk = (i64)nxs * D * nys;
     hipLaunchKernelGGL((k_norm_xy_walk<10>), dim3((unsigned)nblk), dim3(64), 0, s, p2,
-                       (float4*)tmp1, W, H, nxs, 1.0f / 21.0f);
+                       (float4*)tmp1, W, H, nxs, nys, rows, 1.0f / 21.0f);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     mark("normalize/minmax_x_box_xy", s);
     e = box_line<true, TS>((const float2*)tmp1, nullptr, norm, img, min_intensity, W, D, WH, W, H,

@@ -59,7 +59,9 @@ def assert_regions_match(g, o, hess):
                                             ((6, 40, 40), 100, 5, 0), ((40, 3, 2), 2, 25, 0),
                                             ((1, 1, 1), 3, 3, 0), ((6, 45, 301), 10, 4, 0),
                                             ((3, 130, 259), 10, 10, 100), ((4, 2, 129), 10, 1, 0),
-                                            ((2, 23, 128), 10, 2, 0), ((3, 11, 9), 10, 3, 0)])
+                                            ((2, 23, 128), 10, 2, 0), ((3, 11, 9), 10, 3, 0),
+                                            # tall planes: the row walker's two segments
+                                            ((5, 700, 150), 10, 4, 0), ((3, 341, 70), 10, 2, 0)])
 def test_normalize_bitexact(hd, shape, rxy, rz, m):
     img = RNG.integers(0, 4000, size=shape, dtype=np.uint16)
     got = to_np(hd.Normalize3DFilter(img, m, rxy, rz))
@@ -80,7 +82,10 @@ def test_normalize_full_u16_range_bitexact(hd, shape, rxy, rz):
 @pytest.mark.parametrize("shape,s,trunc", [((20, 36, 44), (2.0, 2.0, 3.2, 3.2), 4.0),
                                            ((33, 17, 65), (2.0, 1.6, 3.2, 2.56), 4.0),
                                            ((16, 64, 64), (4.0, 1.8, 6.0, 2.7), 2.0),
-                                           ((3, 5, 7), (1.0, 1.0, 1.5, 1.5), 2.0)])
+                                           ((3, 5, 7), (1.0, 1.0, 1.5, 1.5), 2.0),
+                                           # tall planes: the row walker's two segments
+                                           ((4, 700, 100), (2.0, 1.6, 3.2, 2.56), 4.0),
+                                           ((3, 433, 70), (2.0, 2.0, 3.2, 3.2), 4.0)])
 def test_dog_bitexact(hd, shape, s, trunc):
     norm = (RNG.standard_normal(shape) * 0.3).astype(np.float32)
     rz2 = hd.initGaussian3DKernel(*s, truncate=trunc)
