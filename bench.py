@@ -300,27 +300,41 @@ def run_batched(hd, forms, n_tiles, steps):
     out = dict(stacks=n_tiles, total_gvoxel=total / 1e9, largest_gvoxel=max(Ns) / 1e9,
                shapes_zyx=shapes, forms={})
     stream = torch.cuda.current_stream()
+    import tempfile
+    tune_dir = tempfile.mkdtemp(prefix="hdog_tune_")
     for f in forms:
         params = hd.make_params(shapes[0], C1["rxy"], C1["rz"], C1["min_intensity"],
                                 C1["sigma_xy1"], C1["sigma_z1"], C1["sigma_xy2"], C1["sigma_z2"],
                                 C1["truncate"], C1["gamma"], False, f)
-        t0 = time.perf_counter()
-        hd.batch_plan(imgs, params)
-        plan_s = time.perf_counter() - t0
-        hd.batch_run()
-        log(f"[batch {f}] planned + captured in {plan_s:.1f} s, warmup done")
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+        def timed_graph():
+            hd.batch_run()  # warmup
+            ev0.record(stream)
+            for _ in range(steps):
+                hd.batch_run()
+            ev1.record(stream)
+            torch.cuda.synchronize()
+            return ev0.elapsed_time(ev1) / steps
+
+        # the built-in launch shapes first, then the autotuned ones
+        hd.batch_plan(imgs, params)
+        ms_default = timed_graph()
+        hd.batch_release()
+        cache = os.path.join(tune_dir, f"{f}.txt")
         t0 = time.perf_counter()
-        ev0.record(stream)
-        for _ in range(steps):
-            hd.batch_run()
-        ev1.record(stream)
-        torch.cuda.synchronize()
+        hd.batch_plan(imgs, params, autotune_cache=cache)
+        plan_s = time.perf_counter() - t0
+        tuned = [l.split() for l in open(cache)]
+        log(f"[batch {f}] autotuned {len(tuned)} stack shapes, planned + captured in "
+            f"{plan_s:.1f} s")
+        t0 = time.perf_counter()
+        ms = timed_graph()
         wall = time.perf_counter() - t0
-        ms = ev0.elapsed_time(ev1) / steps
         regions = sum(hd.batch_tile_info(t)[0] for t in range(n_tiles))
         hd.batch_release()
-        log(f"[batch {f}] {steps} graph launches: {ms:.1f} ms each, {regions} regions")
+        log(f"[batch {f}] {steps} graph launches: {ms:.1f} ms each (built-in launch shapes "
+            f"{ms_default:.1f} ms), {regions} regions")
         # the same stacks through the per-stack entry point, one call (and host wait) each;
         # an untimed pass first grows the context scratch to the largest stack
         for im in imgs:
@@ -336,7 +350,13 @@ def run_batched(hd, forms, n_tiles, steps):
         hd.release_scratch()
         torch.cuda.empty_cache()
         log(f"[batch {f}] per-stack calls: {ms_per:.1f} ms, {per_regions} regions")
+        segs = {}
+        for w, h, d, _, a, b in tuned:
+            segs[f"{a}/{b}"] = segs.get(f"{a}/{b}", 0) + 1
         out["forms"][f] = dict(value=total / (ms * 1e-3) / 1e9, ms_per_batch=ms,
+                               ms_per_batch_builtin_shapes=ms_default,
+                               autotuned_shapes=len(tuned),
+                               autotuned_segments_norm_dog=segs,
                                wall_ms_per_batch=wall * 1e3 / steps, plan_s=plan_s,
                                num_regions=regions, per_stack_calls_ms=ms_per,
                                per_stack_value=total / (ms_per * 1e-3) / 1e9,

@@ -1,10 +1,12 @@
 // capi.hip -- the C-ABI of include/hdog.h: context, argument checks, scratch management
 // and stage sequencing.  No process-global state: everything lives in hdog_ctx.
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "hdog_internal.hpp"
@@ -58,6 +60,11 @@ struct hdog_ctx {
   hipEvent_t up_done[2] = {nullptr, nullptr};   // upload into the slot complete
   hipEvent_t run_done[2] = {nullptr, nullptr};  // last run reading the slot complete
   int up_shape[2][3] = {{0, 0, 0}, {0, 0, 0}};  // (W, H, D) of each slot's upload
+  // launch shapes chosen by hdog_autotune per stack shape and storage form
+  struct Tuned {
+    int W, H, D, precision, norm_segments, gauss_segments;
+  };
+  std::vector<Tuned> tuned;
 };
 
 thread_local hdog::Prof* hdog::tl_prof = nullptr;
@@ -149,6 +156,21 @@ void* noalloc_addr(int device) {
   if (device < 0 || device >= kMaxDevices) return nullptr;
   std::lock_guard<std::mutex> lock(g_noalloc_mu);
   return g_noalloc_addr[device];
+}
+
+// the tuned launch shape of a stack shape, or nullptr (built-in rule)
+const hdog_ctx::Tuned* find_tuned(const hdog_ctx* ctx, int W, int H, int D, int precision) {
+  for (const hdog_ctx::Tuned& t : ctx->tuned)
+    if (t.W == W && t.H == H && t.D == D && t.precision == precision) return &t;
+  return nullptr;
+}
+int tuned_norm(const hdog_ctx* ctx, int W, int H, int D, int precision) {
+  const hdog_ctx::Tuned* t = find_tuned(ctx, W, H, D, precision);
+  return t ? t->norm_segments : 0;
+}
+int tuned_gauss(const hdog_ctx* ctx, int W, int H, int D, int precision) {
+  const hdog_ctx::Tuned* t = find_tuned(ctx, W, H, D, precision);
+  return t ? t->gauss_segments : 0;
 }
 
 // the FP64 re-evaluation band of the reduced-precision forms (0 selects the default)
@@ -285,10 +307,11 @@ int run_stages(hdog_ctx* ctx, hipStream_t s, const hdog_params* p, const uint16_
   ctx->pool.n = 0;
   mark(nullptr, s);
   HIPCHK(launch_normalize<TN>(img, norm, ctx->tmp1.p, ctx->tmp2.p, p->min_intensity, W, H, D,
-                              p->radius_xy, p->radius_z, s));
+                              p->radius_xy, p->radius_z, s, nullptr, nullptr,
+                              tuned_norm(ctx, W, H, D, p->precision)));
   const float nz = powf(ctx->sigma[0], p->gamma * 2) * powf(ctx->sigma[1], p->gamma);
   HIPCHK(launch_dog<TN, TD>(norm, dog, ctx->tmp1.p, ctx->tmp2.p, (const float*)ctx->weights.p,
-                            ctx->r, nz, W, H, D, s));
+                            ctx->r, nz, W, H, D, s, tuned_gauss(ctx, W, H, D, p->precision)));
   // the Hessian pass also leaves compact candidate records and the candidate bit mask;
   // labeling and the region pass work from those, not from the mask and element volumes
   // (written only when pd / hess are non-NULL)
@@ -410,11 +433,13 @@ hipError_t batch_tile(hdog_ctx* ctx, BatchState* B, int t, hipStream_t s) {
   uint32_t* labels = (uint32_t*)B->labels.p;  // parent array, candidate voxels only
   hipError_t e;
   if ((e = launch_normalize<TN>(T.image, norm, B->tmp1.p, B->tmp2.p, p->min_intensity, W, H, D,
-                                p->radius_xy, p->radius_z, s)) != hipSuccess)
+                                p->radius_xy, p->radius_z, s, nullptr, nullptr,
+                                tuned_norm(ctx, W, H, D, p->precision))) != hipSuccess)
     return e;
   const float nz = powf(ctx->sigma[0], p->gamma * 2) * powf(ctx->sigma[1], p->gamma);
   if ((e = launch_dog<TN, TD>(norm, dog, B->tmp1.p, B->tmp2.p, (const float*)B->weights.p, ctx->r,
-                              nz, W, H, D, s)) != hipSuccess)
+                              nz, W, H, D, s, tuned_gauss(ctx, W, H, D, p->precision))) !=
+      hipSuccess)
     return e;
   const i64 ntiles = hessian_tiles(W, H, D);
   if ((e = launch_hessian<TD, TH>(dog, hess, nullptr, W, H, D, s, (float*)B->cand.p,
@@ -481,6 +506,89 @@ hipError_t batch_capture_any(hdog_ctx* ctx, BatchState* B, hipStream_t s) {
     case HDOG_BF16_DOG32: return batch_capture<b16, float, b16>(ctx, B, s);
     default: return batch_capture<float, float, float>(ctx, B, s);
   }
+}
+
+// ---- autotuner of the run-time launch shapes ---------------------------------------
+// Candidate row-segment counts of the two row walkers, 1..4; counts that give the same
+// segmentation as a smaller one are skipped.
+constexpr int kMaxSegments = 4;
+
+// median of three event-timed runs of f (after one untimed run)
+template <typename F>
+hipError_t time_median(hipStream_t s, F&& f, float* ms) {
+  hipEvent_t a, b;
+  hipError_t e = hipEventCreate(&a);
+  if (e != hipSuccess) return e;
+  if ((e = hipEventCreate(&b)) != hipSuccess) { (void)hipEventDestroy(a); return e; }
+  float t[3] = {0, 0, 0};
+  e = f();
+  for (int k = 0; e == hipSuccess && k < 3; ++k) {
+    if ((e = hipEventRecord(a, s)) != hipSuccess) break;
+    if ((e = f()) != hipSuccess) break;
+    if ((e = hipEventRecord(b, s)) != hipSuccess) break;
+    if ((e = hipEventSynchronize(b)) != hipSuccess) break;
+    e = hipEventElapsedTime(&t[k], a, b);
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  std::sort(t, t + 3);
+  *ms = t[1];
+  return e;
+}
+
+// Measured search: the normalization stage with every candidate segment count of its row
+// walker, then the DoG stage likewise, on the caller's stack; the fastest of each.
+template <typename TN, typename TD>
+int autotune_t(hdog_ctx* ctx, hipStream_t s, const hdog_params* p, const uint16_t* img,
+               int* best_norm, int* best_gauss, float* ms_norm, float* ms_gauss) {
+  const int W = p->width, H = p->height, D = p->depth;
+  const i64 N = (i64)W * H * D;
+  HIPCHK(ensure(ctx->tmp1, (size_t)N * 8));
+  HIPCHK(ensure(ctx->tmp2, (size_t)N * 8));
+  HIPCHK(ensure(ctx->norm, (size_t)N * sizeof(TN)));
+  HIPCHK(ensure(ctx->dog, (size_t)N * sizeof(TD)));
+  TN* norm = (TN*)ctx->norm.p;
+  TD* dog = (TD*)ctx->dog.p;
+  const float nz = powf(ctx->sigma[0], p->gamma * 2) * powf(ctx->sigma[1], p->gamma);
+  const int wb_gauss = 2 * std::max(ctx->r[0], ctx->r[2]) + 1;
+  struct Stage {
+    int wb, *best;
+    float* ms;
+    bool dog;
+  } stages[2] = {{21, best_norm, ms_norm, false}, {wb_gauss, best_gauss, ms_gauss, true}};
+  for (const Stage& st : stages) {
+    *st.best = 1;
+    *st.ms = 1e30f;
+    int seen[kMaxSegments + 1] = {0};
+    int nseen = 0;
+    for (int want = 1; want <= kMaxSegments; ++want) {
+      int nys, rows;
+      walk_segments(H, st.wb, &nys, &rows, want);
+      bool dup = false;
+      for (int k = 0; k < nseen; ++k) dup |= seen[k] == nys;
+      if (dup) continue;
+      seen[nseen++] = nys;
+      float ms = 0.0f;
+      if (st.dog)
+        HIPCHK(time_median(s, [&] {
+          return launch_dog<TN, TD>(norm, dog, ctx->tmp1.p, ctx->tmp2.p,
+                                    (const float*)ctx->weights.p, ctx->r, nz, W, H, D, s, want);
+        }, &ms));
+      else
+        HIPCHK(time_median(s, [&] {
+          return launch_normalize<TN>(img, norm, ctx->tmp1.p, ctx->tmp2.p, p->min_intensity, W, H,
+                                      D, p->radius_xy, p->radius_z, s, nullptr, nullptr, want);
+        }, &ms));
+      if (ms < *st.ms) {
+        *st.ms = ms;
+        *st.best = want;
+      }
+    }
+    if (!st.dog)  // the DoG stage is timed on the normalized stack
+      HIPCHK(launch_normalize<TN>(img, norm, ctx->tmp1.p, ctx->tmp2.p, p->min_intensity, W, H, D,
+                                  p->radius_xy, p->radius_z, s, nullptr, nullptr, *st.best));
+  }
+  return HDOG_OK;
 }
 
 }  // namespace
@@ -822,6 +930,65 @@ int hdog_run_device(hdog_ctx* ctx, void* stream, const hdog_params* p, const uin
       return run_stages<float, float, float>(ctx, s, p, d_img, (float*)d_norm, (float*)d_dog,
                                              (float*)d_hessian, d_pd, d_labels, dense, num_regions);
   }
+}
+
+int hdog_autotune(hdog_ctx* ctx, void* stream, const hdog_params* p, const uint16_t* d_img,
+                  const char* cache_path, int* norm_segments, int* gauss_segments) {
+  if (!ctx) return HDOG_ERR_ARG;
+  int rc = check_params(ctx, p);
+  if (rc) return rc;
+  const int W = p->width, H = p->height, D = p->depth, prec = p->precision;
+  int ns = 0, gs = 0;
+  bool found = false;
+  if (cache_path && *cache_path) {  // lines "width height depth precision norm gauss"
+    if (FILE* f = std::fopen(cache_path, "r")) {
+      int w, h, d, pr, a, b;
+      while (std::fscanf(f, "%d %d %d %d %d %d", &w, &h, &d, &pr, &a, &b) == 6)
+        if (w == W && h == H && d == D && pr == prec && a >= 1 && a <= kMaxSegments && b >= 1 &&
+            b <= kMaxSegments) {
+          ns = a;
+          gs = b;
+          found = true;
+        }
+      std::fclose(f);
+    }
+  }
+  if (!found) {
+    if (!d_img) return fail(ctx, HDOG_ERR_ARG, "no cached launch shape and no stack to time");
+    HIPCHK(hipSetDevice(ctx->device));
+    if ((rc = ensure_gauss(ctx, p))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    float tn = 0, tg = 0;
+    typedef __half h16;
+    typedef __hip_bfloat16 b16;
+    switch (prec) {
+      case HDOG_FP16: rc = autotune_t<h16, h16>(ctx, s, p, d_img, &ns, &gs, &tn, &tg); break;
+      case HDOG_BF16: rc = autotune_t<b16, b16>(ctx, s, p, d_img, &ns, &gs, &tn, &tg); break;
+      case HDOG_FP16_DOG32: rc = autotune_t<h16, float>(ctx, s, p, d_img, &ns, &gs, &tn, &tg); break;
+      case HDOG_BF16_DOG32: rc = autotune_t<b16, float>(ctx, s, p, d_img, &ns, &gs, &tn, &tg); break;
+      default: rc = autotune_t<float, float>(ctx, s, p, d_img, &ns, &gs, &tn, &tg);
+    }
+    if (rc) return rc;
+    if (cache_path && *cache_path) {
+      if (FILE* f = std::fopen(cache_path, "a")) {
+        std::fprintf(f, "%d %d %d %d %d %d\n", W, H, D, prec, ns, gs);
+        std::fclose(f);
+      } else {
+        return fail(ctx, HDOG_ERR_ARG, "cannot append to the tuning cache %s", cache_path);
+      }
+    }
+  }
+  bool have = false;
+  for (hdog_ctx::Tuned& t : ctx->tuned)
+    if (t.W == W && t.H == H && t.D == D && t.precision == prec) {
+      t.norm_segments = ns;
+      t.gauss_segments = gs;
+      have = true;
+    }
+  if (!have) ctx->tuned.push_back({W, H, D, prec, ns, gs});
+  if (norm_segments) *norm_segments = ns;
+  if (gauss_segments) *gauss_segments = gs;
+  return HDOG_OK;
 }
 
 int hdog_process_substack(hdog_ctx* ctx, void* stream, const hdog_params* p,

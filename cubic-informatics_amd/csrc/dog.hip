@@ -368,12 +368,12 @@ hipError_t gauss_x(const TS* src, float2* dst, const float* w1, const float* w2,
 // Fused x + y passes for the compile-time radius pairs; false when (r1, r2) has none.
 template <typename TS>
 bool gauss_xy(const TS* src, float2* dst, const float* w1, const float* w2, int r1, int r2,
-              int W, int H, int D, hipStream_t s, hipError_t* e) {
+              int W, int H, int D, hipStream_t s, hipError_t* e, int segments) {
 #define X(A, B)                                                                              \
   if (r1 == A && r2 == B) {                                                                  \
     const int nxs = (W + 63) / 64;                                                           \
     int nys, rows;                                                                           \
-    walk_segments(H, 2 * (A > B ? A : B) + 1, &nys, &rows);                                  \
+    walk_segments(H, 2 * (A > B ? A : B) + 1, &nys, &rows, segments);                        \
     const i64 nblk = (i64)nxs * D * nys;                                                     \
     hipLaunchKernelGGL((k_gauss_xy_walk<A, B, TS>), dim3((unsigned)nblk), dim3(64), 0, s,    \
                        src, dst, w1, w2, W, H, nxs, nys, rows);                              \
@@ -419,7 +419,8 @@ hipError_t gauss_line(const float2* src, float2* dst, TS* dog, const float* w1, 
 
 template <typename TN, typename TD>
 hipError_t launch_dog(const TN* norm, TD* dog, void* tmp1, void* tmp2, const float* weights,
-                      const int r[4], float normalizer, int W, int H, int D, hipStream_t s) {
+                      const int r[4], float normalizer, int W, int H, int D, hipStream_t s,
+                      int segments) {
   const i64 WH = (i64)W * H;
   const float* wxy1 = weights;
   const float* wz1 = weights + kMaxTaps;
@@ -427,7 +428,7 @@ hipError_t launch_dog(const TN* norm, TD* dog, void* tmp1, void* tmp2, const flo
   const float* wz2 = weights + 3 * kMaxTaps;
   hipError_t e = hipSuccess;
   const float2* gxy = (const float2*)tmp2;
-  if (gauss_xy<TN>(norm, (float2*)tmp1, wxy1, wxy2, r[0], r[2], W, H, D, s, &e)) {
+  if (gauss_xy<TN>(norm, (float2*)tmp1, wxy1, wxy2, r[0], r[2], W, H, D, s, &e, segments)) {
     if (e != hipSuccess) return e;
     mark("dog/gauss_xy", s);
     gxy = (const float2*)tmp1;
@@ -471,7 +472,7 @@ hipError_t launch_gauss_pair(const float* src, void* tmp1, void* tmp2, const flo
 
 #define HDOG_INST_DOG(TN, TD)                                                                \
   template hipError_t launch_dog<TN, TD>(const TN*, TD*, void*, void*, const float*, const int*, \
-                                         float, int, int, int, hipStream_t);
+                                         float, int, int, int, hipStream_t, int);
 HDOG_INST_DOG(float, float)
 HDOG_INST_DOG(__half, __half)
 HDOG_INST_DOG(__hip_bfloat16, __hip_bfloat16)

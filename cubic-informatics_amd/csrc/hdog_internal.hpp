@@ -166,14 +166,16 @@ inline void mark(const char* name, hipStream_t s) {
 // walks one contiguous range of tiles in order: tiles that share cache lines (halo rows,
 // 128-byte lines split between narrow tiles) then meet in the same L2.  A bijection of
 // [0, n) for any n.
-// Row walkers (one wave per plane strip) split tall planes into two row segments, each
+// Row walkers (one wave per plane strip) split tall planes into row segments, each
 // starting with the 2R rows of context its window needs: a C1 plane strip as one wave left
 // the last round of waves part-empty (2.5 and 1.4 rounds of resident waves for the two
-// walkers).  Segment length: a multiple of the ring period wb.
-inline void walk_segments(int H, int wb, int* nys, int* rows) {
-  *nys = H >= 16 * wb ? 2 : 1;
-  const int per = (H + *nys - 1) / *nys;
+// walkers).  `want` segments (a tuned launch shape), or by default two for planes of at
+// least 16 ring periods; segment length: a multiple of the ring period wb.
+inline void walk_segments(int H, int wb, int* nys, int* rows, int want = 0) {
+  const int n = want > 0 ? want : (H >= 16 * wb ? 2 : 1);
+  const int per = (H + n - 1) / n;
   *rows = (per + wb - 1) / wb * wb;
+  *nys = (H + *rows - 1) / *rows;
 }
 
 __device__ __forceinline__ unsigned xcd_tile(unsigned b, unsigned n) {
@@ -201,14 +203,18 @@ template <typename T> __device__ __forceinline__ float round_to(float v) {
 // ---- launchers (return hipSuccess or the launch error) ----------------------------
 // normalize stage; tmp1/tmp2 each hold >= 8*N bytes.  ero / dil (optional): the box means
 // of the min and max filters, E and D, as separate float volumes.
+// segments: row segments of the row walker (0: walk_segments' default).
 template <typename TS>
 hipError_t launch_normalize(const uint16_t* img, TS* norm, void* tmp1, void* tmp2,
                             float min_intensity, int W, int H, int D, int rxy, int rz,
-                            hipStream_t s, float* ero = nullptr, float* dil = nullptr);
-// DoG stage; weights: device array [4][kMaxTaps] (xy1, z1, xy2, z2); tmp1/tmp2 >= 8*N bytes
+                            hipStream_t s, float* ero = nullptr, float* dil = nullptr,
+                            int segments = 0);
+// DoG stage; weights: device array [4][kMaxTaps] (xy1, z1, xy2, z2); tmp1/tmp2 >= 8*N bytes;
+// segments as for launch_normalize
 template <typename TN, typename TD>
 hipError_t launch_dog(const TN* norm, TD* dog, void* tmp1, void* tmp2, const float* weights,
-                      const int r[4], float normalizer, int W, int H, int D, hipStream_t s);
+                      const int r[4], float normalizer, int W, int H, int D, hipStream_t s,
+                      int segments = 0);
 hipError_t launch_gauss_pair(const float* src, void* tmp1, void* tmp2, const float* const w[6],
                              const int r[6], float nz, float* dog, float2* pair_out, int W, int H,
                              int D, hipStream_t s);

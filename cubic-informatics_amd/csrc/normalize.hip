@@ -666,7 +666,7 @@ __global__ __launch_bounds__(NT) void k_split_ed(const float2* __restrict__ ed,
 template <typename TS>
 hipError_t launch_normalize(const uint16_t* img, TS* norm, void* tmp1, void* tmp2,
                             float min_intensity, int W, int H, int D, int rxy, int rz,
-                            hipStream_t s, float* ero, float* dil) {
+                            hipStream_t s, float* ero, float* dil, int segments) {
   const i64 WH = (i64)W * H, N = WH * D;
   if (rxy == 0 || rz == 0) {
     const i64 nb = std::min<i64>((N + NT - 1) / NT, 1 << 20);
@@ -689,7 +689,7 @@ hipError_t launch_normalize(const uint16_t* img, TS* norm, void* tmp1, void* tmp
     constexpr int CW = 128;
     const int nxs = (W + CW - 1) / CW;
     int nys, rows;
-    walk_segments(H, 21, &nys, &rows);
+    walk_segments(H, 21, &nys, &rows, segments);
     const i64 nblk = (i64)nxs * D * nys;
     hipLaunchKernelGGL((k_norm_xy_walk<10>), dim3((unsigned)nblk), dim3(64), 0, s, p2,
                        (float4*)tmp1, W, H, nxs, nys, rows, 1.0f / 21.0f);
@@ -738,11 +738,11 @@ hipError_t launch_normalize(const uint16_t* img, TS* norm, void* tmp1, void* tmp
 }
 
 template hipError_t launch_normalize<float>(const uint16_t*, float*, void*, void*, float, int,
-                                            int, int, int, int, hipStream_t, float*, float*);
+                                            int, int, int, int, hipStream_t, float*, float*, int);
 template hipError_t launch_normalize<__half>(const uint16_t*, __half*, void*, void*, float, int,
-                                             int, int, int, int, hipStream_t, float*, float*);
+                                             int, int, int, int, hipStream_t, float*, float*, int);
 template hipError_t launch_normalize<__hip_bfloat16>(const uint16_t*, __hip_bfloat16*, void*,
                                                      void*, float, int, int, int, int, int,
-                                                     hipStream_t, float*, float*);
+                                                     hipStream_t, float*, float*, int);
 
 }  // namespace hdog

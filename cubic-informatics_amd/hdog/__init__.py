@@ -198,12 +198,38 @@ class HDoG:
         self._check(C.hdog_regions_download(self._h, self._stream(), ctypes.byref(reg)))
         return self._finish_regions(o)
 
+    # -- autotuned launch shapes ------------------------------------------------
+    def autotune(self, img, params, cache_path=None):
+        """Measured choice of the row walkers' segment counts for the stack shape of
+        `params` (timed on `img`, a device uint16 (D, H, W) tensor, unless `cache_path`
+        already holds the shape); kept in the context for later runs of that shape.
+        Returns (normalization segments, DoG segments)."""
+        ns, gs = ctypes.c_int(), ctypes.c_int()
+        ptr = _ptr(self._dev(img, torch.uint16)) if img is not None else None
+        path = cache_path.encode() if cache_path else None
+        self._check(C.hdog_autotune(self._h, self._stream(), ctypes.byref(params), ptr, path,
+                                    ctypes.byref(ns), ctypes.byref(gs)))
+        return ns.value, gs.value
+
     # -- batched stacks (one captured graph) -------------------------------------
-    def batch_plan(self, images, params, max_regions_per_stack=0, arena_capacity=0):
+    def batch_plan(self, images, params, max_regions_per_stack=0, arena_capacity=0,
+                   autotune_cache=None):
         """Capture the pipeline of every stack in `images` (uint16/int16 device tensors of
         shape (D, H, W), kept alive by this object) into one graph.  `params` from
-        make_params; its sizes are ignored."""
+        make_params; its sizes are ignored.  With `autotune_cache` (a file path) every
+        stack shape's launch shapes are first chosen by the autotuner (measured, or read
+        from the file)."""
         self._batch_images = [im if im.is_contiguous() else im.contiguous() for im in images]
+        if autotune_cache:
+            done = set()
+            for im in self._batch_images:
+                shape = tuple(im.shape)
+                if shape in done:
+                    continue
+                done.add(shape)
+                p = type(params).from_buffer_copy(params)
+                p.depth, p.height, p.width = shape
+                self.autotune(im, p, autotune_cache)
         tiles = (C.Tile * len(images))()
         for t, im in enumerate(self._batch_images):
             D, H, W = im.shape

@@ -83,6 +83,8 @@ hdog_host_free = _sig("hdog_host_free", [_vp, _vp])
 hdog_substack_upload = _sig("hdog_substack_upload", [_vp, ctypes.POINTER(Params), _vp, _i, _i])
 hdog_substack_run = _sig("hdog_substack_run", [_vp, _vp, ctypes.POINTER(Params), _i,
                                                ctypes.POINTER(Regions)])
+hdog_autotune = _sig("hdog_autotune", [_vp, _vp, ctypes.POINTER(Params), _vp, ctypes.c_char_p,
+                                       ctypes.POINTER(_i), ctypes.POINTER(_i)])
 hdog_batch_plan = _sig("hdog_batch_plan", [_vp, ctypes.POINTER(Params), ctypes.POINTER(Tile), _i,
                                            _i64, _i64])
 hdog_batch_run = _sig("hdog_batch_run", [_vp, _vp])
@@ -115,6 +117,6 @@ EXPORTED = ["hdog_create", "hdog_destroy", "hdog_last_error", "hdog_release_scra
             "hdog_process_substack", "hdog_run_device", "hdog_synth_volume",
             "hdog_set_profiling", "hdog_stage_times", "hdog_soft_forward", "hdog_soft_backward",
             "hdog_soft_normalize_forward", "hdog_soft_normalize_backward",
-            "hdog_batch_plan", "hdog_batch_run", "hdog_batch_tile_info", "hdog_batch_download",
+            "hdog_autotune", "hdog_batch_plan", "hdog_batch_run", "hdog_batch_tile_info", "hdog_batch_download",
             "hdog_batch_release", "hdog_host_alloc", "hdog_host_free", "hdog_substack_upload",
             "hdog_substack_run"]

@@ -204,6 +204,18 @@ int hdog_run_device(hdog_ctx* ctx, void* stream, const hdog_params* p, const uin
                     void* d_norm, void* d_dog, void* d_hessian, uint8_t* d_pd,
                     uint32_t* d_labels, int64_t* num_regions);
 
+/* Autotuner of the run-time launch shapes for one stack shape and storage form (no
+ * counterpart in the reference, whose tile shapes are compile-time constants,
+ * src/3D/NormalizeFilter.cu:13-47, src/3D/DoG3DFilter.cu:29-58): the number of row
+ * segments each of the normalization's and the DoG's row walkers splits a plane into,
+ * chosen by timing every candidate (1..4) on d_img with HIP events (median of 3).  With
+ * cache_path, a line "width height depth precision norm gauss" for the shape is read from
+ * that file when present (d_img may then be NULL), and a new measurement is appended.  The
+ * choice is kept in the context and used by hdog_run_device and hdog_batch_plan for stacks
+ * of that shape; it changes launch shapes only, never results. */
+int hdog_autotune(hdog_ctx* ctx, void* stream, const hdog_params* p, const uint16_t* d_img,
+                  const char* cache_path, int* norm_segments, int* gauss_segments);
+
 /* ---- batched stacks ------------------------------------------------------------
  * Many whole stacks (tiles of one brain) resident on the device, run as ONE captured
  * hipGraph launch.  Replaces the per-stack fan-out of script/HDoG_gpu.py:67-104, which

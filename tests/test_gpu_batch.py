@@ -165,3 +165,75 @@ def test_batch_rejects_bad_plans(hd):
     bad = torch.zeros((0, 8, 8), dtype=torch.uint16, device=hd.device)
     with pytest.raises(HDoGError):
         hd.batch_plan([img, bad], params(hd, (4, 8, 8)))
+
+
+def c3_shapes(n=40, seed=3):
+    """Stack shapes (z, y, x) of the batched workload: z ~ U{64..640}, y and x each from
+    {512, 1024, 1536, 2160} (the bench's generator)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    z = rng.integers(64, 641, n)
+    y = rng.choice([512, 1024, 1536, 2160], n)
+    x = rng.choice([512, 1024, 1536, 2160], n)
+    return [(int(a), int(b), int(c)) for a, b, c in zip(z, y, x)]
+
+
+def test_autotune_cache_and_unchanged_results(hd, tmp_path):
+    # the tuner's choice is read back from its cache file by a fresh context without a
+    # stack to time, and tuned launch shapes give the untuned regions bit for bit
+    import torch
+    from hdog import HDoG
+    shape = (24, 700, 260)
+    vol = synth.make_volume("c1", shape=shape, n_blobs=120, seed=51)
+    img = torch.from_numpy(vol).to(hd.device)
+    p = params(hd, shape)
+    ref = per_stack(hd, vol, "fp32")
+    cache = str(tmp_path / "tune.txt")
+    ns, gs = hd.autotune(img, p, cache)
+    assert 1 <= ns <= 4 and 1 <= gs <= 4
+    line = open(cache).read().split()
+    assert [int(v) for v in line] == [shape[2], shape[1], shape[0], 0, ns, gs]
+    fresh = HDoG(0)
+    try:
+        assert fresh.autotune(None, p, cache) == (ns, gs)
+        with pytest.raises(Exception):
+            fresh.autotune(None, params(hd, (24, 700, 261)), cache)  # not cached, no stack
+    finally:
+        fresh.close()
+    for want in range(1, 5):  # every launch shape the tuner may pick gives the same regions
+        with open(cache, "w") as f:
+            f.write(f"{shape[2]} {shape[1]} {shape[0]} 0 {want} {want}\n")
+        assert hd.autotune(None, p, cache) == (want, want)
+        assert_same(per_stack(hd, vol, "fp32"), ref, f"segments {want}")
+
+
+def test_batch_of_c3_stacks_tuned_matches_per_stack_and_oracle(hd, tmp_path):
+    # the three smallest stacks of the batched workload itself, rendered by its generator,
+    # through the autotuned graph, the per-stack path and the oracle
+    import torch
+    cfg = synth.WORKLOADS["c1"]
+    shapes = c3_shapes()
+    pick = sorted(range(len(shapes)), key=lambda t: np.prod(shapes[t]))[:3]
+    imgs, vols = [], []
+    for t in pick:
+        sh = shapes[t]
+        nb = int(sh[0] * sh[1] * sh[2] / 2e4)
+        blobs = synth.blob_table(sh, nb, 1000 + t, cfg["sigma"], cfg["zfactor"], cfg["amp"])
+        im = hd.synth_volume(sh, blobs, 1000 + t, cfg["mean0"], cfg["mean1"], cfg["sd"])
+        imgs.append(im)
+        vols.append(im.cpu().numpy())
+    print("C3 stacks", [shapes[t] for t in pick], flush=True)
+    hd.batch_plan(imgs, params(hd, shapes[pick[0]]), autotune_cache=str(tmp_path / "c3.txt"))
+    try:
+        hd.batch_run()
+        got = [hd.batch_download(t) for t in range(len(imgs))]
+    finally:
+        hd.batch_release()
+    for k, v in enumerate(vols):
+        assert_same(got[k], per_stack(hd, v, "fp32"), f"C3 stack {v.shape}")
+        o = oracle.substack(v, P)["regions"]
+        assert len(got[k]["label"]) == len(o["label"])
+        for key in ("label", "count", "size16", "isum", "imax"):
+            np.testing.assert_array_equal(got[k][key][1:], o[key][1:], err_msg=key)
+        np.testing.assert_array_equal(got[k]["centroid"][1:], o["centroid"][1:])
+    del imgs
+    torch.cuda.empty_cache()
