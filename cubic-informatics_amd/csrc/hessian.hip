@@ -326,6 +326,219 @@ __global__ __launch_bounds__(NT, 5) void k_hessian(const TD* __restrict__ dog, T
   if (cand_cnt && threadIdx.x == 0) cand_cnt[tile] = s_cnt;
 }
 
+// ---- records-only form -----------------------------------------------------------------
+// When neither the element volumes nor the byte mask are wanted (the whole-path step), the
+// elements only decide the candidate test and fill the records.  The sign of a zero
+// element reaches neither: the tests compare with zero (a -0 and a +0 compare alike) and
+// the region sums add records into accumulators that start at +0.  So every difference is
+// taken as (hi - lo) * 0.5f, without the leading 0 + of the per-stage form, which only
+// turns a -0 into +0; every non-zero value is the same, bit for bit.
+//
+// Layout: a wave owns 64 x-adjacent columns (one lane each) and kRecTY consecutive output
+// rows; a work-group is the kHessTY / kRecTY waves of one Hessian tile row block, walking
+// kRecNZ Hessian tiles in z.  Everything a lane needs in y and z stays in its registers:
+// dog rows y-2 .. y+kRecTY+1 of each slice arrive from global memory (one row descriptor
+// each, so that rows and slices outside the volume, and columns outside the plane, read
+// zero), Hy and the y second differences are register arithmetic, and the z differences
+// walk three-slice register rings.  The x neighbours come from the adjacent lanes by DPP
+// wave shifts.  The columns beyond the wave come from one more load per row (`halo`),
+// whose lanes 0, 1, 62 and 63 hold columns x0-1, x0-2, x0+65 and x0+64: a shift keeps
+// the destination's own value in the lane without a source, so copies of the halo are
+// the destinations of the left and right shifts, and a pair swap of it brings x0-2 /
+// x0+65 to lanes 0 / 63 for the first differences just outside the wave, Hx(x0-1) and
+// Hx(x0+64).  No LDS beyond the record counters, no barrier inside the walk.  (Two halo
+// loads per row, one per shift, save the copies but cost the registers of a fourth wave
+// per SIMD, which measured slower.)
+constexpr int kRecTY = 4;                     // output rows per lane
+constexpr int kRecWaves = kHessTY / kRecTY;   // waves per work-group
+constexpr int kRecNZ = 2;                     // Hessian tiles per work-group, along z
+static_assert(kHessTX == 64 && kHessTY % kRecTY == 0, "one wave spans a tile row");
+
+// v of lane i - 1 (lane 0 keeps old) and of lane i + 1 (lane 63 keeps old); one DPP move
+// each, written over old.  The whole wave must be active.
+__device__ __forceinline__ float from_left(float old, float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float from_right(float old, float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+// lanes 2k and 2k + 1 exchange v
+__device__ __forceinline__ float swap_pairs(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xB1, 0xf, 0xf, false));
+}
+
+template <typename TD, typename TS, bool BAND>
+__global__ __launch_bounds__(64 * kRecWaves) void k_hessian_rec(
+    const TD* __restrict__ dog, int W, int H, int D, int nbx, int nby, float* __restrict__ cand,
+    uint32_t* __restrict__ cand_cnt, float band) {
+  constexpr int TY = kRecTY, NR = TY + 4, NH = TY + 2;  // dog rows, first-difference rows
+  __shared__ uint32_t s_cnt[kRecNZ];
+  if (threadIdx.x < kRecNZ) s_cnt[threadIdx.x] = 0u;
+  const unsigned g = xcd_tile(blockIdx.x, gridDim.x);
+  const int bx = (int)(g % (unsigned)nbx), by = (int)(g / (unsigned)nbx % (unsigned)nby);
+  const int gz = (int)(g / (unsigned)nbx / (unsigned)nby);
+  const int lane = threadIdx.x & 63;
+  // first output row of the wave in the tile (wave-uniform: the row descriptors stay scalar)
+  const int ty0 = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) * TY;
+  const int x0 = bx * 64, x = x0 + lane, y = by * kHessTY + ty0;
+  const int z0 = gz * kRecNZ * kHessTZ, z1 = min(z0 + kRecNZ * kHessTZ, D);
+  constexpr uint32_t SZ = sizeof(TD);
+  // byte offsets in a row: a negative column wraps past the record count
+  const uint32_t vo = (uint32_t)x * SZ;
+  const int hc = lane == 1 ? x0 - 2 : lane == 62 ? x0 + 65 : lane == 63 ? x0 + 64 : x0 - 1;
+  const uint32_t ho = (uint32_t)hc * SZ;
+  const bool xin = x < W;
+  const bool hin = lane == 0 ? x0 > 0 : (lane == 63 ? x0 + 64 < W : true);
+  const float hsign = lane == 0 ? -0.5f : 0.5f;  // Hx(x0-1) = -(d(x0-2) - d(x0)) / 2
+  const uint32_t WH = (uint32_t)W * (uint32_t)H;
+
+  using Bits = typename StoredBits<TD>::type;
+  Bits qd[3][NR], qh[3][NH];  // slices in flight: dog rows, halo columns
+  float Hx[3][NH], Hy[3][NH], Hz[3][TY], Hxx[3][TY];
+  const uint32_t row_bytes = (uint32_t)W * SZ;
+  // CLIP: some of the wave's dog rows lie outside the plane
+  auto fetch = [&](int s, auto q_c, auto clip_c) {
+    constexpr int q = decltype(q_c)::value;
+    constexpr bool CLIP = decltype(clip_c)::value;
+    // slices past the walk's last one are loaded too, with no records: every step issues
+    // the same loads
+    const uint32_t nrec = s >= 0 && s < D && s <= z1 + 1 ? row_bytes : 0u;
+    // rows and slices outside the volume get no records (their addresses are not used)
+    const char* rows = (const char*)dog + ((int64_t)s * WH + (int64_t)(y - 2) * W) * SZ;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+      const uint32_t n = !CLIP || (unsigned)(y - 2 + r) < (unsigned)H ? nrec : 0u;
+      const __amdgpu_buffer_rsrc_t rs = buf_rsrc(rows + r * row_bytes, n);
+      qd[q][r] = buf_load<TD>(rs, vo);
+      if (r >= 1 && r <= NH) {
+        qh[q][r - 1] = buf_load<TD>(rs, ho);
+      }
+    }
+  };
+  __syncthreads();  // counters cleared
+
+  auto step = [&](int s, auto a_c, auto edge_c, auto clip_c) {
+    constexpr int A = decltype(a_c)::value;  // ring slot of slice s (and of s - 3)
+    constexpr int B = (A + 2) % 3;           // slot of s - 1 (and of s + 2)
+    constexpr int C = (A + 1) % 3;           // slot of s - 2 (and of s + 1)
+    constexpr bool EDGE = decltype(edge_c)::value;  // a column of the wave or halo is outside
+    constexpr bool CLIP = decltype(clip_c)::value;  // a row of the wave or halo is outside
+    float d[NR], h[NH];
+#pragma unroll
+    for (int r = 0; r < NR; ++r) d[r] = from_bits<TD>(qd[A][r]);
+#pragma unroll
+    for (int j = 0; j < NH; ++j) h[j] = from_bits<TD>(qh[A][j]);
+    // outputs of slice z = s - 2
+    const int z = s - 2;
+    if (z >= z0 && z < z1) {
+      const int tz = (z - z0) / kHessTZ;  // Hessian tile of the slice within the walk
+      const unsigned tile = (unsigned)(((gz * kRecNZ + tz) * nby + by) * nbx + bx);
+      const uint32_t lz = (uint32_t)(z - (z0 + tz * kHessTZ));
+      const __amdgpu_buffer_rsrc_t rr =
+          buf_rsrc(cand + (size_t)tile * kCandTile * 7, (uint32_t)kCandTile * 7u * 4u);
+#pragma unroll
+      for (int r = 0; r < TY; ++r) {
+        const float xx = Hxx[C][r];
+        const float xy = (Hx[C][r + 2] - Hx[C][r]) * 0.5f;
+        const float xz = (Hx[B][r + 1] - Hx[A][r + 1]) * 0.5f;
+        const float yy = (Hy[C][r + 2] - Hy[C][r]) * 0.5f;
+        const float yz = (Hy[B][r + 1] - Hy[A][r + 1]) * 0.5f;
+        const float zz = (Hz[B][r] - Hz[A][r]) * 0.5f;
+        const bool m = BAND ? negdef_band(xx, xy, xz, yy, yz, zz, band) != 0
+                            : negdef32(xx, xy, xz, yy, yz, zz) != 0;
+        // voxel 0 is background whatever its mask (its label is 0): no record
+        const bool is_c = m && (!EDGE || xin) && (!CLIP || y + r < H) && (x | (y + r) | z) != 0;
+        // one LDS reservation per wave and row, consecutive slots in lane order
+        const unsigned long long bal = __ballot(is_c);
+        if (bal) {
+          uint32_t base = 0;
+          if (lane == 0) base = atomicAdd(&s_cnt[tz], (uint32_t)__popcll(bal));
+          base = __builtin_amdgcn_readfirstlane(base);
+          if (is_c) {
+            const uint32_t slot = base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+            const uint32_t cidx = (uint32_t)lane + 64u * ((uint32_t)(ty0 + r) + kHessTY * lz);
+            const int vb = (int)(slot * 4u);
+            const float h6[6] = {round_to<TS>(xx), round_to<TS>(xy), round_to<TS>(xz),
+                                 round_to<TS>(yy), round_to<TS>(yz), round_to<TS>(zz)};
+            __builtin_amdgcn_raw_buffer_store_b32(cidx, rr, vb, 0, 0);
+#pragma unroll
+            for (int q = 0; q < 6; ++q)
+              __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(h6[q]), rr, vb,
+                                                    (1 + q) * kCandTile * 4, 0);
+          }
+        }
+      }
+    }
+    // first differences of slice s, and its x second difference
+    if (s >= z0 - 1 && s <= z1) {
+      float o[TY];  // lane 0: Hx(x0-1), lane 63: Hx(x0+64)
+#pragma unroll
+      for (int r = 0; r < TY; ++r) {
+        o[r] = (swap_pairs(h[r + 1]) - d[r + 2]) * hsign;
+        if (EDGE && !hin) o[r] = 0.0f;
+      }
+#pragma unroll
+      for (int j = 0; j < NH; ++j) {
+        const float c = d[j + 1];
+        Hx[A][j] = (from_right(h[j], c) - from_left(h[j], c)) * 0.5f;
+        if (EDGE && !xin) Hx[A][j] = 0.0f;
+        Hy[A][j] = (d[j + 2] - d[j]) * 0.5f;
+      }
+      if (CLIP) {
+#pragma unroll
+        for (int j = 0; j < NH; ++j)
+          if (y - 1 + j < 0 || y - 1 + j >= H) Hy[A][j] = 0.0f;
+      }
+      if (s >= z0 && s < z1) {
+#pragma unroll
+        for (int r = 0; r < TY; ++r) {
+          const float c = Hx[A][r + 1];
+          Hxx[A][r] = (from_right(o[r], c) - from_left(o[r], c)) * 0.5f;
+        }
+      }
+    }
+    // Hz(s), zero outside the volume, from dog(s + 1) (in flight since the last step) and
+    // dog(s - 1), whose registers the next loads take over
+    if (s >= z0 - 1 && s <= z1) {
+#pragma unroll
+      for (int r = 0; r < TY; ++r)
+        Hz[A][r] = (from_bits<TD>(qd[C][r + 2]) - from_bits<TD>(qd[B][r + 2])) * 0.5f;
+      if (s < 0 || s >= D) {
+#pragma unroll
+        for (int r = 0; r < TY; ++r) Hz[A][r] = 0.0f;
+      }
+    }
+    fetch(s + 2, std::integral_constant<int, B>(), clip_c);
+  };
+  using I0 = std::integral_constant<int, 0>;
+  using I1 = std::integral_constant<int, 1>;
+  using I2 = std::integral_constant<int, 2>;
+  // steps z0-2 .. z1+1, rounded up to whole rotations (the steps past z1+1 load nothing
+  // and compute nothing): one loop exit, so that the waits for a slice's loads never
+  // also wait for the loads issued after it
+  const int rotations = (z1 - z0 + 4 + 2) / 3;
+  auto walk = [&](auto edge_c, auto clip_c) {
+    fetch(z0 - 2, I0(), clip_c);
+    fetch(z0 - 1, I1(), clip_c);
+    for (int i = 0; i < rotations; ++i) {  // uniform
+      const int s = z0 - 2 + 3 * i;
+      step(s, I0(), edge_c, clip_c);
+      step(s + 1, I1(), edge_c, clip_c);
+      step(s + 2, I2(), edge_c, clip_c);
+    }
+  };
+  // interior waves (the common case) run without the border tests
+  if (x0 == 0 || x0 + 64 >= W || y - 2 < 0 || y + TY + 1 >= H)
+    walk(std::true_type(), std::true_type());
+  else
+    walk(std::false_type(), std::false_type());
+  __syncthreads();
+  if (threadIdx.x < kRecNZ && z0 + (int)threadIdx.x * kHessTZ < D)
+    cand_cnt[(((unsigned)gz * kRecNZ + threadIdx.x) * nby + by) * nbx + bx] = s_cnt[threadIdx.x];
+}
+
 }  // namespace
 
 template <typename TD, typename TH>
@@ -337,8 +550,14 @@ hipError_t launch_hessian(const TD* dog, TH* hess, uint8_t* pd, int W, int H, in
   if ((i64)W * H * (i64)sizeof(TD) >= 0xFFFFFFF0ll) return hipErrorInvalidValue;
   // reduced-precision storage anywhere on the path: FP64 re-evaluation band for the minors
   constexpr bool reduced = !std::is_same<TD, float>::value || !std::is_same<TH, float>::value;
-  hipLaunchKernelGGL((k_hessian<TD, TH, reduced>), dim3((unsigned)nblk), dim3(NT), 0, s, dog,
-                     hess, pd, W, H, D, nbx, nby, cand, cand_cnt, band);
+  if (!hess && !pd && cand && cand_cnt) {  // records only
+    const i64 ngroups = (i64)nbx * nby * ((nbz + kRecNZ - 1) / kRecNZ);
+    hipLaunchKernelGGL((k_hessian_rec<TD, TH, reduced>), dim3((unsigned)ngroups),
+                       dim3(64 * kRecWaves), 0, s, dog, W, H, D, nbx, nby, cand, cand_cnt, band);
+  } else {
+    hipLaunchKernelGGL((k_hessian<TD, TH, reduced>), dim3((unsigned)nblk), dim3(NT), 0, s, dog,
+                       hess, pd, W, H, D, nbx, nby, cand, cand_cnt, band);
+  }
   mark("hessian/hessian_pd", s);
   return hipGetLastError();
 }
