@@ -351,7 +351,7 @@ __global__ __launch_bounds__(NT, 5) void k_hessian(const TD* __restrict__ dog, T
 // per SIMD, which measured slower.)
 constexpr int kRecTY = 4;                     // output rows per lane
 constexpr int kRecWaves = kHessTY / kRecTY;   // waves per work-group
-constexpr int kRecNZ = 2;                     // Hessian tiles per work-group, along z
+constexpr int kRecNZ = 4;                     // Hessian tiles per work-group, along z
 static_assert(kHessTX == 64 && kHessTY % kRecTY == 0, "one wave spans a tile row");
 
 // v of lane i - 1 (lane 0 keeps old) and of lane i + 1 (lane 63 keeps old); one DPP move
@@ -446,28 +446,39 @@ __global__ __launch_bounds__(64 * kRecWaves) void k_hessian_rec(
         const float yy = (Hy[C][r + 2] - Hy[C][r]) * 0.5f;
         const float yz = (Hy[B][r + 1] - Hy[A][r + 1]) * 0.5f;
         const float zz = (Hz[B][r] - Hz[A][r]) * 0.5f;
-        const bool m = BAND ? negdef_band(xx, xy, xz, yy, yz, zz, band) != 0
-                            : negdef32(xx, xy, xz, yy, yz, zz) != 0;
-        // voxel 0 is background whatever its mask (its label is 0): no record
-        const bool is_c = m && (!EDGE || xin) && (!CLIP || y + r < H) && (x | (y + r) | z) != 0;
-        // one LDS reservation per wave and row, consecutive slots in lane order
-        const unsigned long long bal = __ballot(is_c);
-        if (bal) {
-          uint32_t base = 0;
-          if (lane == 0) base = atomicAdd(&s_cnt[tz], (uint32_t)__popcll(bal));
-          base = __builtin_amdgcn_readfirstlane(base);
-          if (is_c) {
-            const uint32_t slot = base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
-            const uint32_t cidx = (uint32_t)lane + 64u * ((uint32_t)(ty0 + r) + kHessTY * lz);
-            const int vb = (int)(slot * 4u);
-            const float h6[6] = {round_to<TS>(xx), round_to<TS>(xy), round_to<TS>(xz),
-                                 round_to<TS>(yy), round_to<TS>(yz), round_to<TS>(zz)};
-            __builtin_amdgcn_raw_buffer_store_b32(cidx, rr, vb, 0, 0);
+        bool m;
+        if (BAND) {
+          m = negdef_band(xx, xy, xz, yy, yz, zz, band) != 0;
+        } else {  // negdef32's minors, without branches
+          const float m2 = xx * yy - xy * xy;
+          const float m3 =
+              xx * yy * zz + 2 * xy * yz * xz - xx * yz * yz - yy * xz * xz - zz * xy * xy;
+          m = (xx < 0.0f) & (m2 > 0.0f) & (m3 < 0.0f);
+        }
+        // voxel 0 is background whatever its mask (its label is 0): no record (only an
+        // EDGE and CLIP wave holds it)
+        const bool is_c = m && (!EDGE || xin) && (!CLIP || y + r < H) &&
+                          (!(EDGE && CLIP) || (x | (y + r) | z) != 0);
+        // one LDS reservation per wave and row, consecutive slots in lane order (ds_append
+        // would do both in one instruction, but as compiled for this target it adds to the
+        // counter at LDS address 0 whatever the address asked for)
+        const unsigned long long act = __ballot(is_c);
+        uint32_t base = 0;
+        if (act && lane == 0) base = atomicAdd(&s_cnt[tz], (uint32_t)__popcll(act));
+        base = __builtin_amdgcn_readfirstlane(base);
+        if (is_c) {
+          const uint32_t slot = base + __builtin_amdgcn_mbcnt_hi(
+                                           (uint32_t)(act >> 32),
+                                           __builtin_amdgcn_mbcnt_lo((uint32_t)act, 0u));
+          const uint32_t cidx = (uint32_t)lane + 64u * ((uint32_t)(ty0 + r) + kHessTY * lz);
+          const int vb = (int)(slot * 4u);
+          const float h6[6] = {round_to<TS>(xx), round_to<TS>(xy), round_to<TS>(xz),
+                               round_to<TS>(yy), round_to<TS>(yz), round_to<TS>(zz)};
+          __builtin_amdgcn_raw_buffer_store_b32(cidx, rr, vb, 0, 0);
 #pragma unroll
-            for (int q = 0; q < 6; ++q)
-              __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(h6[q]), rr, vb,
-                                                    (1 + q) * kCandTile * 4, 0);
-          }
+          for (int q = 0; q < 6; ++q)
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(h6[q]), rr, vb,
+                                                  (1 + q) * kCandTile * 4, 0);
         }
       }
     }
