@@ -48,6 +48,7 @@ HBM_PEAK_GBS = 8000.0
 KERNEL_BYTES = {
     "normalize/minmax_z": lambda sn, sd, sh, cf: 2 + 4,
     "normalize/minmax_y": lambda sn, sd, sh, cf: 4 + 4,
+    "normalize/minmax_zy": lambda sn, sd, sh, cf: 2 + 4,
     "normalize/minmax_x_box_x": lambda sn, sd, sh, cf: 4 + 8,
     "normalize/box_y": lambda sn, sd, sh, cf: 8 + 8,
     "normalize/minmax_x_box_xy": lambda sn, sd, sh, cf: 4 + 8,

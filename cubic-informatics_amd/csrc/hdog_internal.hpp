@@ -25,6 +25,9 @@ inline i64 block_count_words(i64 N) {
 }
 // word 3 of a raw buffer descriptor on gfx9-family GPUs (32-bit data format; no swizzle)
 constexpr int kBufferWord3 = 0x00020000;
+// cache-policy operand of a buffer store: nontemporal (streaming), as a nontemporal global
+// store is compiled on gfx950
+constexpr int kCachePolicyNT = 2;
 // Hessian tiles: 64 x 8 outputs per work-group over 16 slices.  Candidate records are kept
 // per tile in a block of kCandTile slots x 7 words (voxel index, then the six elements).
 // The tile height and the region pass's tile group are build-time choices (-DHDOG_HESS_TY,

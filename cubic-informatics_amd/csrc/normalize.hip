@@ -25,6 +25,11 @@ namespace hdog {
 namespace {
 
 constexpr int NT = 256;
+// slices of raw input in flight per thread in the fused y / z min/max walk (a divisor of
+// the z window 2 RZ + 1)
+#ifndef HDOG_ZY_PF
+#define HDOG_ZY_PF 3
+#endif
 typedef unsigned short us2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ uint32_t pkmin(uint32_t a, uint32_t b) {
@@ -332,6 +337,151 @@ __global__ __launch_bounds__(NT) void k_minmax_walk(const void* __restrict__ src
     S[WB - 1] = cur[WB - 1];
 #pragma unroll
     for (int j = WB - 2; j >= 0; --j) S[j] = pkmin(cur[j], S[j + 1]);
+  }
+}
+
+// Min / max along y and z in one pass: a work-group of 16 waves owns a 64-column x 32-row
+// tile of every plane and walks z.  Per slice:
+//   1. the packed rows y0-RY .. y0+31+RY of the slice (clamped to the plane; raw u16
+//      loaded PF slices ahead) enter an LDS buffer (two buffers, alternating: one barrier
+//      per slice);
+//   2. each thread (lane = column, wave = row pair) forms the y window minima of its two
+//      rows from the 2RY+2 values of its column: a shared window middle plus one end value
+//      each;
+//   3. these enter the thread's z walk: van Herk blocks of WB = 2RZ+1 slices over the
+//      clamp-extended line g[k] = slice(clamp(k - RZ)), held in place in a register array:
+//      slot j holds the suffix minimum S[j] of the previous block until slice j of the
+//      current block takes it, and a running prefix minimum P covers the current block,
+//      so output c = min(S[j+1], P) leaves when g[c + 2RZ] arrives; at a block's end its
+//      suffix minima replace it in place and give the output c = k - 2RZ as S[0].
+// Min is exact, so y then z equals the z then y of the unfused passes bit for bit.  The
+// z-min/maxed volume between them (8 B/voxel of traffic) is never stored.
+template <int RZ, int RY, int PF>
+__global__ __launch_bounds__(1024) void k_minmax_zy_walk(const uint16_t* __restrict__ img,
+                                                        uint32_t* __restrict__ dst, int W, int H,
+                                                        int D, int nyt) {
+  constexpr int TY = 32, KR = 2, NWV = TY / KR;  // rows per tile, rows per thread, waves
+  constexpr int SR = TY + 2 * RY;                 // staged rows per slice
+  constexpr int NLD = (SR + NWV - 1) / NWV;       // row loads per lane and slice
+  constexpr int SRP = NLD * NWV;  // rows of an LDS buffer: every wave loads and stores NLD
+                                  // rows (rows past SR repeat a clamped row, unused)
+  constexpr int WB = 2 * RZ + 1;
+  constexpr int NW = 2 * RY + KR;                 // window values per thread
+  static_assert(WB % PF == 0, "queue slots must be static in the unrolled block");
+  static_assert(RY >= 2, "shared window middle");
+  __shared__ uint32_t S[2][SRP * 64];
+  const unsigned t = xcd_tile(blockIdx.x, gridDim.x);
+  const int yt = (int)(t % (unsigned)nyt), xt = (int)(t / (unsigned)nyt);
+  const int lane = (int)(threadIdx.x & 63);
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // wave-uniform
+  const int x = xt * 64 + lane, y0 = yt * TY;
+  const int xc = min(x, W - 1);
+  const size_t WH = (size_t)W * (size_t)H;
+  // staged row wv + NWV j of every slice: its offset in the slice (rows clamped); no
+  // load or LDS store is under a condition
+  uint32_t off[NLD];
+#pragma unroll
+  for (int j = 0; j < NLD; ++j)
+    off[j] = (uint32_t)clampi(y0 - RY + wv + NWV * j, 0, H - 1) * (uint32_t)W + (uint32_t)xc;
+  // loads and stores go through a buffer descriptor of the slice (scalar) with 32-bit
+  // lane offsets: with 64-bit lane addresses the compiler kept one per unrolled step
+  uint32_t q[PF][NLD];  // raw values of slice k wait in slot k % PF
+  auto fetch = [&](int k, int slot) {
+    const __amdgpu_buffer_rsrc_t r =
+        buf_rsrc(img + (size_t)clampi(k - RZ, 0, D - 1) * WH, (uint32_t)(WH * 2));
+#pragma unroll
+    for (int j = 0; j < NLD; ++j)
+      q[slot][j] = __builtin_amdgcn_raw_buffer_load_b16(r, (int)(off[j] * 2u), 0, 0);
+  };
+  // slice k's packed rows into buffer k & 1, the loads of slice k + PF into the freed slot
+  auto stage = [&](int k, int slot) -> const uint32_t* {
+    uint32_t* Sb = S[k & 1];
+#pragma unroll
+    for (int j = 0; j < NLD; ++j)
+      Sb[(wv + NWV * j) * 64 + lane] = pack(q[slot][j]);
+    fetch(k + PF, slot);
+    __syncthreads();
+    return Sb;
+  };
+  const int ty = KR * wv;  // first output row of this thread in the tile
+  // y window minima of rows ty, ty + 1: staged rows ty .. ty + 2RY + 1 of column lane
+  auto ymin = [&](const uint32_t* Sb, uint32_t o[KR]) {
+    uint32_t w[NW];
+#pragma unroll
+    for (int i = 0; i < NW; ++i) w[i] = Sb[(ty + i) * 64 + lane];
+    // middle w[1 .. 2RY], shared by both windows, in four interleaved chains
+    uint32_t m[4] = {w[1], w[2], w[3], w[4]};
+#pragma unroll
+    for (int i = 5; i <= 2 * RY; ++i) m[(i - 1) & 3] = pkmin(m[(i - 1) & 3], w[i]);
+    const uint32_t mid = pkmin(pkmin(m[0], m[1]), pkmin(m[2], m[3]));
+    o[0] = pkmin(mid, w[0]);
+    o[1] = pkmin(mid, w[2 * RY + 1]);
+  };
+  // byte offsets of the thread's outputs in a slice; outside the plane past the slice
+  // descriptor's record count, so those stores are dropped
+  uint32_t ro[KR];
+#pragma unroll
+  for (int i = 0; i < KR; ++i)
+    ro[i] = x < W && y0 + ty + i < H ? ((uint32_t)(y0 + ty + i) * (uint32_t)W + (uint32_t)x) * 4u
+                                     : 0xFFFFFFF0u;
+  auto put = [&](int c, const uint32_t v[KR]) {
+    const __amdgpu_buffer_rsrc_t r = buf_rsrc(dst + (size_t)c * WH, (uint32_t)(WH * 4));
+#pragma unroll
+    for (int i = 0; i < KR; ++i)  // streaming (nontemporal) stores
+      __builtin_amdgcn_raw_buffer_store_b32(v[i], r, (int)ro[i], 0, kCachePolicyNT);
+  };
+  uint32_t zs[KR][WB], P[KR];
+  auto suffix = [&]() {
+#pragma unroll
+    for (int i = 0; i < KR; ++i)
+#pragma unroll
+      for (int j = WB - 2; j >= 0; --j) zs[i][j] = pkmin(zs[i][j], zs[i][j + 1]);
+  };
+#pragma unroll
+  for (int k = 0; k < PF; ++k) fetch(k, k);
+  // block 0 (slices k = 0 .. WB-1) only fills the slots; its suffix minimum S[0] is
+  // output 0
+#pragma unroll
+  for (int j = 0; j < WB; ++j) {
+    const uint32_t* Sb = stage(j, j % PF);
+    uint32_t v[KR];
+    ymin(Sb, v);
+#pragma unroll
+    for (int i = 0; i < KR; ++i) zs[i][j] = v[i];
+  }
+  suffix();
+  {
+    uint32_t v[KR];
+#pragma unroll
+    for (int i = 0; i < KR; ++i) v[i] = zs[i][0];
+    put(0, v);
+  }
+  // output c leaves when slice k = c + 2RZ has arrived; the last is c = D - 1.  The walk's
+  // end leaves the kernel (uniform over the work-group: every wave has the same k).
+  const int kend = D + 2 * RZ;
+  for (int k0 = WB;; k0 += WB) {
+#pragma unroll
+    for (int j = 0; j < WB; ++j) {
+      const int k = k0 + j;
+      if (k >= kend) return;
+      const uint32_t* Sb = stage(k, j % PF);  // (k0 + j) % PF == j % PF
+      uint32_t v[KR], o[KR];
+      ymin(Sb, v);
+#pragma unroll
+      for (int i = 0; i < KR; ++i) {
+        P[i] = j == 0 ? v[i] : pkmin(P[i], v[i]);
+        if (j < WB - 1) o[i] = pkmin(zs[i][j + 1], P[i]);
+        zs[i][j] = v[i];
+      }
+      if (j < WB - 1) {
+        put(k - 2 * RZ, o);
+      } else {
+        suffix();
+#pragma unroll
+        for (int i = 0; i < KR; ++i) o[i] = zs[i][0];
+        put(k - 2 * RZ, o);
+      }
+    }
   }
 }
 
@@ -678,11 +828,21 @@ hipError_t launch_normalize(const uint16_t* img, TS* norm, void* tmp1, void* tmp
   hipError_t e;
   uint32_t* p1 = (uint32_t*)tmp1;
   uint32_t* p2 = (uint32_t*)tmp2;
-  // min/max along z (lines over x, y fixed per work-group), then along y
-  if ((e = minmax_line(img, true, p1, W, D, WH, W, H, rz, s)) != hipSuccess) return e;
-  mark("normalize/minmax_z", s);
-  if ((e = minmax_line(p1, false, p2, W, H, W, WH, D, rxy, s)) != hipSuccess) return e;
-  mark("normalize/minmax_y", s);
+  if (rxy == 10 && rz == 10 && WH * 4 < 0xFFFFFFF0ll) {
+    // min/max along y and z in one z walk over 64 x 32 tiles (slices addressed by buffer
+    // descriptors with 32-bit byte offsets)
+    const int nxt = (W + 63) / 64, nyt = (H + 31) / 32;
+    hipLaunchKernelGGL((k_minmax_zy_walk<10, 10, HDOG_ZY_PF>), dim3((unsigned)((i64)nxt * nyt)),
+                       dim3(1024), 0, s, img, p2, W, H, D, nyt);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    mark("normalize/minmax_zy", s);
+  } else {
+    // min/max along z (lines over x, y fixed per work-group), then along y
+    if ((e = minmax_line(img, true, p1, W, D, WH, W, H, rz, s)) != hipSuccess) return e;
+    mark("normalize/minmax_z", s);
+    if ((e = minmax_line(p1, false, p2, W, H, W, WH, D, rxy, s)) != hipSuccess) return e;
+    mark("normalize/minmax_y", s);
+  }
   if (rxy == 10 && !(ero && dil)) {
     // min/max along x, box along x and box along y in one row walker -> (E, D) in tmp1,
     // then box along z fused with normalization

@@ -61,14 +61,20 @@ def assert_regions_match(g, o, hess):
                                             ((3, 130, 259), 10, 10, 100), ((4, 2, 129), 10, 1, 0),
                                             ((2, 23, 128), 10, 2, 0), ((3, 11, 9), 10, 3, 0),
                                             # tall planes: the row walker's two segments
-                                            ((5, 700, 150), 10, 4, 0), ((3, 341, 70), 10, 2, 0)])
+                                            ((5, 700, 150), 10, 4, 0), ((3, 341, 70), 10, 2, 0),
+                                            # radius 10 in y and z: the fused y / z walk
+                                            # (64 x 32 tiles, blocks of 21 slices)
+                                            ((1, 1, 1), 10, 10, 0), ((45, 70, 65), 10, 10, 0),
+                                            ((22, 1, 130), 10, 10, 0), ((30, 97, 1), 10, 10, 50),
+                                            ((64, 33, 64), 10, 10, 0), ((2, 64, 129), 10, 10, 0)])
 def test_normalize_bitexact(hd, shape, rxy, rz, m):
     img = RNG.integers(0, 4000, size=shape, dtype=np.uint16)
     got = to_np(hd.Normalize3DFilter(img, m, rxy, rz))
     np.testing.assert_array_equal(got, oracle.normalize3d(img, m, rxy, rz))
 
 
-@pytest.mark.parametrize("shape,rxy,rz", [((5, 40, 270), 10, 3), ((7, 30, 66), 3, 2)])
+@pytest.mark.parametrize("shape,rxy,rz", [((5, 40, 270), 10, 3), ((7, 30, 66), 3, 2),
+                                          ((23, 40, 130), 10, 10)])
 def test_normalize_full_u16_range_bitexact(hd, shape, rxy, rz):
     # values at both ends of the u16 range: the packed (v, 65535 - v) min/max and the
     # integer box sums of 21 such values
