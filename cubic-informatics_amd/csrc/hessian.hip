@@ -349,8 +349,16 @@ __global__ __launch_bounds__(NT, 5) void k_hessian(const TD* __restrict__ dog, T
 // Hx(x0+64).  No LDS beyond the record counters, no barrier inside the walk.  (Two halo
 // loads per row, one per shift, save the copies but cost the registers of a fourth wave
 // per SIMD, which measured slower.)
+// A work-group spans kRecTilesY Hessian tiles in y: the dog rows above and below each
+// wave's rows are then mostly read by the same work-group at the same time (its y halo is
+// 4 rows for 8 kRecTilesY output rows); with one tile per work-group the halo rows were
+// read again from HBM by the work-group of the neighbouring tile, which runs at another z.
+#ifndef HDOG_REC_TILES_Y
+#define HDOG_REC_TILES_Y 2
+#endif
 constexpr int kRecTY = 4;                     // output rows per lane
-constexpr int kRecWaves = kHessTY / kRecTY;   // waves per work-group
+constexpr int kRecTilesY = HDOG_REC_TILES_Y;  // Hessian tiles per work-group, along y
+constexpr int kRecWaves = kRecTilesY * kHessTY / kRecTY;  // waves per work-group
 constexpr int kRecNZ = 4;                     // Hessian tiles per work-group, along z
 static_assert(kHessTX == 64 && kHessTY % kRecTY == 0, "one wave spans a tile row");
 
@@ -371,17 +379,20 @@ __device__ __forceinline__ float swap_pairs(float v) {
 
 template <typename TD, typename TS, bool BAND>
 __global__ __launch_bounds__(64 * kRecWaves) void k_hessian_rec(
-    const TD* __restrict__ dog, int W, int H, int D, int nbx, int nby, float* __restrict__ cand,
+    const TD* __restrict__ dog, int W, int H, int D, int nbx, int nby, int nbyg,
+    float* __restrict__ cand,
     uint32_t* __restrict__ cand_cnt, float band) {
   constexpr int TY = kRecTY, NR = TY + 4, NH = TY + 2;  // dog rows, first-difference rows
-  __shared__ uint32_t s_cnt[kRecNZ];
-  if (threadIdx.x < kRecNZ) s_cnt[threadIdx.x] = 0u;
+  __shared__ uint32_t s_cnt[kRecTilesY][kRecNZ];
+  if (threadIdx.x < kRecTilesY * kRecNZ) s_cnt[threadIdx.x / kRecNZ][threadIdx.x % kRecNZ] = 0u;
   const unsigned g = xcd_tile(blockIdx.x, gridDim.x);
-  const int bx = (int)(g % (unsigned)nbx), by = (int)(g / (unsigned)nbx % (unsigned)nby);
-  const int gz = (int)(g / (unsigned)nbx / (unsigned)nby);
+  const int bx = (int)(g % (unsigned)nbx), byg = (int)(g / (unsigned)nbx % (unsigned)nbyg);
+  const int gz = (int)(g / (unsigned)nbx / (unsigned)nbyg);
   const int lane = threadIdx.x & 63;
   // first output row of the wave in the tile (wave-uniform: the row descriptors stay scalar)
-  const int ty0 = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) * TY;
+  const int wy0 = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) * TY;
+  const int tt = wy0 / kHessTY, ty0 = wy0 % kHessTY;  // the wave's tile, its row in the tile
+  const int by = byg * kRecTilesY + tt;  // past nby: every row is below the plane
   const int x0 = bx * 64, x = x0 + lane, y = by * kHessTY + ty0;
   const int z0 = gz * kRecNZ * kHessTZ, z1 = min(z0 + kRecNZ * kHessTZ, D);
   constexpr uint32_t SZ = sizeof(TD);
@@ -464,7 +475,7 @@ __global__ __launch_bounds__(64 * kRecWaves) void k_hessian_rec(
         // counter at LDS address 0 whatever the address asked for)
         const unsigned long long act = __ballot(is_c);
         uint32_t base = 0;
-        if (act && lane == 0) base = atomicAdd(&s_cnt[tz], (uint32_t)__popcll(act));
+        if (act && lane == 0) base = atomicAdd(&s_cnt[tt][tz], (uint32_t)__popcll(act));
         base = __builtin_amdgcn_readfirstlane(base);
         if (is_c) {
           const uint32_t slot = base + __builtin_amdgcn_mbcnt_hi(
@@ -546,8 +557,12 @@ __global__ __launch_bounds__(64 * kRecWaves) void k_hessian_rec(
   else
     walk(std::false_type(), std::false_type());
   __syncthreads();
-  if (threadIdx.x < kRecNZ && z0 + (int)threadIdx.x * kHessTZ < D)
-    cand_cnt[(((unsigned)gz * kRecNZ + threadIdx.x) * nby + by) * nbx + bx] = s_cnt[threadIdx.x];
+  if (threadIdx.x < kRecTilesY * kRecNZ) {
+    const int ct = (int)threadIdx.x / kRecNZ, cz = (int)threadIdx.x % kRecNZ;
+    const int cby = byg * kRecTilesY + ct;
+    if (cby < nby && z0 + cz * kHessTZ < D)
+      cand_cnt[(((unsigned)gz * kRecNZ + cz) * nby + cby) * nbx + bx] = s_cnt[ct][cz];
+  }
 }
 
 }  // namespace
@@ -562,9 +577,11 @@ hipError_t launch_hessian(const TD* dog, TH* hess, uint8_t* pd, int W, int H, in
   // reduced-precision storage anywhere on the path: FP64 re-evaluation band for the minors
   constexpr bool reduced = !std::is_same<TD, float>::value || !std::is_same<TH, float>::value;
   if (!hess && !pd && cand && cand_cnt) {  // records only
-    const i64 ngroups = (i64)nbx * nby * ((nbz + kRecNZ - 1) / kRecNZ);
+    const int nbyg = (nby + kRecTilesY - 1) / kRecTilesY;
+    const i64 ngroups = (i64)nbx * nbyg * ((nbz + kRecNZ - 1) / kRecNZ);
     hipLaunchKernelGGL((k_hessian_rec<TD, TH, reduced>), dim3((unsigned)ngroups),
-                       dim3(64 * kRecWaves), 0, s, dog, W, H, D, nbx, nby, cand, cand_cnt, band);
+                       dim3(64 * kRecWaves), 0, s, dog, W, H, D, nbx, nby, nbyg, cand, cand_cnt,
+                       band);
   } else {
     hipLaunchKernelGGL((k_hessian<TD, TH, reduced>), dim3((unsigned)nblk), dim3(NT), 0, s, dog,
                        hess, pd, W, H, D, nbx, nby, cand, cand_cnt, band);
