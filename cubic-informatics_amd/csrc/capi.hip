@@ -20,15 +20,34 @@ struct Buf {
   size_t bytes = 0;
 };
 
-// A captured batch of resident stacks: it owns every buffer its graph points at.
+// One branch of a batch graph: the scratch of one stack at a time, sized for the largest
+// stack of the branch.  The separable passes' pair buffers (tmp1, tmp2: 16 bytes per voxel)
+// live inside the candidate-record block (28 bytes per voxel of tile-padded address space):
+// they are dead before the Hessian pass writes records, and the next stack of the branch
+// writes them only after the previous stack's region pass has read its records.
+struct BatchLane {
+  Buf norm, dog, labels, rid, bcnt, rmask, rmulti, dR, acc, cand, cand_cnt, fgbits, cand_lab;
+  void* tmp1 = nullptr;
+  void* tmp2 = nullptr;
+  i64 maxN = 0, maxT = 0, maxF = 0, acc_cap = 0;
+  hipStream_t s = nullptr;  // capture stream of the branch
+  std::vector<int> stacks;  // in launch order
+};
+
+// A captured batch of resident stacks: it owns every buffer its graph points at.  Stacks
+// are dealt longest first to the branch with fewer voxels so far; the branches are
+// independent chains of the graph, so one stack's labeling and region pass (latency-bound)
+// run beside another stack's filters (bandwidth- and issue-bound).
 struct BatchState {
   std::vector<hdog_tile> tiles;
   hdog_params p{};
-  Buf tmp1, tmp2, norm, dog, pd, labels, rid, bcnt, rmask, rmulti, dR, acc, weights, tout,
-      running, cand, cand_cnt, fgbits, cand_lab;
+  std::vector<BatchLane> lanes;
+  Buf weights, tout, running;
   Buf a_label, a_size, a_vox, a_cen, a_eig, a_nrm, a_hess, a_isum, a_imax;
-  i64 acc_cap = 0, arena_cap = 0;
+  i64 acc_cap = 0, arena_cap = 0;  // acc_cap: the largest of the branches'
   hipStream_t cs = nullptr;
+  hipEvent_t fork = nullptr;
+  std::vector<hipEvent_t> joins;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   TileOut* h_tout = nullptr;  // pinned copy of the per-stack placements
@@ -408,13 +427,19 @@ void batch_free(BatchState* B) {
   if (!B) return;
   if (B->exec) (void)hipGraphExecDestroy(B->exec);
   if (B->graph) (void)hipGraphDestroy(B->graph);
+  if (B->fork) (void)hipEventDestroy(B->fork);
+  for (hipEvent_t e : B->joins)
+    if (e) (void)hipEventDestroy(e);  // (a null handle would leave an error for the next call)
+  for (BatchLane& L : B->lanes) {
+    if (L.s && L.s != B->cs) (void)hipStreamDestroy(L.s);
+    for (Buf* b : {&L.norm, &L.dog, &L.labels, &L.rid, &L.bcnt, &L.rmask, &L.rmulti, &L.dR,
+                   &L.acc, &L.cand, &L.cand_cnt, &L.fgbits, &L.cand_lab})
+      release(*b);
+  }
   if (B->cs) (void)hipStreamDestroy(B->cs);
   if (B->h_tout) (void)hipHostFree(B->h_tout);
-  for (Buf* b : {&B->tmp1, &B->tmp2, &B->norm, &B->dog, &B->pd, &B->labels, &B->rid, &B->rmask, &B->rmulti,
-                 &B->cand, &B->cand_cnt, &B->cand_lab, &B->fgbits,
-                 &B->bcnt, &B->dR, &B->acc, &B->weights, &B->tout, &B->running, &B->a_label,
-                 &B->a_size, &B->a_vox, &B->a_cen, &B->a_eig, &B->a_nrm, &B->a_hess, &B->a_isum,
-                 &B->a_imax})
+  for (Buf* b : {&B->weights, &B->tout, &B->running, &B->a_label, &B->a_size, &B->a_vox,
+                 &B->a_cen, &B->a_eig, &B->a_nrm, &B->a_hess, &B->a_isum, &B->a_imax})
     release(*b);
   delete B;
 }
@@ -422,54 +447,53 @@ void batch_free(BatchState* B) {
 // Enqueues the whole pipeline of tile t on stream s with device-side region counts: no
 // host synchronization, allocation or memset, so it can be captured.
 template <typename TN, typename TD, typename TH>
-hipError_t batch_tile(hdog_ctx* ctx, BatchState* B, int t, hipStream_t s) {
+hipError_t batch_tile(hdog_ctx* ctx, BatchState* B, BatchLane& L, int t, hipStream_t s) {
   const hdog_tile& T = B->tiles[t];
   const int W = T.width, H = T.height, D = T.depth;
   const i64 N = (i64)W * H * D;
   const hdog_params* p = &B->p;
-  TN* norm = (TN*)B->norm.p;
-  TD* dog = (TD*)B->dog.p;
+  TN* norm = (TN*)L.norm.p;
+  TD* dog = (TD*)L.dog.p;
   TH* hess = nullptr;  // not materialized: the region pass reads the candidate records
-  uint32_t* labels = (uint32_t*)B->labels.p;  // parent array, candidate voxels only
+  uint32_t* labels = (uint32_t*)L.labels.p;  // parent array, candidate voxels only
   hipError_t e;
-  if ((e = launch_normalize<TN>(T.image, norm, B->tmp1.p, B->tmp2.p, p->min_intensity, W, H, D,
+  if ((e = launch_normalize<TN>(T.image, norm, L.tmp1, L.tmp2, p->min_intensity, W, H, D,
                                 p->radius_xy, p->radius_z, s, nullptr, nullptr,
                                 tuned_norm(ctx, W, H, D, p->precision))) != hipSuccess)
     return e;
   const float nz = powf(ctx->sigma[0], p->gamma * 2) * powf(ctx->sigma[1], p->gamma);
-  if ((e = launch_dog<TN, TD>(norm, dog, B->tmp1.p, B->tmp2.p, (const float*)B->weights.p, ctx->r,
-                              nz, W, H, D, s, tuned_gauss(ctx, W, H, D, p->precision))) !=
-      hipSuccess)
+  if ((e = launch_dog<TN, TD>(norm, dog, L.tmp1, L.tmp2, (const float*)B->weights.p, ctx->r, nz, W,
+                              H, D, s, tuned_gauss(ctx, W, H, D, p->precision))) != hipSuccess)
     return e;
   const i64 ntiles = hessian_tiles(W, H, D);
-  if ((e = launch_hessian<TD, TH>(dog, hess, nullptr, W, H, D, s, (float*)B->cand.p,
-                                  (uint32_t*)B->cand_cnt.p, minor_band(&B->p))) != hipSuccess)
+  if ((e = launch_hessian<TD, TH>(dog, hess, nullptr, W, H, D, s, (float*)L.cand.p,
+                                  (uint32_t*)L.cand_cnt.p, minor_band(&B->p))) != hipSuccess)
     return e;
   i64 nbo = 0;
-  uint32_t* dR = (uint32_t*)B->dR.p;
-  unsigned long long* masks = (unsigned long long*)B->rmask.p;
+  uint32_t* dR = (uint32_t*)L.dR.p;
+  unsigned long long* masks = (unsigned long long*)L.rmask.p;
   const i64 nb = (N + kRootBlock - 1) / kRootBlock;
-  if ((e = launch_ccl_candidates((const float*)B->cand.p, (const uint32_t*)B->cand_cnt.p, ntiles,
-                                 (unsigned long long*)B->fgbits.p, labels,
-                                 (uint32_t*)B->cand_lab.p, masks,
-                                 (unsigned long long*)B->rmulti.p, nb * (kRootBlock / 64), false,
+  if ((e = launch_ccl_candidates((const float*)L.cand.p, (const uint32_t*)L.cand_cnt.p, ntiles,
+                                 (unsigned long long*)L.fgbits.p, labels,
+                                 (uint32_t*)L.cand_lab.p, masks,
+                                 (unsigned long long*)L.rmulti.p, nb * (kRootBlock / 64), false,
                                  W, H, D, s)) != hipSuccess)
     return e;
-  if ((e = launch_region_count_masks(masks, (uint32_t*)B->bcnt.p, dR, N, s, &nbo)) != hipSuccess)
+  if ((e = launch_region_count_masks(masks, (uint32_t*)L.bcnt.p, dR, N, s, &nbo)) != hipSuccess)
     return e;
   TileOut* tout = (TileOut*)B->tout.p + t;
-  if ((e = launch_tile_reserve(dR, B->acc_cap, B->arena_cap, (u64*)B->running.p, tout, s)) !=
+  if ((e = launch_tile_reserve(dR, L.acc_cap, B->arena_cap, (u64*)B->running.p, tout, s)) !=
       hipSuccess)
     return e;
-  RegionAcc* acc = (RegionAcc*)B->acc.p;
-  if ((e = launch_region_assign(masks, (const uint32_t*)B->bcnt.p, (uint32_t*)B->rid.p, acc,
-                                B->acc_cap, N, s)) != hipSuccess)
+  RegionAcc* acc = (RegionAcc*)L.acc.p;
+  if ((e = launch_region_assign(masks, (const uint32_t*)L.bcnt.p, (uint32_t*)L.rid.p, acc,
+                                L.acc_cap, N, s)) != hipSuccess)
     return e;
-  if ((e = launch_region_accum_cand<TN>((const float*)B->cand.p, (const uint32_t*)B->cand_cnt.p,
-                                        ntiles, (const uint32_t*)B->cand_lab.p,
-                                        (const unsigned long long*)B->rmulti.p,
-                                        (const uint32_t*)B->rid.p, norm, T.image,
-                                        acc, B->acc_cap, W, H, p->is_ave_mode, s)) != hipSuccess)
+  if ((e = launch_region_accum_cand<TN>((const float*)L.cand.p, (const uint32_t*)L.cand_cnt.p,
+                                        ntiles, (const uint32_t*)L.cand_lab.p,
+                                        (const unsigned long long*)L.rmulti.p,
+                                        (const uint32_t*)L.rid.p, norm, T.image,
+                                        acc, L.acc_cap, W, H, p->is_ave_mode, s)) != hipSuccess)
     return e;
   const i64 cap = B->arena_cap;
   RegionOut o;
@@ -485,26 +509,36 @@ hipError_t batch_tile(hdog_ctx* ctx, BatchState* B, int t, hipStream_t s) {
   o.hess = (float*)B->a_hess.p;
   o.isum = (u64*)B->a_isum.p;
   o.imax = (uint16_t*)B->a_imax.p;
-  return launch_region_finalize(acc, B->acc_cap, p->is_ave_mode, o, cap, s, tout);
+  return launch_region_finalize(acc, L.acc_cap, p->is_ave_mode, o, cap, s, tout);
 }
 
+// Captures on B->cs: the arena counter is cleared, the other branches fork from it through
+// an event, each branch enqueues its stacks in order, and B->cs joins them at the end.
 template <typename TN, typename TD, typename TH>
-hipError_t batch_capture(hdog_ctx* ctx, BatchState* B, hipStream_t s) {
-  hipError_t e = launch_fill_u64((u64*)B->running.p, 1, 0, s);
-  for (int t = 0; e == hipSuccess && t < (int)B->tiles.size(); ++t)
-    e = batch_tile<TN, TD, TH>(ctx, B, t, s);
+hipError_t batch_capture(hdog_ctx* ctx, BatchState* B) {
+  hipError_t e = launch_fill_u64((u64*)B->running.p, 1, 0, B->cs);
+  if (e == hipSuccess) e = hipEventRecord(B->fork, B->cs);
+  for (size_t l = 1; e == hipSuccess && l < B->lanes.size(); ++l)
+    e = hipStreamWaitEvent(B->lanes[l].s, B->fork, 0);
+  for (size_t l = 0; l < B->lanes.size(); ++l)
+    for (int t : B->lanes[l].stacks)
+      if (e == hipSuccess) e = batch_tile<TN, TD, TH>(ctx, B, B->lanes[l], t, B->lanes[l].s);
+  for (size_t l = 1; l < B->lanes.size(); ++l) {
+    if (e == hipSuccess) e = hipEventRecord(B->joins[l], B->lanes[l].s);
+    if (e == hipSuccess) e = hipStreamWaitEvent(B->cs, B->joins[l], 0);
+  }
   return e;
 }
 
-hipError_t batch_capture_any(hdog_ctx* ctx, BatchState* B, hipStream_t s) {
+hipError_t batch_capture_any(hdog_ctx* ctx, BatchState* B) {
   typedef __half h16;
   typedef __hip_bfloat16 b16;
   switch (B->p.precision) {
-    case HDOG_FP16: return batch_capture<h16, h16, h16>(ctx, B, s);
-    case HDOG_BF16: return batch_capture<b16, b16, b16>(ctx, B, s);
-    case HDOG_FP16_DOG32: return batch_capture<h16, float, h16>(ctx, B, s);
-    case HDOG_BF16_DOG32: return batch_capture<b16, float, b16>(ctx, B, s);
-    default: return batch_capture<float, float, float>(ctx, B, s);
+    case HDOG_FP16: return batch_capture<h16, h16, h16>(ctx, B);
+    case HDOG_BF16: return batch_capture<b16, b16, b16>(ctx, B);
+    case HDOG_FP16_DOG32: return batch_capture<h16, float, h16>(ctx, B);
+    case HDOG_BF16_DOG32: return batch_capture<b16, float, b16>(ctx, B);
+    default: return batch_capture<float, float, float>(ctx, B);
   }
 }
 
@@ -1184,7 +1218,8 @@ int hdog_batch_plan(hdog_ctx* ctx, const hdog_params* p, const hdog_tile* tiles,
     return fail(ctx, HDOG_ERR_ARG, "negative capacity");
   HIPCHK(hipSetDevice(ctx->device));
   hdog_batch_release(ctx);
-  i64 maxN = 0, totN = 0, maxT = 0, maxF = 0;
+  std::vector<i64> Ns(n_tiles);
+  i64 totN = 0;
   for (int t = 0; t < n_tiles; ++t) {
     hdog_params q = *p;
     q.width = tiles[t].width;
@@ -1193,47 +1228,87 @@ int hdog_batch_plan(hdog_ctx* ctx, const hdog_params* p, const hdog_tile* tiles,
     int rc = check_params(ctx, &q);
     if (rc) return fail(ctx, rc, "stack %d: %s", t, std::string(ctx->err).c_str());
     if (!tiles[t].image) return fail(ctx, HDOG_ERR_ARG, "stack %d: NULL image", t);
-    const i64 N = (i64)q.width * q.height * q.depth;
-    maxN = std::max(maxN, N);
-    maxT = std::max(maxT, hessian_tiles(q.width, q.height, q.depth));
-    maxF = std::max(maxF, fg_words(q.width, q.height, q.depth));
-    totN += N;
+    Ns[t] = (i64)q.width * q.height * q.depth;
+    totN += Ns[t];
   }
   int rc = ensure_gauss(ctx, p);
   if (rc) return rc;
+  const size_t es = p->precision == HDOG_FP32 ? 4 : 2;
+  const size_t ed = (p->precision == HDOG_FP16 || p->precision == HDOG_BF16) ? 2 : 4;
+  // longest first to the branch with fewer voxels so far
+  std::vector<int> order(n_tiles);
+  for (int t = 0; t < n_tiles; ++t) order[t] = t;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return Ns[a] > Ns[b]; });
+  auto deal = [&](int nl) {
+    std::vector<BatchLane> L(nl);
+    std::vector<i64> load(nl, 0);
+    for (int t : order) {
+      const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      load[l] += Ns[t];
+      BatchLane& b = L[l];
+      b.stacks.push_back(t);
+      b.maxN = std::max(b.maxN, Ns[t]);
+      b.maxT = std::max(b.maxT, hessian_tiles(tiles[t].width, tiles[t].height, tiles[t].depth));
+      b.maxF = std::max(b.maxF, fg_words(tiles[t].width, tiles[t].height, tiles[t].depth));
+    }
+    for (BatchLane& b : L)
+      b.acc_cap = std::min<i64>(max_regions_per_stack > 0 ? max_regions_per_stack + 1
+                                                          : b.maxN / 64 + 2,
+                                0xFFFFFFFFll);
+    return L;
+  };
+  // the pair buffers' place in the record block (second one 256-byte aligned)
+  auto tmp2_off = [](const BatchLane& b) { return ((size_t)b.maxN * 8 + 255) / 256 * 256; };
+  auto lane_bytes = [&](const BatchLane& b) -> std::vector<std::pair<Buf*, size_t>> {
+    BatchLane& m = const_cast<BatchLane&>(b);
+    const i64 nb = (b.maxN + kRootBlock - 1) / kRootBlock;
+    return {{&m.norm, (size_t)b.maxN * es}, {&m.dog, (size_t)b.maxN * ed},
+            {&m.labels, (size_t)b.maxN * 4}, {&m.rid, (size_t)b.maxN * 4},
+            {&m.bcnt, (size_t)block_count_words(b.maxN) * 4},
+            {&m.rmask, (size_t)nb * (kRootBlock / 8)}, {&m.rmulti, (size_t)nb * (kRootBlock / 8)},
+            {&m.dR, 64}, {&m.acc, (size_t)b.acc_cap * sizeof(RegionAcc)},
+            {&m.cand, std::max((size_t)b.maxT * kCandTile * 7 * 4, tmp2_off(b) + (size_t)b.maxN * 8)},
+            {&m.cand_cnt, (size_t)b.maxT * 4}, {&m.cand_lab, (size_t)b.maxT * kCandTile * 4},
+            {&m.fgbits, (size_t)b.maxF * 8}};
+  };
   BatchState* B = new BatchState;
   B->tiles.assign(tiles, tiles + n_tiles);
   B->p = *p;
-  B->acc_cap = max_regions_per_stack > 0 ? max_regions_per_stack + 1 : maxN / 64 + 2;
   B->arena_cap = arena_capacity > 0 ? arena_capacity : totN / 128 + n_tiles;
-  if (B->acc_cap > (i64)0xFFFFFFFFll) B->acc_cap = 0xFFFFFFFFll;
-  const size_t es = p->precision == HDOG_FP32 ? 4 : 2;
-  const size_t ed = (p->precision == HDOG_FP16 || p->precision == HDOG_BF16) ? 2 : 4;
   const i64 cap = B->arena_cap;
-  const struct {
-    Buf* b;
-    size_t bytes;
-  } need[] = {{&B->tmp1, (size_t)maxN * 8}, {&B->tmp2, (size_t)maxN * 8},
-              {&B->norm, maxN * es}, {&B->dog, maxN * ed},
-              {&B->labels, (size_t)maxN * 4}, {&B->rid, (size_t)maxN * 4},
-              {&B->bcnt, (size_t)block_count_words(maxN) * 4},
-              {&B->rmask, (size_t)((maxN + kRootBlock - 1) / kRootBlock) * (kRootBlock / 8)},
-              {&B->rmulti, (size_t)((maxN + kRootBlock - 1) / kRootBlock) * (kRootBlock / 8)},
-              {&B->dR, 64},
-              {&B->acc, (size_t)B->acc_cap * sizeof(RegionAcc)}, {&B->weights, ctx->weights.bytes},
-              {&B->tout, (size_t)n_tiles * sizeof(TileOut)}, {&B->running, 64},
-              {&B->a_label, (size_t)cap * 4}, {&B->a_size, (size_t)cap * 2},
-              {&B->a_vox, (size_t)cap * 8}, {&B->a_cen, (size_t)cap * 12},
-              {&B->a_eig, (size_t)cap * 8}, {&B->a_nrm, (size_t)cap * 4},
-              {&B->a_hess, (size_t)cap * 24}, {&B->a_isum, (size_t)cap * 8},
-              {&B->a_imax, (size_t)cap * 2},
-              {&B->cand, (size_t)maxT * kCandTile * 7 * 4}, {&B->cand_cnt, (size_t)maxT * 4},
-              {&B->cand_lab, (size_t)maxT * kCandTile * 4}, {&B->fgbits, (size_t)maxF * 8}};
+  // two branches when their scratch fits in 90 % of the free device memory, else one
+  size_t shared_bytes = (size_t)cap * 70 + ctx->weights.bytes + (size_t)n_tiles * sizeof(TileOut);
+  size_t free_b = 0, total_b = 0;
+  (void)hipMemGetInfo(&free_b, &total_b);
+  B->lanes = deal(n_tiles >= 2 ? 2 : 1);
+  if (B->lanes.size() > 1) {
+    size_t need = shared_bytes;
+    for (const BatchLane& b : B->lanes)
+      for (const auto& n : lane_bytes(b)) need += n.second;
+    if ((double)need > 0.9 * (double)free_b) B->lanes = deal(1);
+  }
   auto bad = [&](hipError_t e, const char* what) {
     batch_free(B);
     return fail(ctx, HDOG_ERR_HIP, "batch plan: %s failed: %s", what, hipGetErrorString(e));
   };
   hipError_t e;
+  for (BatchLane& b : B->lanes) {
+    B->acc_cap = std::max(B->acc_cap, b.acc_cap);  // for messages
+    for (const auto& n : lane_bytes(b))
+      if ((e = ensure(*n.first, n.second)) != hipSuccess) return bad(e, "device allocation");
+    b.tmp1 = b.cand.p;
+    b.tmp2 = (char*)b.cand.p + tmp2_off(b);
+  }
+  const struct {
+    Buf* b;
+    size_t bytes;
+  } need[] = {{&B->weights, ctx->weights.bytes},
+              {&B->tout, (size_t)n_tiles * sizeof(TileOut)}, {&B->running, 64},
+              {&B->a_label, (size_t)cap * 4}, {&B->a_size, (size_t)cap * 2},
+              {&B->a_vox, (size_t)cap * 8}, {&B->a_cen, (size_t)cap * 12},
+              {&B->a_eig, (size_t)cap * 8}, {&B->a_nrm, (size_t)cap * 4},
+              {&B->a_hess, (size_t)cap * 24}, {&B->a_isum, (size_t)cap * 8},
+              {&B->a_imax, (size_t)cap * 2}};
   for (const auto& n : need)
     if ((e = ensure(*n.b, n.bytes)) != hipSuccess) return bad(e, "device allocation");
   if ((e = hipHostMalloc((void**)&B->h_tout, n_tiles * sizeof(TileOut), 0)) != hipSuccess)
@@ -1243,9 +1318,19 @@ int hdog_batch_plan(hdog_ctx* ctx, const hdog_params* p, const hdog_tile* tiles,
     return bad(e, "weight copy");
   if ((e = hipStreamCreateWithFlags(&B->cs, hipStreamNonBlocking)) != hipSuccess)
     return bad(e, "hipStreamCreate");
+  if ((e = hipEventCreateWithFlags(&B->fork, hipEventDisableTiming)) != hipSuccess)
+    return bad(e, "hipEventCreate");
+  B->joins.assign(B->lanes.size(), nullptr);
+  B->lanes[0].s = B->cs;
+  for (size_t l = 1; l < B->lanes.size(); ++l) {
+    if ((e = hipStreamCreateWithFlags(&B->lanes[l].s, hipStreamNonBlocking)) != hipSuccess)
+      return bad(e, "hipStreamCreate");
+    if ((e = hipEventCreateWithFlags(&B->joins[l], hipEventDisableTiming)) != hipSuccess)
+      return bad(e, "hipEventCreate");
+  }
   if ((e = hipStreamBeginCapture(B->cs, hipStreamCaptureModeThreadLocal)) != hipSuccess)
     return bad(e, "hipStreamBeginCapture");
-  const hipError_t ec = batch_capture_any(ctx, B, B->cs);
+  const hipError_t ec = batch_capture_any(ctx, B);
   e = hipStreamEndCapture(B->cs, &B->graph);
   if (ec != hipSuccess) return bad(ec, "kernel launch during capture");
   if (e != hipSuccess) return bad(e, "hipStreamEndCapture");

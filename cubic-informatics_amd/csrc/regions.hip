@@ -521,17 +521,26 @@ __global__ __launch_bounds__(NT) void k_finalize(const RegionAcc* __restrict__ a
   }
 }
 
+// The arena counter is advanced by compare-and-swap: stacks of a batch may reserve their
+// ranges concurrently (the branches of a batch graph).
 __global__ void k_tile_reserve(const uint32_t* __restrict__ d_R, i64 acc_cap, i64 arena_cap,
                                u64* running, TileOut* t) {
   if (threadIdx.x != 0) return;
   const i64 R = (i64)*d_R;
-  const u64 off = *running;
-  const bool over = R > acc_cap || (i64)off + (R - 1) > arena_cap;
+  u64 off = __hip_atomic_load(running, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  bool over;
+  for (;;) {
+    over = R > acc_cap || (i64)off + (R - 1) > arena_cap;
+    if (over) break;
+    const u64 seen = atomicCAS((unsigned long long*)running, (unsigned long long)off,
+                               (unsigned long long)(off + (u64)(R - 1)));
+    if (seen == off) break;
+    off = seen;  // another stack reserved first: retry from its end
+  }
   t->offset = off;
   t->count = over ? 0 : (u64)(R - 1);
   t->overflow = over ? 1u : 0u;
   t->found = (uint32_t)(R - 1);
-  if (!over) *running = off + (u64)(R - 1);
 }
 
 __global__ __launch_bounds__(NT) void k_fill_u64(u64* p, i64 n, u64 v) {

@@ -221,10 +221,14 @@ int hdog_autotune(hdog_ctx* ctx, void* stream, const hdog_params* p, const uint1
  * hipGraph launch.  Replaces the per-stack fan-out of script/HDoG_gpu.py:67-104, which
  * starts one HDoG3D_main process per stack and runs each as depth-32 substacks.
  *
- * hdog_batch_plan allocates every buffer for the largest stack, a per-stack region
- * accumulator of max_regions_per_stack entries (0: largest N / 64) and a region arena of
- * arena_capacity records shared by all stacks (0: total N / 128), and captures the
- * pipeline of p (sizes ignored, precision honoured) for every tile into one graph.  The
+ * hdog_batch_plan deals the stacks, longest first, to two branches of the graph (one when
+ * the second branch's scratch does not fit in device memory); a branch runs its stacks in
+ * order on buffers sized for its largest stack, with a region accumulator of
+ * max_regions_per_stack entries (0: its largest N / 64), and the branches run
+ * concurrently, so one stack's labeling overlaps another stack's filters.  A region arena
+ * of arena_capacity records (0: total N / 128) is shared by all stacks in the order they
+ * reserve it.  The pipeline of p (sizes ignored, precision honoured) of every tile is
+ * captured into one graph.  The
  * tiles' images must stay valid and unchanged in shape while the plan exists; their
  * contents may change between runs.  hdog_batch_run launches the graph on `stream`,
  * waits for it and returns HDOG_ERR_CAPACITY (naming the stack) when a stack's regions
