@@ -11,16 +11,22 @@
 //   * slices are read into page-locked buffers by several threads, one substack ahead,
 //     and each substack's host-to-device copy runs on a copy stream while the previous
 //     substack's kernels run (the reference reads, copies and computes serially,
-//     src/3D/HDoG3D_main.cpp:155-170);
+//     src/3D/HDoG3D_main.cpp:155-170); the substacks of all stacks form one pipeline, so
+//     the next stack's first substack is read while the last one of a stack runs, and the
+//     records are written by a writer thread while the next substack runs;
 //   * a substack step depth - 2*margin <= 0 is rejected (the reference loops forever);
 //   * a missing parameter key exits with code 2 and a message (the reference aborts);
 //   * optional HDoG_param.truncate / HDoG_param.precision keys.
 // Exit codes: 0 ok, 2 usage / parameter / file error, 1 device error.
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <iostream>
+#include <mutex>
+#include <sstream>
 #include <string>
 #include <thread>
 #include <vector>
@@ -119,6 +125,17 @@ int main(int argc, char** argv) {
   hp.precision = precision_code(p.precision);
 
   const size_t plane = (size_t)p.width * p.height;
+  // every substack of every stack, in order: one pipeline over the whole parameter file
+  struct Item {
+    size_t is;
+    int z0, d;
+  };
+  std::vector<Item> items;
+  for (size_t is = 0; is < p.src_paths.size(); ++is) {
+    const int L = (int)p.src_paths[is].size();
+    for (int z0 = 0; z0 < L - p.depth_margin; z0 += step)
+      items.push_back({is, z0, std::min(p.depth, L - z0)});
+  }
   // three page-locked substack buffers: one being read from disk, one being copied to the
   // device, one whose copy the running substack consumed
   uint16_t* buf[3] = {nullptr, nullptr, nullptr};
@@ -129,117 +146,199 @@ int main(int argc, char** argv) {
       hdog_destroy(ctx);
       return 1;
     }
-  const unsigned nthreads = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
-  RegionHost reg;
+  const unsigned nthreads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
   int rc = 0;
 
-  for (size_t is = 0; is < p.src_paths.size() && rc == 0; ++is) {
-    const auto& src = p.src_paths[is];
-    const int L = (int)src.size();
-    if (std::remove(p.dst_paths[is].c_str()) == 0)
-      std::cout << "removed previous result " << is << std::endl;
-    else
-      std::cout << "newly create result " << is << std::endl;
-    std::cout << "length: " << L << std::endl;
-
-    std::vector<int> z0s;
-    for (int z0 = 0; z0 < L - p.depth_margin; z0 += step) z0s.push_back(z0);
-    const size_t nz = z0s.size();
-
-    // loader: reads the slices of substack k into buf[k % 3] on `nthreads` threads
-    struct Load {
-      std::string err;
-      double took = 0;
-      std::thread th;
-    };
-    auto load = [&](size_t k, Load* ld) {
-      auto t = Clock::now();
-      const int z0 = z0s[k];
-      const int d = std::min(p.depth, L - z0);
-      uint16_t* dst = buf[k % 3];
-      std::vector<std::string> errs(nthreads);
-      std::vector<std::thread> pool;
-      for (unsigned w = 0; w < nthreads; ++w)
-        pool.emplace_back([&, w] {
-          for (int i = (int)w; i < d; i += (int)nthreads)
-            if (!load_slice(src[z0 + i], dst + plane * i, plane, errs[w])) return;
-        });
-      for (auto& th : pool) th.join();
-      for (auto& e : errs)
-        if (!e.empty() && ld->err.empty()) ld->err = e;
-      ld->took = ms_since(t);
-    };
-    auto upload = [&](size_t k) {
-      const int d = std::min(p.depth, L - z0s[k]);
-      return hdog_substack_upload(ctx, &hp, buf[k % 3], d, (int)(k & 1));
-    };
-    Load cur, next;
-    if (nz > 0) {
-      load(0, &cur);
-      if (cur.err.empty() && upload(0) != HDOG_OK) cur.err = hdog_last_error(ctx);
+  // writer: announces each stack (its result file removed) and appends the records of
+  // each substack, in order, on its own thread, so the next substack's kernels run
+  // meanwhile.  At most two substacks' regions wait for it.
+  struct Job {
+    size_t is = 0;
+    int z0 = 0, d = 0;
+    int64_t R = -1;  // -1: announce stack `is`
+    RegionHost reg;
+  };
+  std::mutex mu;
+  std::condition_variable cv;
+  std::deque<Job> queue;
+  std::vector<RegionHost> spare;  // region arrays returned by the writer
+  bool closing = false, write_failed = false;
+  std::thread writer([&] {
+    for (;;) {
+      Job j;
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return closing || !queue.empty(); });
+        if (queue.empty()) return;
+        j = std::move(queue.front());
+        queue.pop_front();
+      }
+      std::ostringstream out;
+      if (j.R < 0) {
+        if (std::remove(p.dst_paths[j.is].c_str()) == 0)
+          out << "removed previous result " << j.is << "\n";
+        else
+          out << "newly create result " << j.is << "\n";
+        out << "length: " << p.src_paths[j.is].size() << "\n";
+      } else {
+        auto t = Clock::now();
+        bool ok = save_features(p.dst_paths[j.is], j.R, j.reg.cx.data(), j.reg.cy.data(),
+                                j.reg.cz.data(), j.reg.st.data(), j.reg.bl.data(),
+                                j.reg.mx.data(), j.reg.sz.data(), j.d, j.z0, p) >= 0;
+        if (!ok) out << "Unable to open " << p.dst_paths[j.is] << "\n";
+        if (j.d == p.depth)
+          out << "[save] middle of the stack : " << j.z0 + p.depth_margin << " - "
+              << j.z0 + j.d - p.depth_margin - 1 << " (" << j.d - 2 * p.depth_margin << ")\n";
+        else
+          out << "[save] end of the stack : " << j.z0 + p.depth_margin << " - " << j.z0 + j.d - 1
+              << " (" << j.d - p.depth_margin << ")\n";
+        out << "saveImage took " << ms_since(t) << "msec\n";
+        std::lock_guard<std::mutex> lk(mu);
+        if (!ok) write_failed = true;
+        spare.push_back(std::move(j.reg));
+      }
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        std::cout << out.str() << std::flush;
+      }
+      cv.notify_all();
     }
-    if (nz > 1) next.th = std::thread(load, 1, &next);
-    for (size_t k = 0; k < nz; ++k) {
-      if (!cur.err.empty()) {
-        std::cout << cur.err << std::endl;
-        rc = 2;
-        break;
-      }
-      const int z0 = z0s[k];
-      const int d = std::min(p.depth, L - z0);
-      std::cout << "----------" << std::endl;
-      std::cout << "processSubstack(z0=" << z0 << ", depth=" << d << ")" << std::endl;
-      std::cout << "\tfirst_src_path: " << src[z0] << std::endl;
-      std::cout << "\tlast_src_path: " << src[z0 + d - 1] << std::endl;
-      std::cout << "loadImage took " << cur.took << "msec" << std::endl;
-      // substack k+1: its slices are read (joined here) and its copy starts before k runs,
-      // so the copy overlaps k's kernels; substack k+2 is read meanwhile
-      Load after;
-      if (k + 1 < nz) {
-        next.th.join();
-        if (next.err.empty() && upload(k + 1) != HDOG_OK) next.err = hdog_last_error(ctx);
-        if (k + 2 < nz) after.th = std::thread(load, k + 2, &after);
-      }
-      auto t = Clock::now();
-      // the first download is a capacity probe that returns the region count
-      hdog_regions probe{};
-      int e = hdog_substack_run(ctx, nullptr, &hp, (int)(k & 1), &probe);
-      if (e == HDOG_ERR_CAPACITY || e == HDOG_OK) {
-        hdog_regions v = reg.view(probe.count);
-        e = hdog_regions_download(ctx, nullptr, &v);
-      }
-      if (e != HDOG_OK) {
-        std::cout << "HDoG failed: " << hdog_last_error(ctx) << std::endl;
-        rc = e == HDOG_ERR_ARG ? 2 : 1;
-        if (after.th.joinable()) after.th.join();
-        break;
-      }
-      const int64_t R = probe.count;
-      std::cout << "num_regions: " << R << std::endl;
-      std::cout << "GPU memcpy & kernel execution took " << ms_since(t) << "msec" << std::endl;
-      t = Clock::now();
-      if (save_features(p.dst_paths[is], R, reg.cx.data(), reg.cy.data(), reg.cz.data(),
-                        reg.st.data(), reg.bl.data(), reg.mx.data(), reg.sz.data(), d, z0, p) < 0) {
-        std::cout << "Unable to open " << p.dst_paths[is] << std::endl;
-        rc = 2;
-      }
-      if (d == p.depth)
-        std::cout << "[save] middle of the stack : " << z0 + p.depth_margin << " - "
-                  << z0 + d - p.depth_margin - 1 << " (" << d - 2 * p.depth_margin << ")"
-                  << std::endl;
-      else
-        std::cout << "[save] end of the stack : " << z0 + p.depth_margin << " - " << z0 + d - 1
-                  << " (" << d - p.depth_margin << ")" << std::endl;
-      std::cout << "saveImage took " << ms_since(t) << "msec" << std::endl;
-      cur.err = std::move(next.err);
-      cur.took = next.took;
-      next.err = std::move(after.err);
-      next.took = after.took;
-      next.th = std::move(after.th);
-      if (rc) break;
+  });
+  auto post = [&](Job&& j) {
+    std::unique_lock<std::mutex> lk(mu);
+    cv.wait(lk, [&] {
+      size_t saves = 0;
+      for (const Job& q : queue) saves += q.R >= 0;
+      return saves < 2;
+    });
+    queue.push_back(std::move(j));
+    cv.notify_all();
+  };
+  size_t announced = 0;  // stacks announced to the writer so far
+  auto announce_upto = [&](size_t is) {
+    for (; announced <= is && announced < p.src_paths.size(); ++announced) {
+      Job j;
+      j.is = announced;
+      post(std::move(j));
     }
-    if (next.th.joinable()) next.th.join();
+  };
+
+  // loader: reads the slices of item k into buf[k % 3] on `nthreads` threads
+  struct Load {
+    std::string err;
+    double took = 0;
+    std::thread th;
+  };
+  auto load = [&](size_t k, Load* ld) {
+    auto t = Clock::now();
+    const Item& it = items[k];
+    const auto& src = p.src_paths[it.is];
+    uint16_t* dst = buf[k % 3];
+    std::vector<std::string> errs(nthreads);
+    std::vector<std::thread> pool;
+    for (unsigned w = 0; w < nthreads; ++w)
+      pool.emplace_back([&, w] {
+        for (int i = (int)w; i < it.d; i += (int)nthreads)
+          if (!load_slice(src[it.z0 + i], dst + plane * i, plane, errs[w])) return;
+      });
+    for (auto& th : pool) th.join();
+    for (auto& e : errs)
+      if (!e.empty() && ld->err.empty()) ld->err = e;
+    ld->took = ms_since(t);
+  };
+  auto upload = [&](size_t k) {
+    return hdog_substack_upload(ctx, &hp, buf[k % 3], items[k].d, (int)(k & 1));
+  };
+  auto say = [&](const std::string& s) {
+    std::lock_guard<std::mutex> lk(mu);
+    std::cout << s << std::flush;
+  };
+  const size_t nz = items.size();
+  Load cur, next;
+  if (nz > 0) {
+    load(0, &cur);
+    if (cur.err.empty() && upload(0) != HDOG_OK) cur.err = hdog_last_error(ctx);
   }
+  if (nz > 1) next.th = std::thread(load, 1, &next);
+  for (size_t k = 0; k < nz; ++k) {
+    const Item& it = items[k];
+    announce_upto(it.is);
+    if (!cur.err.empty()) {
+      say(cur.err + "\n");
+      rc = 2;
+      break;
+    }
+    const auto& src = p.src_paths[it.is];
+    {
+      std::ostringstream out;
+      out << "----------\n"
+          << "processSubstack(z0=" << it.z0 << ", depth=" << it.d << ")\n"
+          << "\tfirst_src_path: " << src[it.z0] << "\n"
+          << "\tlast_src_path: " << src[it.z0 + it.d - 1] << "\n"
+          << "loadImage took " << cur.took << "msec\n";
+      say(out.str());
+    }
+    // item k+1: its slices are read (joined here) and its copy starts before k runs, so
+    // the copy overlaps k's kernels; item k+2 is read meanwhile
+    Load after;
+    if (k + 1 < nz) {
+      next.th.join();
+      if (next.err.empty() && upload(k + 1) != HDOG_OK) next.err = hdog_last_error(ctx);
+      if (k + 2 < nz) after.th = std::thread(load, k + 2, &after);
+    }
+    auto t = Clock::now();
+    Job j;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      if (!spare.empty()) {
+        j.reg = std::move(spare.back());
+        spare.pop_back();
+      }
+    }
+    // the first download is a capacity probe that returns the region count
+    hdog_regions probe{};
+    int e = hdog_substack_run(ctx, nullptr, &hp, (int)(k & 1), &probe);
+    if (e == HDOG_ERR_CAPACITY || e == HDOG_OK) {
+      hdog_regions v = j.reg.view(probe.count);
+      e = hdog_regions_download(ctx, nullptr, &v);
+    }
+    if (e != HDOG_OK) {
+      say(std::string("HDoG failed: ") + hdog_last_error(ctx) + "\n");
+      rc = e == HDOG_ERR_ARG ? 2 : 1;
+      if (after.th.joinable()) after.th.join();
+      break;
+    }
+    {
+      std::ostringstream out;
+      out << "num_regions: " << probe.count << "\n"
+          << "GPU memcpy & kernel execution took " << ms_since(t) << "msec\n";
+      say(out.str());
+    }
+    j.is = it.is;
+    j.z0 = it.z0;
+    j.d = it.d;
+    j.R = probe.count;
+    post(std::move(j));
+    cur.err = std::move(next.err);
+    cur.took = next.took;
+    next.err = std::move(after.err);
+    next.took = after.took;
+    next.th = std::move(after.th);
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      if (write_failed) rc = 2;
+    }
+    if (rc) break;
+  }
+  if (next.th.joinable()) next.th.join();
+  if (rc == 0) announce_upto(p.src_paths.size());  // stacks too short for any substack
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    closing = true;
+  }
+  cv.notify_all();
+  writer.join();
+  if (write_failed && rc == 0) rc = 2;
   for (auto& b : buf) hdog_host_free(ctx, b);
   hdog_destroy(ctx);
   return rc;

@@ -8,10 +8,14 @@ OUTDIR, writes a parameter file (radius 10, DoG sigma1 = (z 1.6, xy 2.0), sigma2
 sigma1, truncation 4, margins 100) and runs the host program on it, timing the whole run
 from the process start (slice reads, H2D, GPU work, D2H, record writes).
 
+With --stacks N the parameter file lists the same slices N times (N result files), as a
+parameter file of N stacks of one brain would: the program then runs one pipeline over
+all of them (the next stack's slices are read while the previous one runs).
+
 Prints one JSON line: wall seconds, stack voxels per second, the number of substacks and
 the host program's per-substack timings.
 
-    python e2e_stack.py OUTDIR [--depth 512] [--slices 512] [--exe PATH]
+    python e2e_stack.py OUTDIR [--depth 512] [--slices 512] [--stacks 1] [--exe PATH]
 """
 import argparse
 import json
@@ -32,6 +36,7 @@ def main():
     ap.add_argument("--depth", type=int, default=512, help="HDoG_param.depth (substack depth)")
     ap.add_argument("--height", type=int, default=2160)
     ap.add_argument("--width", type=int, default=2560)
+    ap.add_argument("--stacks", type=int, default=1, help="times the stack is listed")
     ap.add_argument("--exe", default=os.path.join(HERE, "HDoG3D_main"))
     ap.add_argument("--keep", action="store_true", help="reuse slices already in OUTDIR")
     args = ap.parse_args()
@@ -67,7 +72,8 @@ def main():
                        "gamma_n": 1.0, "min_size": 0, "truncate": 4.0,
                        "sigma_dog": {"xy_small": 2.0, "z_small": 1.6, "xy_large": 3.2,
                                      "z_large": 2.56}},
-        "stacks": [{"src_paths": paths, "dst_path": os.path.join(args.outdir, "cells.bin")}],
+        "stacks": [{"src_paths": paths, "dst_path": os.path.join(args.outdir, f"cells_{k}.bin")}
+                   for k in range(args.stacks)],
     }
     pfile = os.path.join(args.outdir, "param.json")
     with open(pfile, "w") as f:
@@ -82,9 +88,10 @@ def main():
 
     def col(pat):
         return [float(x) for x in re.findall(pat, out)]
-    n_vox = shape[0] * shape[1] * shape[2]
+    n_vox = shape[0] * shape[1] * shape[2] * args.stacks
     res = {
-        "stack_zyx": list(shape), "depth": args.depth, "wall_s": wall,
+        "stack_zyx": list(shape), "stacks": args.stacks, "depth": args.depth, "wall_s": wall,
+        "init_ms": col(r"Initialization took ([0-9.]+)msec"),
         "gvoxel_per_s": n_vox / wall / 1e9,
         "substacks": len(col(r"processSubstack\(z0=(\d+)")),
         "load_ms": col(r"loadImage took ([0-9.]+)msec"),

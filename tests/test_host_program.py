@@ -168,3 +168,44 @@ def test_host_program_end_to_end(tmp_path, L, depth, truncate):
         np.testing.assert_array_equal(got[k], exp[k], err_msg=k)
     for k in ("structureness", "blobness"):
         np.testing.assert_allclose(got[k], exp[k], rtol=1e-5, err_msg=k)
+
+
+@pytest.mark.gpu
+def test_host_program_several_stacks(tmp_path):
+    # the substacks of all stacks run as one pipeline (the next stack is read while the
+    # previous one runs, records are written by a writer thread): every stack's file holds
+    # exactly its own records, and a stack too short for any substack has its stale result
+    # removed and none written, as in the reference's loop
+    sig, truncate = (1.5, 1.5, 2.4, 2.4), 2.0
+    lengths = (40, 6, 29)
+    vols, stacks = [], []
+    for k, L in enumerate(lengths):
+        if L > 10:
+            vol = synth.make_volume("c0", shape=(L, 48, 64), n_blobs=25, seed=100 + k)
+        else:  # too short for blobs away from the edges, and for any substack
+            vol = np.random.default_rng(k).integers(150, 250, (L, 48, 64)).astype(np.uint16)
+        paths = synth.write_stack(vol, os.path.join(str(tmp_path), f"src{k}"))
+        dst = os.path.join(str(tmp_path), "out", f"stack{k}.bin")
+        os.makedirs(os.path.dirname(dst), exist_ok=True)
+        open(dst, "wb").write(b"stale")
+        vols.append(vol)
+        stacks.append({"src_paths": paths, "dst_path": dst})
+    param = make_param(tmp_path, stacks, depth=24, truncate=truncate, sigmas=sig, min_size=2)
+    r = run([param])
+    assert r.returncode == 0, r.stdout + r.stderr
+    rz2 = oracle.gaussian_radius(sig[3], truncate)
+    P = dict(depth=24, margin=rz2 + 1, left=2, right=2, top=2, bottom=2, min_size=2,
+             p=dict(min_intensity=0, rxy=3, rz=2, sigma_xy1=sig[0], sigma_z1=sig[1],
+                    sigma_xy2=sig[2], sigma_z2=sig[3], truncate=truncate, gamma=1.0))
+    for k, vol in enumerate(vols):
+        dst = stacks[k]["dst_path"]
+        exp = expected_records(vol, P)
+        if len(substack_starts(len(vol), 24, P["margin"])) == 0:
+            assert not os.path.exists(dst), f"stack {k}: stale result kept"
+            continue
+        got = np.fromfile(dst, dtype=DT_LOCAL)
+        assert len(got) == len(exp) and len(exp) > 0, (k, len(got), len(exp))
+        for key in ("local_x", "local_y", "local_z", "intensity", "size", "padding"):
+            np.testing.assert_array_equal(got[key], exp[key], err_msg=f"stack {k}: {key}")
+        for key in ("structureness", "blobness"):
+            np.testing.assert_allclose(got[key], exp[key], rtol=1e-5, err_msg=f"stack {k}: {key}")
