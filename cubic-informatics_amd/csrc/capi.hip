@@ -310,14 +310,29 @@ int regional_features_t(hdog_ctx* ctx, hipStream_t s, uint32_t* labels, const TH
 
 // pd and hess may be NULL (not materialized); labels is the union-find's parent array and
 // becomes the full label volume only with `dense`.
+// The separable passes' pair buffers (8 bytes per voxel each) of the whole-path call and
+// the tuner: they live in the candidate-record block (28 bytes per voxel of tile-padded
+// address space), being dead before the Hessian pass writes the first record.
+hipError_t pair_buffers(hdog_ctx* ctx, int W, int H, int D, void** tmp1, void** tmp2) {
+  const i64 N = (i64)W * H * D;
+  const size_t off2 = ((size_t)N * 8 + 255) / 256 * 256;
+  const hipError_t e = ensure(ctx->cand, std::max((size_t)hessian_tiles(W, H, D) * kCandTile * 7 * 4,
+                                                  off2 + (size_t)N * 8));
+  *tmp1 = ctx->cand.p;
+  *tmp2 = (char*)ctx->cand.p + off2;
+  return e;
+}
+
 template <typename TN, typename TD, typename TH>
 int run_stages(hdog_ctx* ctx, hipStream_t s, const hdog_params* p, const uint16_t* img, TN* norm,
                TD* dog, TH* hess, uint8_t* pd, uint32_t* labels, bool dense,
                int64_t* num_regions) {
   const int W = p->width, H = p->height, D = p->depth;
   const i64 N = (i64)W * H * D;
-  HIPCHK(ensure(ctx->tmp1, (size_t)N * 8));
-  HIPCHK(ensure(ctx->tmp2, (size_t)N * 8));
+  CandRecords cand;
+  cand.ntiles = hessian_tiles(W, H, D);
+  void *tmp1, *tmp2;
+  HIPCHK(pair_buffers(ctx, W, H, D, &tmp1, &tmp2));
   struct ProfScope {  // points the launchers' marks at this context's pool for one call
     explicit ProfScope(Prof* p) { tl_prof = p; }
     ~ProfScope() { tl_prof = nullptr; }
@@ -325,19 +340,16 @@ int run_stages(hdog_ctx* ctx, hipStream_t s, const hdog_params* p, const uint16_
   ctx->ev_valid = false;
   ctx->pool.n = 0;
   mark(nullptr, s);
-  HIPCHK(launch_normalize<TN>(img, norm, ctx->tmp1.p, ctx->tmp2.p, p->min_intensity, W, H, D,
-                              p->radius_xy, p->radius_z, s, nullptr, nullptr,
+  HIPCHK(launch_normalize<TN>(img, norm, tmp1, tmp2, p->min_intensity, W, H, D, p->radius_xy,
+                              p->radius_z, s, nullptr, nullptr,
                               tuned_norm(ctx, W, H, D, p->precision)));
   const float nz = powf(ctx->sigma[0], p->gamma * 2) * powf(ctx->sigma[1], p->gamma);
-  HIPCHK(launch_dog<TN, TD>(norm, dog, ctx->tmp1.p, ctx->tmp2.p, (const float*)ctx->weights.p,
-                            ctx->r, nz, W, H, D, s, tuned_gauss(ctx, W, H, D, p->precision)));
+  HIPCHK(launch_dog<TN, TD>(norm, dog, tmp1, tmp2, (const float*)ctx->weights.p, ctx->r, nz, W,
+                            H, D, s, tuned_gauss(ctx, W, H, D, p->precision)));
   // the Hessian pass also leaves compact candidate records and the candidate bit mask;
   // labeling and the region pass work from those, not from the mask and element volumes
   // (written only when pd / hess are non-NULL)
-  CandRecords cand;
-  cand.ntiles = hessian_tiles(W, H, D);
   const i64 nb = (N + kRootBlock - 1) / kRootBlock;
-  HIPCHK(ensure(ctx->cand, (size_t)cand.ntiles * kCandTile * 7 * 4));
   HIPCHK(ensure(ctx->cand_cnt, (size_t)cand.ntiles * 4));
   HIPCHK(ensure(ctx->cand_lab, (size_t)cand.ntiles * kCandTile * 4));
   HIPCHK(ensure(ctx->fgbits, (size_t)fg_words(W, H, D) * 8));
@@ -577,8 +589,8 @@ int autotune_t(hdog_ctx* ctx, hipStream_t s, const hdog_params* p, const uint16_
                int* best_norm, int* best_gauss, float* ms_norm, float* ms_gauss) {
   const int W = p->width, H = p->height, D = p->depth;
   const i64 N = (i64)W * H * D;
-  HIPCHK(ensure(ctx->tmp1, (size_t)N * 8));
-  HIPCHK(ensure(ctx->tmp2, (size_t)N * 8));
+  void *tmp1, *tmp2;
+  HIPCHK(pair_buffers(ctx, W, H, D, &tmp1, &tmp2));
   HIPCHK(ensure(ctx->norm, (size_t)N * sizeof(TN)));
   HIPCHK(ensure(ctx->dog, (size_t)N * sizeof(TD)));
   TN* norm = (TN*)ctx->norm.p;
@@ -605,12 +617,12 @@ int autotune_t(hdog_ctx* ctx, hipStream_t s, const hdog_params* p, const uint16_
       float ms = 0.0f;
       if (st.dog)
         HIPCHK(time_median(s, [&] {
-          return launch_dog<TN, TD>(norm, dog, ctx->tmp1.p, ctx->tmp2.p,
+          return launch_dog<TN, TD>(norm, dog, tmp1, tmp2,
                                     (const float*)ctx->weights.p, ctx->r, nz, W, H, D, s, want);
         }, &ms));
       else
         HIPCHK(time_median(s, [&] {
-          return launch_normalize<TN>(img, norm, ctx->tmp1.p, ctx->tmp2.p, p->min_intensity, W, H,
+          return launch_normalize<TN>(img, norm, tmp1, tmp2, p->min_intensity, W, H,
                                       D, p->radius_xy, p->radius_z, s, nullptr, nullptr, want);
         }, &ms));
       if (ms < *st.ms) {
@@ -619,7 +631,7 @@ int autotune_t(hdog_ctx* ctx, hipStream_t s, const hdog_params* p, const uint16_
       }
     }
     if (!st.dog)  // the DoG stage is timed on the normalized stack
-      HIPCHK(launch_normalize<TN>(img, norm, ctx->tmp1.p, ctx->tmp2.p, p->min_intensity, W, H, D,
+      HIPCHK(launch_normalize<TN>(img, norm, tmp1, tmp2, p->min_intensity, W, H, D,
                                   p->radius_xy, p->radius_z, s, nullptr, nullptr, *st.best));
   }
   return HDOG_OK;
