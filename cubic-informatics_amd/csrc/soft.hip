@@ -42,6 +42,28 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Adds the work-group's sums of a[k] to out[k] (K <= 2) with one FP64 atomic each: the
+// reductions run on a grid of at most kReduceGroups work-groups, so a handful of atomics
+// meet at one address (one atomic per wave over a grid sized for the volume serialized
+// hundreds of thousands of them there).
+constexpr int kReduceGroups = 1024;
+template <int K>
+__device__ __forceinline__ void group_add(double (&a)[K], double* out) {
+  __shared__ double part[NT / 64][K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) a[k] = wave_sum(a[k]);
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) part[threadIdx.x >> 6][k] = a[k];
+  __syncthreads();
+  if (threadIdx.x < K) {
+    double t = 0.0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) t += part[w][threadIdx.x];
+    unsafeAtomicAdd(out + threadIdx.x, t);
+  }
+}
+
 __device__ __forceinline__ double det3(double xx, double xy, double xz, double yy, double yz,
                                        double zz) {
   return xx * yy * zz + 2 * xy * yz * xz - xx * yz * yz - yy * xz * xz - zz * xy * xy;
@@ -65,12 +87,8 @@ __global__ __launch_bounds__(NT) void k_soft_loss(const float* __restrict__ hess
     ls += softplus(u) - (target[i] ? u : 0.0);
     cnt += 1.0;
   }
-  ls = wave_sum(ls);
-  cnt = wave_sum(cnt);
-  if ((threadIdx.x & 63) == 0) {
-    unsafeAtomicAdd(sums, ls);
-    unsafeAtomicAdd(sums + 1, cnt);
-  }
+  double a[2] = {ls, cnt};
+  group_add<2>(a, sums);
 }
 
 // hess (6N) is overwritten by dL/dH; sums[0] += dL/dtau contributions
@@ -101,8 +119,8 @@ __global__ __launch_bounds__(NT) void k_soft_grad_h(float* __restrict__ hess,
     hess[4 * N + i] = (float)(gd * 2.0 * (xy * xz - xx * yz));
     hess[5 * N + i] = (float)(gd * (xx * yy - xy * xy));
   }
-  gt = wave_sum(gt);
-  if ((threadIdx.x & 63) == 0) unsafeAtomicAdd(sums, gt);
+  double a[1] = {gt};
+  group_add<1>(a, sums);
 }
 
 // zero-padded central difference of field f along axis `a` at (x, y, z)
@@ -153,12 +171,8 @@ __global__ __launch_bounds__(NT) void k_dot_pair(const float* __restrict__ a,
     s0 += v * p.x;
     s1 += v * p.y;
   }
-  s0 = wave_sum(s0);
-  s1 = wave_sum(s1);
-  if ((threadIdx.x & 63) == 0) {
-    unsafeAtomicAdd(out, s0);
-    unsafeAtomicAdd(out + 1, s1);
-  }
+  double t[2] = {s0, s1};
+  group_add<2>(t, out);
 }
 
 __global__ __launch_bounds__(NT) void k_dot(const float* __restrict__ a,
@@ -166,8 +180,8 @@ __global__ __launch_bounds__(NT) void k_dot(const float* __restrict__ a,
   double s = 0.0;
   for (i64 i = (i64)blockIdx.x * NT + threadIdx.x; i < N; i += (i64)gridDim.x * NT)
     s += (double)a[i] * b[i];
-  s = wave_sum(s);
-  if ((threadIdx.x & 63) == 0) unsafeAtomicAdd(out, s);
+  double t[1] = {s};
+  group_add<1>(t, out);
 }
 
 __global__ __launch_bounds__(NT) void k_pack_pair(const float* __restrict__ a,
@@ -182,6 +196,9 @@ __global__ void k_zero_f64(double* p, int n) {
 }
 
 unsigned grid_for(i64 N) { return (unsigned)std::min<i64>((N + NT - 1) / NT, 1 << 16); }
+unsigned grid_reduce(i64 N) {
+  return (unsigned)std::max<i64>(1, std::min<i64>((N + NT - 1) / NT, kReduceGroups));
+}
 
 struct Line {  // the line through voxel i along one axis
   i64 base, stride;
@@ -301,15 +318,44 @@ __global__ __launch_bounds__(NT) void k_soft_norm_adj(const float* __restrict__ 
   }
 }
 
-// subgradient routing: gv[argmin(i)] += dL/dmin(i), gv[argmax(i)] += dL/dmax(i)
+// Sum of v over the run of equal consecutive keys (lanes of one wave) that starts at each
+// run's first lane, where *head is set; every lane of the wave must be active.  A segmented
+// suffix sum by lane shifts: after the step of offset o each lane holds the sum over
+// [lane, lane + 2o) within its run.
+__device__ __forceinline__ float run_sum(uint32_t key, float v, bool* head) {
+  const int lane = (int)(threadIdx.x & 63);
+  const uint32_t prev = __shfl_up(key, 1, 64);
+  *head = lane == 0 || prev != key;
+  const unsigned long long heads = __ballot(*head);
+  const unsigned long long after = lane == 63 ? 0ull : heads & (~0ull << (lane + 1));
+  const int tail = after ? __builtin_ctzll(after) - 1 : 63;  // last lane of the run
+  for (int o = 1; o < 64; o <<= 1) {
+    const float w = __shfl_down(v, o, 64);
+    if (lane + o <= tail) v += w;
+  }
+  return v;
+}
+
+// subgradient routing: gv[argmin(i)] += dL/dmin(i), gv[argmax(i)] += dL/dmax(i).  Windows
+// of neighbouring voxels mostly share their extreme, so a wave first sums its lanes' values
+// per run of equal targets and adds each run with one atomic (one atomic per voxel made the
+// contended targets serialize).
 __global__ __launch_bounds__(NT) void k_soft_route(const float2* __restrict__ g,
                                                    const uint32_t* __restrict__ amn,
                                                    const uint32_t* __restrict__ amx,
                                                    float* __restrict__ gv, i64 N) {
-  for (i64 i = (i64)blockIdx.x * NT + threadIdx.x; i < N; i += (i64)gridDim.x * NT) {
-    const float2 q = g[i];
-    if (q.x != 0.0f) atomicAdd(gv + amn[i], q.x);
-    if (q.y != 0.0f) atomicAdd(gv + amx[i], q.y);
+  constexpr uint32_t kNone = 0xFFFFFFFFu;  // lanes past the volume (N < 2^32 - 1)
+  const int lane = (int)(threadIdx.x & 63);
+  for (i64 b = (i64)blockIdx.x * NT + (threadIdx.x - lane); b < N; b += (i64)gridDim.x * NT) {
+    const i64 i = b + lane;  // the whole wave iterates together
+    const bool ok = i < N;
+    const float2 q = ok ? g[i] : make_float2(0.0f, 0.0f);
+    const uint32_t kmn = ok ? amn[i] : kNone, kmx = ok ? amx[i] : kNone;
+    bool hmn, hmx;
+    const float smn = run_sum(kmn, q.x, &hmn);
+    const float smx = run_sum(kmx, q.y, &hmx);
+    if (hmn && kmn != kNone && smn != 0.0f) atomicAdd(gv + kmn, smn);
+    if (hmx && kmx != kNone && smx != 0.0f) atomicAdd(gv + kmx, smx);
   }
 }
 
@@ -380,15 +426,15 @@ hipError_t launch_soft_normalize_adjoint(const float* v, const float2* ed, const
 hipError_t launch_soft_loss(const float* hess, const uint8_t* mask, const uint8_t* target, i64 N,
                             double inv_tau, double* sums, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, s, sums, 2);
-  hipLaunchKernelGGL(k_soft_loss, dim3(grid_for(N)), dim3(NT), 0, s, hess, mask, target, N, inv_tau,
-                     sums);
+  hipLaunchKernelGGL(k_soft_loss, dim3(grid_reduce(N)), dim3(NT), 0, s, hess, mask, target, N,
+                     inv_tau, sums);
   return hipGetLastError();
 }
 
 hipError_t launch_soft_grad_h(float* hess, const uint8_t* mask, const uint8_t* target, i64 N,
                               double inv_tau, double inv_count, double* sums, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, s, sums, 1);
-  hipLaunchKernelGGL(k_soft_grad_h, dim3(grid_for(N)), dim3(NT), 0, s, hess, mask, target, N,
+  hipLaunchKernelGGL(k_soft_grad_h, dim3(grid_reduce(N)), dim3(NT), 0, s, hess, mask, target, N,
                      inv_tau, inv_count, sums);
   return hipGetLastError();
 }
@@ -405,13 +451,13 @@ hipError_t launch_hessian_adjoint(const float* gh, float* P, float* out, int W, 
 
 hipError_t launch_dot_pair(const float* a, const float2* pair, i64 N, double* out, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, s, out, 2);
-  hipLaunchKernelGGL(k_dot_pair, dim3(grid_for(N)), dim3(NT), 0, s, a, pair, N, out);
+  hipLaunchKernelGGL(k_dot_pair, dim3(grid_reduce(N)), dim3(NT), 0, s, a, pair, N, out);
   return hipGetLastError();
 }
 
 hipError_t launch_dot(const float* a, const float* b, i64 N, double* out, hipStream_t s) {
   hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, s, out, 1);
-  hipLaunchKernelGGL(k_dot, dim3(grid_for(N)), dim3(NT), 0, s, a, b, N, out);
+  hipLaunchKernelGGL(k_dot, dim3(grid_reduce(N)), dim3(NT), 0, s, a, b, N, out);
   return hipGetLastError();
 }
 
