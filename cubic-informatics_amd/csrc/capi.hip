@@ -603,8 +603,13 @@ int autotune_t(hdog_ctx* ctx, hipStream_t s, const hdog_params* p, const uint16_
     bool dog;
   } stages[2] = {{21, best_norm, ms_norm, false}, {wb_gauss, best_gauss, ms_gauss, true}};
   for (const Stage& st : stages) {
-    *st.best = 1;
-    *st.ms = 1e30f;
+    // the built-in segment count stays unless another one is faster by more than 3 %: a
+    // choice within the timing noise would change nothing but the launch shape
+    int def_nys, def_rows;
+    walk_segments(H, st.wb, &def_nys, &def_rows);
+    float def_ms = 1e30f;
+    int cand_want[kMaxSegments], ncand = 0;
+    float cand_ms[kMaxSegments];
     int seen[kMaxSegments + 1] = {0};
     int nseen = 0;
     for (int want = 1; want <= kMaxSegments; ++want) {
@@ -625,10 +630,31 @@ int autotune_t(hdog_ctx* ctx, hipStream_t s, const hdog_params* p, const uint16_
           return launch_normalize<TN>(img, norm, tmp1, tmp2, p->min_intensity, W, H,
                                       D, p->radius_xy, p->radius_z, s, nullptr, nullptr, want);
         }, &ms));
-      if (ms < *st.ms) {
-        *st.ms = ms;
-        *st.best = want;
+      cand_want[ncand] = want;
+      cand_ms[ncand++] = ms;
+      if (nys == def_nys) def_ms = ms;
+    }
+    *st.best = 0;
+    *st.ms = def_ms;
+    for (int k = 0; k < ncand; ++k) {
+      int nys, rows;
+      walk_segments(H, st.wb, &nys, &rows, cand_want[k]);
+      if (nys == def_nys) {
+        if (*st.best == 0) *st.best = cand_want[k];  // the default's own entry
+      } else if (cand_ms[k] < 0.97f * def_ms && cand_ms[k] < *st.ms) {
+        *st.ms = cand_ms[k];
+        *st.best = cand_want[k];
       }
+    }
+    // the default is among the candidates (1 or 2 segments); otherwise the fastest
+    if (*st.best == 0) {
+      *st.best = cand_want[0];
+      *st.ms = cand_ms[0];
+      for (int k = 1; k < ncand; ++k)
+        if (cand_ms[k] < *st.ms) {
+          *st.ms = cand_ms[k];
+          *st.best = cand_want[k];
+        }
     }
     if (!st.dog)  // the DoG stage is timed on the normalized stack
       HIPCHK(launch_normalize<TN>(img, norm, tmp1, tmp2, p->min_intensity, W, H, D,
