@@ -1323,7 +1323,16 @@ int hdog_batch_plan(hdog_ctx* ctx, const hdog_params* p, const hdog_tile* tiles,
     size_t need = shared_bytes;
     for (const BatchLane& b : B->lanes)
       for (const auto& n : lane_bytes(b)) need += n.second;
-    if ((double)need > 0.9 * (double)free_b) B->lanes = deal(1);
+    if ((double)need > 0.9 * (double)free_b) {
+      // the per-stack path's working buffers (grown again by its next call; the last
+      // call's region arrays stay) may hold the memory, e.g. after the tuner ran
+      for (Buf* w : {&ctx->tmp1, &ctx->tmp2, &ctx->rid, &ctx->bcnt, &ctx->rmask, &ctx->rmulti,
+                     &ctx->dR, &ctx->acc, &ctx->cand_lab, &ctx->fgbits, &ctx->img, &ctx->norm,
+                     &ctx->dog, &ctx->hess, &ctx->pd, &ctx->labels, &ctx->cand, &ctx->cand_cnt})
+        release(*w);
+      (void)hipMemGetInfo(&free_b, &total_b);
+      if ((double)need > 0.9 * (double)free_b) B->lanes = deal(1);
+    }
   }
   auto bad = [&](hipError_t e, const char* what) {
     batch_free(B);
