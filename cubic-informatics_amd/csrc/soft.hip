@@ -123,6 +123,24 @@ __global__ __launch_bounds__(NT) void k_soft_grad_h(float* __restrict__ hess,
   group_add<1>(a, sums);
 }
 
+// Line kernels run one element per thread on a grid of nxb x-blocks per row (y + H z):
+// x, y, z come from the thread and the block with 32-bit arithmetic, not from divisions
+// of a 64-bit linear index.
+struct Pos {
+  int x, y, z;
+  i64 i;  // linear index
+};
+__device__ __forceinline__ bool pos_of(int W, int H, unsigned nxb, Pos& p) {
+  const unsigned row = blockIdx.x / nxb;
+  p.x = (int)(blockIdx.x - row * nxb) * NT + (int)threadIdx.x;
+  p.y = (int)(row % (unsigned)H);
+  p.z = (int)(row / (unsigned)H);
+  p.i = (i64)row * W + p.x;
+  return p.x < W;
+}
+unsigned xblocks(int W) { return (unsigned)((W + NT - 1) / NT); }
+unsigned grid_rows(int W, int H, int D) { return xblocks(W) * (unsigned)H * (unsigned)D; }
+
 // zero-padded central difference of field f along axis `a` at (x, y, z)
 __device__ __forceinline__ float cdiff(const float* f, int x, int y, int z, int a, int W, int H,
                                        int D, i64 WH) {
@@ -137,10 +155,13 @@ __device__ __forceinline__ float cdiff(const float* f, int x, int y, int z, int 
 }
 
 __global__ __launch_bounds__(NT) void k_hadj1(const float* __restrict__ g, float* __restrict__ P,
-                                              int W, int H, int D) {
+                                              int W, int H, int D, unsigned nxb) {
   const i64 WH = (i64)W * H, N = WH * D;
-  for (i64 i = (i64)blockIdx.x * NT + threadIdx.x; i < N; i += (i64)gridDim.x * NT) {
-    const int x = (int)(i % W), y = (int)((i / W) % H), z = (int)(i / WH);
+  Pos q;
+  if (!pos_of(W, H, nxb, q)) return;
+  {
+    const i64 i = q.i;
+    const int x = q.x, y = q.y, z = q.z;
     const float* gxx = g; const float* gxy = g + N; const float* gxz = g + 2 * N;
     const float* gyy = g + 3 * N; const float* gyz = g + 4 * N; const float* gzz = g + 5 * N;
     P[i] = cdiff(gxx, x, y, z, 0, W, H, D, WH) + cdiff(gxy, x, y, z, 1, W, H, D, WH) +
@@ -151,10 +172,13 @@ __global__ __launch_bounds__(NT) void k_hadj1(const float* __restrict__ g, float
 }
 
 __global__ __launch_bounds__(NT) void k_hadj2(const float* __restrict__ P, float* __restrict__ out,
-                                              int W, int H, int D) {
+                                              int W, int H, int D, unsigned nxb) {
   const i64 WH = (i64)W * H, N = WH * D;
-  for (i64 i = (i64)blockIdx.x * NT + threadIdx.x; i < N; i += (i64)gridDim.x * NT) {
-    const int x = (int)(i % W), y = (int)((i / W) % H), z = (int)(i / WH);
+  Pos q;
+  if (!pos_of(W, H, nxb, q)) return;
+  {
+    const i64 i = q.i;
+    const int x = q.x, y = q.y, z = q.z;
     out[i] = cdiff(P, x, y, z, 0, W, H, D, WH) + cdiff(P + N, x, y, z, 1, W, H, D, WH) +
              cdiff(P + 2 * N, x, y, z, 2, W, H, D, WH);
   }
@@ -200,18 +224,16 @@ unsigned grid_reduce(i64 N) {
   return (unsigned)std::max<i64>(1, std::min<i64>((N + NT - 1) / NT, kReduceGroups));
 }
 
-struct Line {  // the line through voxel i along one axis
+struct Line {  // the line through a voxel along one axis
   i64 base, stride;
   int c, L;
 };
-__device__ __forceinline__ Line line_of(i64 i, int axis, int W, int H, int D) {
-  const i64 WH = (i64)W * H;
-  const int x = (int)(i % W), y = (int)((i / W) % H), z = (int)(i / WH);
+__device__ __forceinline__ Line line_at(const Pos& p, int axis, int W, int H, int D) {
   Line l;
-  if (axis == 0) { l.c = x; l.L = W; l.stride = 1; }
-  else if (axis == 1) { l.c = y; l.L = H; l.stride = W; }
-  else { l.c = z; l.L = D; l.stride = WH; }
-  l.base = i - (i64)l.c * l.stride;
+  if (axis == 0) { l.c = p.x; l.L = W; l.stride = 1; }
+  else if (axis == 1) { l.c = p.y; l.L = H; l.stride = W; }
+  else { l.c = p.z; l.L = D; l.stride = (i64)W * H; }
+  l.base = p.i - (i64)l.c * l.stride;
   return l;
 }
 
@@ -223,10 +245,13 @@ __global__ __launch_bounds__(NT) void k_soft_argminmax(const float* __restrict__
                                                        const uint32_t* __restrict__ imx,
                                                        float* __restrict__ omn, uint32_t* __restrict__ oimn,
                                                        float* __restrict__ omx, uint32_t* __restrict__ oimx,
-                                                       int W, int H, int D, int axis, int r) {
-  const i64 N = (i64)W * H * D;
-  for (i64 i = (i64)blockIdx.x * NT + threadIdx.x; i < N; i += (i64)gridDim.x * NT) {
-    const Line l = line_of(i, axis, W, H, D);
+                                                       int W, int H, int D, int axis, int r,
+                                                       unsigned nxb) {
+  Pos q;
+  if (!pos_of(W, H, nxb, q)) return;
+  const i64 i = q.i;
+  {
+    const Line l = line_at(q, axis, W, H, D);
     float bmn = INFINITY, bmx = -INFINITY;
     uint32_t jmn = 0xFFFFFFFFu, jmx = 0xFFFFFFFFu;
     for (int t = -r; t <= r; ++t) {
@@ -244,11 +269,13 @@ __global__ __launch_bounds__(NT) void k_soft_argminmax(const float* __restrict__
 // clamped box mean of the (E, D) pair along one axis (FP32 sums in window order)
 __global__ __launch_bounds__(NT) void k_soft_box(const float2* __restrict__ in,
                                                  float2* __restrict__ out, int W, int H, int D,
-                                                 int axis, int r) {
-  const i64 N = (i64)W * H * D;
+                                                 int axis, int r, unsigned nxb) {
   const float k = 1.0f / (float)(2 * r + 1);
-  for (i64 i = (i64)blockIdx.x * NT + threadIdx.x; i < N; i += (i64)gridDim.x * NT) {
-    const Line l = line_of(i, axis, W, H, D);
+  Pos q;
+  if (!pos_of(W, H, nxb, q)) return;
+  const i64 i = q.i;
+  {
+    const Line l = line_at(q, axis, W, H, D);
     float se = 0.0f, sd = 0.0f;
     for (int t = -r; t <= r; ++t) {
       const float2 v = in[l.base + (i64)clampi(l.c + t, 0, l.L - 1) * l.stride];
@@ -263,11 +290,13 @@ __global__ __launch_bounds__(NT) void k_soft_box(const float2* __restrict__ in,
 // that lands on it, border positions also the taps that clamp onto them
 __global__ __launch_bounds__(NT) void k_soft_box_adj(const float2* __restrict__ in,
                                                      float2* __restrict__ out, int W, int H,
-                                                     int D, int axis, int r) {
-  const i64 N = (i64)W * H * D;
+                                                     int D, int axis, int r, unsigned nxb) {
   const float k = 1.0f / (float)(2 * r + 1);
-  for (i64 i = (i64)blockIdx.x * NT + threadIdx.x; i < N; i += (i64)gridDim.x * NT) {
-    const Line l = line_of(i, axis, W, H, D);
+  Pos q;
+  if (!pos_of(W, H, nxb, q)) return;
+  const i64 i = q.i;
+  {
+    const Line l = line_at(q, axis, W, H, D);
     const int p = l.c, L = l.L;
     float se = 0.0f, sd = 0.0f;
     for (int c = max(0, p - r); c <= min(L - 1, p + r); ++c) {
@@ -378,9 +407,9 @@ hipError_t launch_soft_normalize(const float* v, float* norm, float2* ed, uint32
     const float* vmx = axis ? src + 2 * N : v;
     const uint32_t* imn = axis ? (const uint32_t*)(src + N) : nullptr;
     const uint32_t* imx = axis ? (const uint32_t*)(src + 3 * N) : nullptr;
-    hipLaunchKernelGGL(k_soft_argminmax, dim3(g), dim3(NT), 0, s, vmn, imn, vmx, imx, dst,
-                       (uint32_t*)(dst + N), dst + 2 * N, (uint32_t*)(dst + 3 * N), W, H, D,
-                       axis, rad[axis]);
+    hipLaunchKernelGGL(k_soft_argminmax, dim3(grid_rows(W, H, D)), dim3(NT), 0, s, vmn, imn, vmx,
+                       imx, dst, (uint32_t*)(dst + N), dst + 2 * N, (uint32_t*)(dst + 3 * N), W,
+                       H, D, axis, rad[axis], xblocks(W));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
@@ -395,8 +424,8 @@ hipError_t launch_soft_normalize(const float* v, float* norm, float2* ed, uint32
   float2* bufs[2] = {p0, p1};
   for (int axis = 0; axis < 3; ++axis) {
     // x: p0 -> ed, y: ed -> p0, z: p0 -> ed
-    hipLaunchKernelGGL(k_soft_box, dim3(g), dim3(NT), 0, s, bufs[axis & 1], bufs[(axis + 1) & 1],
-                       W, H, D, axis, rad[axis]);
+    hipLaunchKernelGGL(k_soft_box, dim3(grid_rows(W, H, D)), dim3(NT), 0, s, bufs[axis & 1],
+                       bufs[(axis + 1) & 1], W, H, D, axis, rad[axis], xblocks(W));
   }
   hipLaunchKernelGGL(k_soft_norm, dim3(g), dim3(NT), 0, s, v, ed, norm, min_int, N);
   return hipGetLastError();
@@ -416,8 +445,8 @@ hipError_t launch_soft_normalize_adjoint(const float* v, const float2* ed, const
   float2* bufs[2] = {q0, q1};
   for (int k = 0; k < 3; ++k) {
     const int axis = 2 - k;
-    hipLaunchKernelGGL(k_soft_box_adj, dim3(g), dim3(NT), 0, s, bufs[k & 1], bufs[(k + 1) & 1], W,
-                       H, D, axis, rad[axis]);
+    hipLaunchKernelGGL(k_soft_box_adj, dim3(grid_rows(W, H, D)), dim3(NT), 0, s, bufs[k & 1],
+                       bufs[(k + 1) & 1], W, H, D, axis, rad[axis], xblocks(W));
   }
   hipLaunchKernelGGL(k_soft_route, dim3(g), dim3(NT), 0, s, bufs[1], arg, arg + N, gv, N);
   return hipGetLastError();
@@ -441,11 +470,12 @@ hipError_t launch_soft_grad_h(float* hess, const uint8_t* mask, const uint8_t* t
 
 hipError_t launch_hessian_adjoint(const float* gh, float* P, float* out, int W, int H, int D,
                                   hipStream_t s) {
-  const i64 N = (i64)W * H * D;
-  hipLaunchKernelGGL(k_hadj1, dim3(grid_for(N)), dim3(NT), 0, s, gh, P, W, H, D);
+  hipLaunchKernelGGL(k_hadj1, dim3(grid_rows(W, H, D)), dim3(NT), 0, s, gh, P, W, H, D,
+                     xblocks(W));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_hadj2, dim3(grid_for(N)), dim3(NT), 0, s, P, out, W, H, D);
+  hipLaunchKernelGGL(k_hadj2, dim3(grid_rows(W, H, D)), dim3(NT), 0, s, P, out, W, H, D,
+                     xblocks(W));
   return hipGetLastError();
 }
 
