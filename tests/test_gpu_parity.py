@@ -305,6 +305,29 @@ def test_unmaterialized_hessian_gives_the_same_regions(hd, precision):
         assert_regions_match(g2, o["regions"], o["hess"])
 
 
+@pytest.mark.parametrize("shape", [(20, 40, 129), (19, 37, 65), (17, 70, 193), (21, 45, 130)])
+def test_records_only_path_odd_widths(hd, shape):
+    # neither the Hessian volumes nor the mask asked for: the records-only Hessian kernel
+    # (interior waves and the border forms, including a last strip one column wider than
+    # 64 (W = 64k + 1) whose right halo column lies outside the plane), labels and regions
+    # against the oracle
+    import torch
+    vol = synth.make_volume("c0", shape=shape, n_blobs=max(8, int(np.prod(shape) / 3000)),
+                            seed=shape[2])
+    p = dict(C0P, truncate=4.0)
+    o = oracle.substack(vol, p)
+    params = hd.make_params(vol.shape, p["rxy"], p["rz"], p["min_intensity"], p["sigma_xy1"],
+                            p["sigma_z1"], p["sigma_xy2"], p["sigma_z2"], p["truncate"],
+                            p["gamma"], False, "fp32")
+    bufs = hd.alloc_intermediates(vol.shape, "fp32")
+    del bufs["hess"], bufs["pd"]
+    R = hd.run_device(torch.from_numpy(vol).to(hd.device), params, bufs)
+    g = hd.download_regions()
+    assert_labels_equal(to_np(bufs["labels"]), o["labels"])
+    assert R == len(o["regions"]["label"]) and R > 1
+    assert_regions_match(g, o["regions"], o["hess"])
+
+
 def test_pipeline_is_deterministic(hd):
     vol = synth.make_volume("c0", shape=(40, 96, 96), n_blobs=150, seed=8)
     p = dict(C0P, truncate=4.0)
