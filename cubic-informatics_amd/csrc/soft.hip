@@ -266,6 +266,93 @@ __global__ __launch_bounds__(NT) void k_soft_argminmax(const float* __restrict__
   }
 }
 
+// The same pass along y or z (second and third passes, indices given) as a column walker
+// for radius R: one thread per line (x-consecutive threads), van Herk blocks of 2R+1 over
+// the clamp-extended line g[k] = f[clamp(k - R)] held in place in registers (slot j keeps
+// the previous block's suffix extreme until the current block's j-th element takes it; a
+// running prefix extreme covers the current block), so every element is read once instead
+// of 2R+1 times.  Output c leaves when g[c + 2R] has arrived.  (value, index) pairs under
+// "smaller value, then smaller index" (min) and "larger value, then smaller index" (max)
+// form total orders, so the blocks give the window extremes exactly.
+struct Ext {
+  float vn, vx;       // min value, max value
+  uint32_t jn, jx;    // their indices
+};
+// (by value: a select between two elements' fields must not become a select between
+// their addresses, which keeps the register arrays in scratch memory)
+__device__ __forceinline__ Ext ext_join(Ext a, Ext b) {
+  const float an = a.vn, bn = b.vn, ax = a.vx, bx = b.vx;
+  const uint32_t ajn = a.jn, bjn = b.jn, ajx = a.jx, bjx = b.jx;
+  const bool mn = an < bn || (an == bn && ajn < bjn);
+  const bool mx = ax > bx || (ax == bx && ajx < bjx);
+  Ext o;
+  o.vn = mn ? an : bn;
+  o.jn = mn ? ajn : bjn;
+  o.vx = mx ? ax : bx;
+  o.jx = mx ? ajx : bjx;
+  return o;
+}
+template <int R>
+__global__ __launch_bounds__(NT) void k_soft_argminmax_walk(
+    const float* __restrict__ vmn, const uint32_t* __restrict__ imn,
+    const float* __restrict__ vmx, const uint32_t* __restrict__ imx, float* __restrict__ omn,
+    uint32_t* __restrict__ oimn, float* __restrict__ omx, uint32_t* __restrict__ oimx, int W,
+    int L, i64 sa, i64 so, unsigned nxb) {
+  constexpr int WB = 2 * R + 1;
+  constexpr int PF = WB % 3 == 0 ? 3 : 1;  // loads queued PF steps ahead (static slots)
+  const int x = (int)(blockIdx.x % nxb) * NT + (int)threadIdx.x;
+  if (x >= W) return;
+  const i64 base = (i64)(blockIdx.x / nxb) * so + x;
+  auto load = [&](int k) -> Ext {
+    const i64 p = base + (i64)clampi(k - R, 0, L - 1) * sa;
+    Ext e;
+    e.vn = vmn[p];
+    e.jn = imn[p];
+    e.vx = vmx[p];
+    e.jx = imx[p];
+    return e;
+  };
+  auto put = [&](int c, const Ext& e) {
+    const i64 p = base + (i64)c * sa;
+    omn[p] = e.vn;
+    oimn[p] = e.jn;
+    omx[p] = e.vx;
+    oimx[p] = e.jx;
+  };
+  Ext slot[WB], q[PF], P;
+#pragma unroll
+  for (int k = 0; k < PF; ++k) q[k] = load(k);
+  // block 0 (k = 0 .. WB-1) fills the slots; its suffix extreme S[0] is output 0
+#pragma unroll
+  for (int j = 0; j < WB; ++j) {
+    slot[j] = q[j % PF];
+    q[j % PF] = load(j + PF);
+  }
+#pragma unroll
+  for (int j = WB - 2; j >= 0; --j) slot[j] = ext_join(slot[j], slot[j + 1]);
+  put(0, slot[0]);
+  const int kend = L + 2 * R;
+  for (int k0 = WB;; k0 += WB) {
+#pragma unroll
+    for (int j = 0; j < WB; ++j) {
+      const int k = k0 + j;
+      if (k >= kend) return;
+      const Ext v = q[j % PF];  // (k0 + j) % PF == j % PF
+      q[j % PF] = load(k + PF);
+      P = j == 0 ? v : ext_join(P, v);
+      if (j < WB - 1) {
+        put(k - 2 * R, ext_join(slot[j + 1], P));
+        slot[j] = v;
+      } else {
+        slot[j] = v;
+#pragma unroll
+        for (int t = WB - 2; t >= 0; --t) slot[t] = ext_join(slot[t], slot[t + 1]);
+        put(k - 2 * R, slot[0]);
+      }
+    }
+  }
+}
+
 // clamped box mean of the (E, D) pair along one axis (FP32 sums in window order)
 __global__ __launch_bounds__(NT) void k_soft_box(const float2* __restrict__ in,
                                                  float2* __restrict__ out, int W, int H, int D,
@@ -407,9 +494,19 @@ hipError_t launch_soft_normalize(const float* v, float* norm, float2* ed, uint32
     const float* vmx = axis ? src + 2 * N : v;
     const uint32_t* imn = axis ? (const uint32_t*)(src + N) : nullptr;
     const uint32_t* imx = axis ? (const uint32_t*)(src + 3 * N) : nullptr;
-    hipLaunchKernelGGL(k_soft_argminmax, dim3(grid_rows(W, H, D)), dim3(NT), 0, s, vmn, imn, vmx,
-                       imx, dst, (uint32_t*)(dst + N), dst + 2 * N, (uint32_t*)(dst + 3 * N), W,
-                       H, D, axis, rad[axis], xblocks(W));
+    if (axis > 0 && rad[axis] == 10) {  // y or z: column walker
+      const i64 WH = (i64)W * H;
+      const int L = axis == 1 ? H : D;
+      const i64 sa = axis == 1 ? W : WH, so = axis == 1 ? WH : W;
+      const i64 n_other = axis == 1 ? D : H;
+      hipLaunchKernelGGL((k_soft_argminmax_walk<10>), dim3((unsigned)(xblocks(W) * n_other)),
+                         dim3(NT), 0, s, vmn, imn, vmx, imx, dst, (uint32_t*)(dst + N), dst + 2 * N,
+                         (uint32_t*)(dst + 3 * N), W, L, sa, so, xblocks(W));
+    } else {
+      hipLaunchKernelGGL(k_soft_argminmax, dim3(grid_rows(W, H, D)), dim3(NT), 0, s, vmn, imn,
+                         vmx, imx, dst, (uint32_t*)(dst + N), dst + 2 * N,
+                         (uint32_t*)(dst + 3 * N), W, H, D, axis, rad[axis], xblocks(W));
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
