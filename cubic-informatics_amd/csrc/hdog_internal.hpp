@@ -212,6 +212,11 @@ hipError_t launch_normalize(const uint16_t* img, TS* norm, void* tmp1, void* tmp
                             float min_intensity, int W, int H, int D, int rxy, int rz,
                             hipStream_t s, float* ero = nullptr, float* dil = nullptr,
                             int segments = 0);
+// Clamped box mean of float2 pairs along y (sa = W, so = W*H, n_other = D) or z (sa = W*H,
+// so = W, n_other = H): the sequential sums c-r .. c+r times 1/(2r+1) of the normalization's
+// y and z passes.
+hipError_t launch_box_pair(const float2* src, float2* dst, int W, int L, i64 sa, i64 so,
+                           i64 n_other, int r, hipStream_t s);
 // DoG stage; weights: device array [4][kMaxTaps] (xy1, z1, xy2, z2); tmp1/tmp2 >= 8*N bytes;
 // segments as for launch_normalize
 template <typename TN, typename TD>

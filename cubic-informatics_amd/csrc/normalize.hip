@@ -813,6 +813,11 @@ __global__ __launch_bounds__(NT) void k_split_ed(const float2* __restrict__ ed,
 
 }  // namespace
 
+hipError_t launch_box_pair(const float2* src, float2* dst, int W, int L, i64 sa, i64 so,
+                           i64 n_other, int r, hipStream_t s) {
+  return box_line<false, float>(src, dst, nullptr, nullptr, 0.0f, W, L, sa, so, n_other, r, s);
+}
+
 template <typename TS>
 hipError_t launch_normalize(const uint16_t* img, TS* norm, void* tmp1, void* tmp2,
                             float min_intensity, int W, int H, int D, int rxy, int rz,

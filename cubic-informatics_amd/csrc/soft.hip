@@ -407,6 +407,53 @@ __global__ __launch_bounds__(NT) void k_soft_box_adj(const float2* __restrict__ 
   }
 }
 
+// The transpose along y or z for radius R as a column walker: a register ring of the
+// 2R+1 inputs around each position (zero outside the line), unrolled by the ring period;
+// the two border positions add the taps that clamp onto them from direct loads.
+template <int R>
+__global__ __launch_bounds__(NT) void k_soft_box_adj_walk(const float2* __restrict__ in,
+                                                          float2* __restrict__ out, int W, int L,
+                                                          i64 sa, i64 so, unsigned nxb) {
+  constexpr int WB = 2 * R + 1;
+  const float k = 1.0f / (float)WB;
+  const int x = (int)(blockIdx.x % nxb) * NT + (int)threadIdx.x;
+  if (x >= W) return;
+  const i64 base = (i64)(blockIdx.x / nxb) * so + x;
+  auto at = [&](int c) -> float2 {
+    return (c >= 0 && c < L) ? in[base + (i64)c * sa] : make_float2(0.0f, 0.0f);
+  };
+  float2 ring[WB];  // position p lives in ring[(p + R) % WB]
+#pragma unroll
+  for (int q = 0; q < WB; ++q) ring[q] = at(q - R);
+  for (int c0 = 0;; c0 += WB) {
+#pragma unroll
+    for (int q = 0; q < WB; ++q) {
+      const int c = c0 + q;
+      if (c >= L) return;
+      float se = 0.0f, sd = 0.0f;
+#pragma unroll
+      for (int j = 0; j < WB; ++j) {  // positions c - R .. c + R
+        se += ring[(q + j) % WB].x;
+        sd += ring[(q + j) % WB].y;
+      }
+      if (c == 0)  // taps t - R + j < 0 of the windows of t < R clamp onto 0
+        for (int t = 0; t <= min(L - 1, R - 1); ++t) {
+          const float2 v = in[base + (i64)t * sa];
+          se += v.x * (float)(R - t);
+          sd += v.y * (float)(R - t);
+        }
+      if (c == L - 1)  // taps beyond L - 1 of the windows of t > L - 1 - R clamp onto L - 1
+        for (int t = max(0, L - R); t <= L - 1; ++t) {
+          const float2 v = in[base + (i64)t * sa];
+          se += v.x * (float)(t + R - L + 1);
+          sd += v.y * (float)(t + R - L + 1);
+        }
+      out[base + (i64)c * sa] = make_float2(se * k, sd * k);
+      ring[q] = at(c + R + 1);  // replaces position c - R
+    }
+  }
+}
+
 __global__ __launch_bounds__(NT) void k_soft_norm(const float* __restrict__ v,
                                                   const float2* __restrict__ ed,
                                                   float* __restrict__ norm, float min_int, i64 N) {
@@ -519,10 +566,17 @@ hipError_t launch_soft_normalize(const float* v, float* norm, float2* ed, uint32
   float2* p1 = ed;
   hipLaunchKernelGGL(k_pack_pair, dim3(g), dim3(NT), 0, s, res, res + 2 * N, p0, N);
   float2* bufs[2] = {p0, p1};
+  const i64 WH = (i64)W * H;
   for (int axis = 0; axis < 3; ++axis) {
-    // x: p0 -> ed, y: ed -> p0, z: p0 -> ed
-    hipLaunchKernelGGL(k_soft_box, dim3(grid_rows(W, H, D)), dim3(NT), 0, s, bufs[axis & 1],
-                       bufs[(axis + 1) & 1], W, H, D, axis, rad[axis], xblocks(W));
+    // x: p0 -> ed, y: ed -> p0, z: p0 -> ed; y and z by the normalization's box passes
+    if (axis == 0) {
+      hipLaunchKernelGGL(k_soft_box, dim3(grid_rows(W, H, D)), dim3(NT), 0, s, bufs[0], bufs[1],
+                         W, H, D, axis, rad[axis], xblocks(W));
+    } else {
+      e = launch_box_pair(bufs[axis & 1], bufs[(axis + 1) & 1], W, axis == 1 ? H : D,
+                          axis == 1 ? W : WH, axis == 1 ? WH : W, axis == 1 ? D : H, rad[axis], s);
+      if (e != hipSuccess) return e;
+    }
   }
   hipLaunchKernelGGL(k_soft_norm, dim3(g), dim3(NT), 0, s, v, ed, norm, min_int, N);
   return hipGetLastError();
@@ -540,10 +594,18 @@ hipError_t launch_soft_normalize_adjoint(const float* v, const float2* ed, const
   hipLaunchKernelGGL(k_soft_norm_adj, dim3(g), dim3(NT), 0, s, v, ed, gn, gv, q0, min_int, N);
   // the box was applied x, y, z: its transpose runs z, y, x
   float2* bufs[2] = {q0, q1};
+  const i64 WH = (i64)W * H;
   for (int k = 0; k < 3; ++k) {
     const int axis = 2 - k;
-    hipLaunchKernelGGL(k_soft_box_adj, dim3(grid_rows(W, H, D)), dim3(NT), 0, s, bufs[k & 1],
-                       bufs[(k + 1) & 1], W, H, D, axis, rad[axis], xblocks(W));
+    if (axis > 0 && rad[axis] == 10) {  // z, y: column walker
+      const i64 n_other = axis == 1 ? D : H;
+      hipLaunchKernelGGL((k_soft_box_adj_walk<10>), dim3((unsigned)(xblocks(W) * n_other)),
+                         dim3(NT), 0, s, bufs[k & 1], bufs[(k + 1) & 1], W, axis == 1 ? H : D,
+                         axis == 1 ? (i64)W : WH, axis == 1 ? WH : (i64)W, xblocks(W));
+    } else {
+      hipLaunchKernelGGL(k_soft_box_adj, dim3(grid_rows(W, H, D)), dim3(NT), 0, s, bufs[k & 1],
+                         bufs[(k + 1) & 1], W, H, D, axis, rad[axis], xblocks(W));
+    }
   }
   hipLaunchKernelGGL(k_soft_route, dim3(g), dim3(NT), 0, s, bufs[1], arg, arg + N, gv, N);
   return hipGetLastError();
