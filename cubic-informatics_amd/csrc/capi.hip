@@ -308,8 +308,6 @@ int regional_features_t(hdog_ctx* ctx, hipStream_t s, uint32_t* labels, const TH
   return HDOG_OK;
 }
 
-// pd and hess may be NULL (not materialized); labels is the union-find's parent array and
-// becomes the full label volume only with `dense`.
 // The separable passes' pair buffers (8 bytes per voxel each) of the whole-path call and
 // the tuner: they live in the candidate-record block (28 bytes per voxel of tile-padded
 // address space), being dead before the Hessian pass writes the first record.
@@ -323,6 +321,8 @@ hipError_t pair_buffers(hdog_ctx* ctx, int W, int H, int D, void** tmp1, void** 
   return e;
 }
 
+// pd and hess may be NULL (not materialized); labels is the union-find's parent array and
+// becomes the full label volume only with `dense`.
 template <typename TN, typename TD, typename TH>
 int run_stages(hdog_ctx* ctx, hipStream_t s, const hdog_params* p, const uint16_t* img, TN* norm,
                TD* dog, TH* hess, uint8_t* pd, uint32_t* labels, bool dense,
