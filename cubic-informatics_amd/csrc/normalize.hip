@@ -11,14 +11,15 @@
 //     65535 - min(65535 - v)).
 //   * Each 1-D min/max pass is van Herk / Gil-Werman (3 packed mins per output for any
 //     radius) on an LDS tile; tiles are x-contiguous so every HBM access coalesces.
-//   * Min/max over a clamped box is separable and exact in any order, so the passes run
-//     z, y, x and the x pass is fused with the x pass of the box filter (sums of
-//     integers, exact in any order, so a sliding sum is used).  The y and z box passes
-//     sum FP32 values and keep the reference's sequential order (c-r .. c+r); the z pass
-//     is fused with the normalization.
-//   * For the radii in use, the y / z passes (min/max and box) are column walkers: one
-//     thread per line keeps a compile-time-sized register ring, reads every input once
-//     and needs no LDS.  Other radii use the LDS-tile kernels.
+//   * Min/max over a clamped box is separable and exact in any order, so the y and z
+//     passes run first (for radius 10 in one z walk, k_minmax_zy_walk; otherwise z then
+//     y) and the x pass is fused with the x pass of the box filter (sums of integers,
+//     exact in any order, so a sliding sum is used).  The y and z box passes sum FP32
+//     values and keep the reference's sequential order (c-r .. c+r); the z pass is fused
+//     with the normalization.
+//   * For the radii in use, the z box pass (and the separate y / z min/max passes) are
+//     column walkers: one thread per line keeps a compile-time-sized register ring, reads
+//     every input once and needs no LDS.  Other radii use the LDS-tile kernels.
 #include "hdog_internal.hpp"
 
 namespace hdog {
