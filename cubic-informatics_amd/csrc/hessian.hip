@@ -7,11 +7,13 @@
 // determinant < 0, evaluated in C parse order without contraction (the library is
 // built with -ffp-contract=off).
 //
-// MI355X design: one pass over `dog` instead of nine.  A work-group streams a 64 x 8
-// column of slices through LDS rings (see k_hessian), forms the first derivatives Hx, Hy,
-// Hz on the halo positions the second pass needs (zero outside the volume, as the
-// reference's second pass pads its own input), then writes the six elements (SoA,
-// pitch N) and the mask with coalesced stores.
+// MI355X design: one pass over `dog` instead of nine.  When the elements or the mask are
+// asked for, a work-group streams a 64 x 8 column of slices through LDS rings (k_hessian),
+// forms the first derivatives Hx, Hy, Hz on the halo positions the second pass needs (zero
+// outside the volume, as the reference's second pass pads its own input), then writes the
+// six elements (SoA, pitch N) and the mask with coalesced stores.  The whole-path step
+// asks for neither: k_hessian_rec keeps everything in registers (a lane per column, DPP
+// shifts for the x neighbours) and writes only the candidates' records.
 //
 // Reduced-precision storage (FP16 / BF16): the elements are computed in FP32 from the
 // stored DoG and rounded only on the store; the candidate test uses the FP32 values, and
