@@ -3,7 +3,7 @@
 Bar: bit-exact for the normalized volume, DoG, Hessian elements, candidate mask, labels,
 region ids, voxel counts, u16 sizes, coordinate and intensity sums, max-normalized
 intensity and centroids; Hessian sums within FP64-accumulation-order noise; structureness
-and blobness within 1e-5 relative (north-star FP32 tolerance).  NaN equals NaN.
+and blobness within 1e-5 relative (the FP32 tolerance of the float features).  NaN equals NaN.
 """
 import numpy as np
 import pytest
