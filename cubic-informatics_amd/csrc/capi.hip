@@ -1315,7 +1315,8 @@ int hdog_batch_plan(hdog_ctx* ctx, const hdog_params* p, const hdog_tile* tiles,
   B->arena_cap = arena_capacity > 0 ? arena_capacity : totN / 128 + n_tiles;
   const i64 cap = B->arena_cap;
   // two branches when their scratch fits in 90 % of the free device memory, else one
-  size_t shared_bytes = (size_t)cap * 70 + ctx->weights.bytes + (size_t)n_tiles * sizeof(TileOut);
+  // the arena's region arrays: 72 bytes per region (see `need` below)
+  size_t shared_bytes = (size_t)cap * 72 + ctx->weights.bytes + (size_t)n_tiles * sizeof(TileOut);
   size_t free_b = 0, total_b = 0;
   (void)hipMemGetInfo(&free_b, &total_b);
   B->lanes = deal(n_tiles >= 2 ? 2 : 1);
