@@ -108,8 +108,9 @@ def substack_starts(L, depth, margin):
     return list(range(0, max(L - margin, 0), depth - 2 * margin)) if L - margin > 0 else []
 
 
-def expected_records(vol, P):
-    """Records the reference writes for one stack, from the oracle's regions."""
+def expected_records(vol, P, regions_of=None):
+    """Records the reference writes for one stack, from the oracle's regions (or from the
+    regions regions_of(substack) returns for each zero-filled substack)."""
     depth, margin = P["depth"], P["margin"]
     D, H, W = vol.shape
     out = []
@@ -117,8 +118,7 @@ def expected_records(vol, P):
         d = min(depth, D - z0)
         sub = np.zeros((depth, H, W), np.uint16)
         sub[:d] = vol[z0:z0 + d]
-        o = oracle.substack(sub, P["p"])
-        reg = o["regions"]
+        reg = regions_of(sub) if regions_of else oracle.substack(sub, P["p"])["regions"]
         c = reg["centroid"]
         for i in range(1, len(reg["label"])):
             cx, cy, cz = c[i]
@@ -142,6 +142,14 @@ def test_substack_starts_match_reference_loop():
 
 def test_record_dtype_is_28_bytes():
     assert DT_LOCAL.itemsize == 28
+
+
+def test_unknown_precision_exits_2(tmp_path):
+    param = make_param(tmp_path, [{"src_paths": [], "dst_path": "/unused/x.bin"}],
+                       precision="fp8")
+    r = run([param])
+    assert r.returncode == 2
+    assert "precision" in r.stdout + r.stderr
 
 
 @pytest.mark.gpu
@@ -209,3 +217,44 @@ def test_host_program_several_stacks(tmp_path):
             np.testing.assert_array_equal(got[key], exp[key], err_msg=f"stack {k}: {key}")
         for key in ("structureness", "blobness"):
             np.testing.assert_allclose(got[key], exp[key], rtol=1e-5, err_msg=f"stack {k}: {key}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp16", "bf16_dog32"])
+def test_host_program_precision_key(tmp_path, precision):
+    # HDoG_param.precision selects the library's reduced-precision form for every substack:
+    # the program's records equal those the same form gives for each zero-filled substack
+    # through the library's Python binding (same kernels, so equal bit for bit)
+    from hdog import HDoG
+    sig, truncate, depth, L = (1.5, 1.5, 2.4, 2.4), 2.0, 24, 40
+    vol = synth.make_volume("c0", shape=(L, 48, 64), n_blobs=25, seed=7)
+    paths = synth.write_stack(vol, os.path.join(str(tmp_path), "src"))
+    dst = os.path.join(str(tmp_path), "out", "000000_000000.bin")
+    os.makedirs(os.path.dirname(dst))
+    param = make_param(tmp_path, [{"src_paths": paths, "dst_path": dst}], depth=depth,
+                       truncate=truncate, sigmas=sig, min_size=2, precision=precision)
+    r = run([param])
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = np.fromfile(dst, dtype=DT_LOCAL)
+    h = HDoG(0)
+    prm = h.make_params((depth, 48, 64), 3, 2, 0, *sig, truncate=truncate, gamma=1.0,
+                        precision=precision)
+
+    def regions_of(sub):
+        h.run_device(sub, prm)
+        return h.download_regions()
+
+    rz2 = oracle.gaussian_radius(sig[3], truncate)
+    P = dict(depth=depth, margin=rz2 + 1, left=2, right=2, top=2, bottom=2, min_size=2)
+    exp = expected_records(vol, P, regions_of)
+    assert len(got) == len(exp) and len(exp) > 0
+    for k in DT_LOCAL.names:
+        np.testing.assert_array_equal(got[k], exp[k], err_msg=k)
+    # and the reduced-precision form still finds nearly the FP32 cells
+    P["p"] = dict(min_intensity=0, rxy=3, rz=2, sigma_xy1=sig[0], sigma_z1=sig[1],
+                  sigma_xy2=sig[2], sigma_z2=sig[3], truncate=truncate, gamma=1.0)
+    ref = expected_records(vol, P)
+    near = sum(np.any((np.abs(ref["local_x"] - g["local_x"]) < 1)
+                      & (np.abs(ref["local_y"] - g["local_y"]) < 1)
+                      & (np.abs(ref["local_z"] - g["local_z"]) < 1)) for g in got)
+    assert near >= 0.9 * len(ref), (near, len(ref), len(got))
