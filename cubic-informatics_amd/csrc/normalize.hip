@@ -553,8 +553,9 @@ __global__ __launch_bounds__(NT) void k_box_walk(const float2* __restrict__ src,
 //      per task (a shared window middle plus short prefix / suffix runs); on a strip at
 //      the plane's edge, positions outside [0, W) then take the value at the edge, since
 //      the box pass clamps to the edge of the filtered row;
-//   3. each lane sums B over its two windows (integers, exact in any order; v_dot2 with
-//      (1, 0) / (0, 1) adds one 16-bit half per instruction) -> (E_x, D_x) of its columns;
+//   3. each lane sums B over its two windows (integers, exact in any order) -> (E_x, D_x)
+//      of its columns; B keeps the min channel and the 65535 - max channel in two 16-bit
+//      arrays, so one v_dot2 with (1, 1) adds two window positions;
 //   4. these enter a register ring of the last 2R+1 rows (rows clamped), and each output
 //      row sums its window in the reference order, as the y pass walker does.
 // The (E_x, D_x) volume of the unfused passes is never stored.  Rows are fetched PF rows
@@ -574,7 +575,10 @@ __global__ __launch_bounds__(64) void k_norm_xy_walk(const uint32_t* __restrict_
   static_assert(HX == 2 * R, "task windows assume a halo of exactly 2R");
   static_assert(4 * NTASK + 2 * R + 4 <= LW, "task reads stay inside the line");
   __shared__ __align__(16) uint32_t A[LW];
-  __shared__ __align__(16) uint32_t B[4 * NTASK];
+  // B[b] = window minimum b: its min channel in Bm, its 65535 - max channel in Bx, as
+  // 16-bit entries, positions 2i and 2i + 1 in word i
+  __shared__ __align__(16) uint32_t Bm[2 * NTASK];
+  __shared__ __align__(16) uint32_t Bx[2 * NTASK];
   // work-group = one strip of one plane over the rows [y0, y1) of one of the nys segments
   // of the plane (rows: a multiple of WB)
   const unsigned t = xcd_tile(blockIdx.x, gridDim.x);
@@ -603,7 +607,7 @@ __global__ __launch_bounds__(64) void k_norm_xy_walk(const uint32_t* __restrict_
   const bool edge = x0 - R < 0 || x0 + CW + R > W;  // wave-uniform
   const int blo = R - x0, bhi = W - 1 - x0 + R;     // B index of x = 0 and x = W - 1
   const uint32_t full = (uint32_t)WB * 65535u;
-  const us2 lo1 = {1, 0}, hi1 = {0, 1};
+  const us2 lo1 = {1, 0}, hi1 = {0, 1}, one = {1, 1};
   // (E_x, D_x) of the lane's two columns for row y (slot s); fetches row y + PF into s
   auto xrow = [&](int y, int s) -> float4 {
 #pragma unroll
@@ -621,40 +625,60 @@ __global__ __launch_bounds__(64) void k_norm_xy_walk(const uint32_t* __restrict_
       const uint32_t mid = mid_min<R>(w);
       const uint32_t l2 = w[2], l1 = pkmin(w[1], l2), l0 = pkmin(w[0], l1);
       const uint32_t r1 = w[2 * R + 1], r2 = pkmin(r1, w[2 * R + 2]), r3 = pkmin(r2, w[2 * R + 3]);
-      *reinterpret_cast<uint4*>(B + 4 * l) =
-          make_uint4(pkmin(mid, l0), pkmin(pkmin(mid, l1), r1), pkmin(pkmin(mid, l2), r2),
-                     pkmin(mid, r3));
+      const uint32_t b0 = pkmin(mid, l0), b1 = pkmin(pkmin(mid, l1), r1),
+                     b2 = pkmin(pkmin(mid, l2), r2), b3 = pkmin(mid, r3);
+      // v_perm_b32: perm(b1, b0, 0x05040100) = lo16(b0) | lo16(b1) << 16,
+      //             perm(b1, b0, 0x07060302) = hi16(b0) | hi16(b1) << 16
+      *reinterpret_cast<uint2*>(Bm + 2 * l) =
+          make_uint2(__builtin_amdgcn_perm(b1, b0, 0x05040100u),
+                     __builtin_amdgcn_perm(b3, b2, 0x05040100u));
+      *reinterpret_cast<uint2*>(Bx + 2 * l) =
+          make_uint2(__builtin_amdgcn_perm(b1, b0, 0x07060302u),
+                     __builtin_amdgcn_perm(b3, b2, 0x07060302u));
     }
     __syncthreads();
     if (edge) {  // positions outside the plane take the edge value
-      uint32_t fix[(4 * NTASK + 63) / 64];
+      constexpr int NF = (4 * NTASK + 63) / 64;
+      uint16_t* bm = reinterpret_cast<uint16_t*>(Bm);
+      uint16_t* bx = reinterpret_cast<uint16_t*>(Bx);
+      uint16_t fm[NF], fx[NF];
 #pragma unroll
-      for (int j = 0; j < (4 * NTASK + 63) / 64; ++j) {
+      for (int j = 0; j < NF; ++j) {
         const int b = 64 * j + l;
-        fix[j] = b < 4 * NTASK ? B[clampi(b, blo, bhi)] : 0u;
+        const int c = clampi(b, blo, bhi);
+        fm[j] = b < 4 * NTASK ? bm[c] : (uint16_t)0;
+        fx[j] = b < 4 * NTASK ? bx[c] : (uint16_t)0;
       }
       __syncthreads();
 #pragma unroll
-      for (int j = 0; j < (4 * NTASK + 63) / 64; ++j)
-        if (64 * j + l < 4 * NTASK) B[64 * j + l] = fix[j];
+      for (int j = 0; j < NF; ++j)
+        if (64 * j + l < 4 * NTASK) {
+          bm[64 * j + l] = fm[j];
+          bx[64 * j + l] = fx[j];
+        }
       __syncthreads();
     }
-    // column x0 + 2l + o sums B[2l + o .. 2l + o + 2R]
-    uint32_t v[2 * R + 2];
+    // column x0 + 2l sums positions 2l .. 2l + 2R: words l .. l + R - 1 and the low half
+    // of word l + R; column x0 + 2l + 1 sums 2l + 1 .. 2l + 2R + 1: the high half of word
+    // l and words l + 1 .. l + R
+    uint32_t m[R + 1], x[R + 1];
 #pragma unroll
-    for (int t = 0; t < R + 1; ++t) {
-      const uint2 q = *reinterpret_cast<const uint2*>(B + 2 * l + 2 * t);
-      v[2 * t] = q.x;
-      v[2 * t + 1] = q.y;
+    for (int t = 0; t <= R; ++t) {
+      m[t] = Bm[l + t];
+      x[t] = Bx[l + t];
     }
-    uint32_t se = 0, sdn = 0;
+    uint32_t sm = 0, sx = 0;
 #pragma unroll
-    for (int t = 0; t <= 2 * R; ++t) {
-      se = __builtin_amdgcn_udot2(__builtin_bit_cast(us2, v[t]), lo1, se, false);
-      sdn = __builtin_amdgcn_udot2(__builtin_bit_cast(us2, v[t]), hi1, sdn, false);
+    for (int t = 1; t < R; ++t) {
+      sm = __builtin_amdgcn_udot2(__builtin_bit_cast(us2, m[t]), one, sm, false);
+      sx = __builtin_amdgcn_udot2(__builtin_bit_cast(us2, x[t]), one, sx, false);
     }
-    const uint32_t se1 = se + (v[2 * R + 1] & 0xFFFFu) - (v[0] & 0xFFFFu);
-    const uint32_t sdn1 = sdn + (v[2 * R + 1] >> 16) - (v[0] >> 16);
+    const us2 m0 = __builtin_bit_cast(us2, m[0]), mr = __builtin_bit_cast(us2, m[R]);
+    const us2 x0w = __builtin_bit_cast(us2, x[0]), xr = __builtin_bit_cast(us2, x[R]);
+    const uint32_t se = __builtin_amdgcn_udot2(m0, one, __builtin_amdgcn_udot2(mr, lo1, sm, false), false);
+    const uint32_t se1 = __builtin_amdgcn_udot2(m0, hi1, __builtin_amdgcn_udot2(mr, one, sm, false), false);
+    const uint32_t sdn = __builtin_amdgcn_udot2(x0w, one, __builtin_amdgcn_udot2(xr, lo1, sx, false), false);
+    const uint32_t sdn1 = __builtin_amdgcn_udot2(x0w, hi1, __builtin_amdgcn_udot2(xr, one, sx, false), false);
     float4 o = make_float4((float)se * k, (float)(full - sdn) * k, (float)se1 * k,
                            (float)(full - sdn1) * k);
     // pin the row's outputs here (see the DoG row walker)
