@@ -305,12 +305,14 @@ def test_unmaterialized_hessian_gives_the_same_regions(hd, precision):
         assert_regions_match(g2, o["regions"], o["hess"])
 
 
-@pytest.mark.parametrize("shape", [(20, 40, 129), (19, 37, 65), (17, 70, 193), (21, 45, 130)])
+@pytest.mark.parametrize("shape", [(20, 40, 129), (19, 37, 65), (17, 70, 193), (21, 45, 130),
+                                   (140, 24, 200), (137, 30, 131), (136, 26, 129)])
 def test_records_only_path_odd_widths(hd, shape):
     # neither the Hessian volumes nor the mask asked for: the records-only Hessian kernel
     # (interior waves and the border forms, including a last strip one column wider than
     # 64 (W = 64k + 1) whose right halo column lies outside the plane), labels and regions
-    # against the oracle
+    # against the oracle.  With 133 or more slices a work-group's block of 64 slices can
+    # lie wholly inside the volume, its two leading and four trailing fetched slices too
     import torch
     vol = synth.make_volume("c0", shape=shape, n_blobs=max(8, int(np.prod(shape) / 3000)),
                             seed=shape[2])
