@@ -222,12 +222,13 @@ __global__ __launch_bounds__(NT) void k_gauss_walk(const float2* __restrict__ sr
 
 // x and y passes fused: a one-wave work-group walks the rows of one plane for a strip of
 // 64 columns.  Each row's window x0-16 .. x0+79 goes through an LDS line (two lines,
-// alternating, one barrier per row); lane l forms the x outputs (G1_x, G2_x) of column
-// x0+l, which enter the register ring of the last 2R+1 rows exactly as in the column
-// walker, and the y outputs leave as float2.  The (G1_x, G2_x) volume of the unfused
-// passes is never stored.  Rows are fetched PF rows ahead of their use.  The Gaussian
-// weights are symmetric (set pairwise from one value, w[r - i] == w[r + i]), so tap j
-// reads w[r + |j|]: r + 1 distinct values per Gaussian stay in scalar registers.
+// alternating; the wave's own LDS order needs no barrier); lane l forms the x outputs
+// (G1_x, G2_x) of column x0+l, which enter the register ring of the last 2R+1 rows
+// exactly as in the column walker, and the y outputs leave as float2.  The (G1_x, G2_x)
+// volume of the unfused passes is never stored.  Rows are fetched PF rows ahead of their
+// use.  The Gaussian weights are symmetric (set pairwise from one value,
+// w[r - i] == w[r + i]), so tap j reads w[r + |j|]: r + 1 distinct values per Gaussian,
+// held in registers for the whole walk.
 template <int R1, int R2, typename TS>
 __global__ __launch_bounds__(64) void k_gauss_xy_walk(const TS* __restrict__ src,
                                                       float2* __restrict__ dst,
@@ -294,7 +295,7 @@ __global__ __launch_bounds__(64) void k_gauss_xy_walk(const TS* __restrict__ src
     float* ln = line[y & 1];
     ln[l] = a;
     ln[64 + l] = b;
-    __syncthreads();
+    wave_lds_sync();
     const float* t = ln + l + HX;
     float s1 = 0.0f, s2 = 0.0f;
 #pragma unroll

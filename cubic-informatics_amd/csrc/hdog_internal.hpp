@@ -72,6 +72,14 @@ template <> __device__ __forceinline__ __hip_bfloat16 cvt<__hip_bfloat16>(float 
   return __float2bfloat16(v);
 }
 
+// LDS stores before, LDS loads after, in a work-group of one wave: the LDS instructions of
+// all waves execute in one total order, the order in which each wave issues them, so a
+// load issued after a store sees it without a wait or a barrier instruction; the fence
+// only keeps the compiler from moving the accesses across it.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+}
+
 // Raw buffer loads of one stored element (the bits, converted by from_bits): a load whose
 // byte offset lies at or beyond the descriptor's record count returns zero.
 template <typename T> struct StoredBits { typedef uint16_t type; };

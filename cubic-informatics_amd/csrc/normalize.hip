@@ -613,7 +613,7 @@ __global__ __launch_bounds__(64) void k_norm_xy_walk(const uint32_t* __restrict_
 #pragma unroll
     for (int j = 0; j < NLD; ++j) A[64 * j + l] = qv[s][j];
     fetch(y + PF, s);
-    __syncthreads();
+    wave_lds_sync();
     if (l < NTASK) {
       constexpr int NV = (2 * R + 4) / 4;
       uint32_t w[4 * NV];
@@ -636,7 +636,7 @@ __global__ __launch_bounds__(64) void k_norm_xy_walk(const uint32_t* __restrict_
           make_uint2(__builtin_amdgcn_perm(b1, b0, 0x07060302u),
                      __builtin_amdgcn_perm(b3, b2, 0x07060302u));
     }
-    __syncthreads();
+    wave_lds_sync();
     if (edge) {  // positions outside the plane take the edge value
       constexpr int NF = (4 * NTASK + 63) / 64;
       uint16_t* bm = reinterpret_cast<uint16_t*>(Bm);
@@ -649,14 +649,14 @@ __global__ __launch_bounds__(64) void k_norm_xy_walk(const uint32_t* __restrict_
         fm[j] = b < 4 * NTASK ? bm[c] : (uint16_t)0;
         fx[j] = b < 4 * NTASK ? bx[c] : (uint16_t)0;
       }
-      __syncthreads();
+      wave_lds_sync();
 #pragma unroll
       for (int j = 0; j < NF; ++j)
         if (64 * j + l < 4 * NTASK) {
           bm[64 * j + l] = fm[j];
           bx[64 * j + l] = fx[j];
         }
-      __syncthreads();
+      wave_lds_sync();
     }
     // column x0 + 2l sums positions 2l .. 2l + 2R: words l .. l + R - 1 and the low half
     // of word l + R; column x0 + 2l + 1 sums 2l + 1 .. 2l + 2R + 1: the high half of word
