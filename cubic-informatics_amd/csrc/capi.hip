@@ -79,6 +79,7 @@ struct hdog_ctx {
   hipEvent_t up_done[2] = {nullptr, nullptr};   // upload into the slot complete
   hipEvent_t run_done[2] = {nullptr, nullptr};  // last run reading the slot complete
   int up_shape[2][3] = {{0, 0, 0}, {0, 0, 0}};  // (W, H, D) of each slot's upload
+  int up_valid[2] = {0, 0};                      // slices before the zero fill
   // launch shapes chosen by hdog_autotune per stack shape and storage form
   struct Tuned {
     int W, H, D, precision, norm_segments, gauss_segments;
@@ -1479,6 +1480,44 @@ int hdog_host_free(hdog_ctx* ctx, void* p) {
   return HDOG_OK;
 }
 
+}  // extern "C"
+
+// Fills device slot `slot` on the context's copy stream: `keep` slices from slice `from`
+// of the other slot's upload (device to device), then n_new host slices, then zeros up to
+// p->depth.  The slot is rewritten only after its previous upload and the last run
+// reading it ended; the other slot's upload precedes this copy on the same stream.
+static int upload_slot(hdog_ctx* ctx, const hdog_params* p, int slot, int from, int keep,
+                       const uint16_t* h_img, int n_new) {
+  HIPCHK(hipSetDevice(ctx->device));
+  if (!ctx->copy_stream) HIPCHK(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
+  for (hipEvent_t* ev : {&ctx->up_done[slot], &ctx->run_done[slot]})
+    if (!*ev) HIPCHK(hipEventCreateWithFlags(ev, hipEventDisableTiming));
+  const i64 plane = (i64)p->width * p->height;
+  const i64 N = plane * p->depth;
+  HIPCHK(hipEventSynchronize(ctx->up_done[slot]));
+  HIPCHK(hipEventSynchronize(ctx->run_done[slot]));
+  HIPCHK(ensure(ctx->up_img[slot], (size_t)N * 2));
+  uint16_t* d = (uint16_t*)ctx->up_img[slot].p;
+  if (keep > 0)
+    HIPCHK(hipMemcpyAsync(d, (const uint16_t*)ctx->up_img[1 - slot].p + plane * from,
+                          (size_t)(plane * keep) * 2, hipMemcpyDeviceToDevice, ctx->copy_stream));
+  if (n_new > 0)
+    HIPCHK(hipMemcpyAsync(d + plane * keep, h_img, (size_t)(plane * n_new) * 2,
+                          hipMemcpyHostToDevice, ctx->copy_stream));
+  const int valid = keep + n_new;
+  if (valid < p->depth)  // zero-fill, src/3D/HDoG3D_main.cpp:166-169
+    HIPCHK(hipMemsetAsync(d + plane * valid, 0, (size_t)(N - plane * valid) * 2,
+                          ctx->copy_stream));
+  HIPCHK(hipEventRecord(ctx->up_done[slot], ctx->copy_stream));
+  ctx->up_shape[slot][0] = p->width;
+  ctx->up_shape[slot][1] = p->height;
+  ctx->up_shape[slot][2] = p->depth;
+  ctx->up_valid[slot] = valid;
+  return HDOG_OK;
+}
+
+extern "C" {
+
 int hdog_substack_upload(hdog_ctx* ctx, const hdog_params* p, const uint16_t* h_img,
                          int depth_valid, int slot) {
   if (!ctx) return HDOG_ERR_ARG;
@@ -1487,27 +1526,28 @@ int hdog_substack_upload(hdog_ctx* ctx, const hdog_params* p, const uint16_t* h_
   if (!h_img || depth_valid < 1 || depth_valid > p->depth || slot < 0 || slot > 1)
     return fail(ctx, HDOG_ERR_ARG, "depth_valid %d outside [1, %d] or slot %d not 0 / 1",
                 depth_valid, p->depth, slot);
-  HIPCHK(hipSetDevice(ctx->device));
-  if (!ctx->copy_stream) HIPCHK(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
-  for (hipEvent_t* ev : {&ctx->up_done[slot], &ctx->run_done[slot]})
-    if (!*ev) HIPCHK(hipEventCreateWithFlags(ev, hipEventDisableTiming));
-  const i64 plane = (i64)p->width * p->height;
-  const i64 N = plane * p->depth;
-  // the slot is rewritten only after its previous upload and the last run reading it ended
-  HIPCHK(hipEventSynchronize(ctx->up_done[slot]));
-  HIPCHK(hipEventSynchronize(ctx->run_done[slot]));
-  HIPCHK(ensure(ctx->up_img[slot], (size_t)N * 2));
-  uint16_t* d = (uint16_t*)ctx->up_img[slot].p;
-  HIPCHK(hipMemcpyAsync(d, h_img, (size_t)(plane * depth_valid) * 2, hipMemcpyHostToDevice,
-                        ctx->copy_stream));
-  if (depth_valid < p->depth)  // zero-fill, src/3D/HDoG3D_main.cpp:166-169
-    HIPCHK(hipMemsetAsync(d + plane * depth_valid, 0, (size_t)(N - plane * depth_valid) * 2,
-                          ctx->copy_stream));
-  HIPCHK(hipEventRecord(ctx->up_done[slot], ctx->copy_stream));
-  ctx->up_shape[slot][0] = p->width;
-  ctx->up_shape[slot][1] = p->height;
-  ctx->up_shape[slot][2] = p->depth;
-  return HDOG_OK;
+  return upload_slot(ctx, p, slot, 0, 0, h_img, depth_valid);
+}
+
+int hdog_substack_upload_overlap(hdog_ctx* ctx, const hdog_params* p, int from, int keep,
+                                 const uint16_t* h_img, int n_new, int slot) {
+  if (!ctx) return HDOG_ERR_ARG;
+  int rc = check_params(ctx, p);
+  if (rc) return rc;
+  if (slot < 0 || slot > 1 || from < 0 || keep < 0 || n_new < 0 || keep + n_new < 1 ||
+      keep + n_new > p->depth || (n_new > 0 && !h_img))
+    return fail(ctx, HDOG_ERR_ARG,
+                "slot %d, from %d, keep %d, n_new %d: need slot 0 / 1 and 1 <= keep + n_new <= %d",
+                slot, from, keep, n_new, p->depth);
+  const int o = 1 - slot;
+  const int* up = ctx->up_shape[o];
+  if (keep > 0 && (!ctx->up_done[o] || up[0] != p->width || up[1] != p->height ||
+                   up[2] != p->depth || from + keep > ctx->up_valid[o]))
+    return fail(ctx, HDOG_ERR_STATE,
+                "slot %d holds %d valid slices of %d x %d x %d, not slices %d .. %d of "
+                "%d x %d x %d", o, ctx->up_valid[o], up[0], up[1], up[2], from, from + keep - 1,
+                p->width, p->height, p->depth);
+  return upload_slot(ctx, p, slot, from, keep, h_img, n_new);
 }
 
 int hdog_substack_run(hdog_ctx* ctx, void* stream, const hdog_params* p, int slot,

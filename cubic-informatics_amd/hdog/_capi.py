@@ -83,6 +83,8 @@ hdog_host_free = _sig("hdog_host_free", [_vp, _vp])
 hdog_substack_upload = _sig("hdog_substack_upload", [_vp, ctypes.POINTER(Params), _vp, _i, _i])
 hdog_substack_run = _sig("hdog_substack_run", [_vp, _vp, ctypes.POINTER(Params), _i,
                                                ctypes.POINTER(Regions)])
+hdog_substack_upload_overlap = _sig("hdog_substack_upload_overlap",
+                                    [_vp, ctypes.POINTER(Params), _i, _i, _vp, _i, _i])
 hdog_autotune = _sig("hdog_autotune", [_vp, _vp, ctypes.POINTER(Params), _vp, ctypes.c_char_p,
                                        ctypes.POINTER(_i), ctypes.POINTER(_i)])
 hdog_batch_plan = _sig("hdog_batch_plan", [_vp, ctypes.POINTER(Params), ctypes.POINTER(Tile), _i,
@@ -119,4 +121,4 @@ EXPORTED = ["hdog_create", "hdog_destroy", "hdog_last_error", "hdog_release_scra
             "hdog_soft_normalize_forward", "hdog_soft_normalize_backward",
             "hdog_autotune", "hdog_batch_plan", "hdog_batch_run", "hdog_batch_tile_info", "hdog_batch_download",
             "hdog_batch_release", "hdog_host_alloc", "hdog_host_free", "hdog_substack_upload",
-            "hdog_substack_run"]
+            "hdog_substack_run", "hdog_substack_upload_overlap"]

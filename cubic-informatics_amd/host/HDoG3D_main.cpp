@@ -14,6 +14,11 @@
 //     src/3D/HDoG3D_main.cpp:155-170); the substacks of all stacks form one pipeline, so
 //     the next stack's first substack is read while the last one of a stack runs, and the
 //     records are written by a writer thread while the next substack runs;
+//   * consecutive substacks of a stack share all but depth - 2*margin slices: those are
+//     copied on the device from the previous substack's upload, and only the new slices
+//     are read from files and copied over the link;
+//   * the region arrays arrive in page-locked buffers (a direct copy from the device
+//     rather than one staged through the runtime's own page-locked memory);
 //   * a substack step depth - 2*margin <= 0 is rejected (the reference loops forever);
 //   * a missing parameter key exits with code 2 and a message (the reference aborts);
 //   * optional HDoG_param.truncate / HDoG_param.precision keys.
@@ -25,6 +30,7 @@
 #include <cstring>
 #include <deque>
 #include <iostream>
+#include <memory>
 #include <mutex>
 #include <sstream>
 #include <string>
@@ -41,23 +47,38 @@ double ms_since(Clock::time_point t) {
   return std::chrono::duration<double, std::milli>(Clock::now() - t).count();
 }
 
+// The region arrays of one substack, in one page-locked block that grows when a substack
+// has more regions than it holds (freed when the last owner lets go of it).
 struct RegionHost {
-  std::vector<float> cx, cy, cz, st, bl, mx;
-  std::vector<uint16_t> sz;
-  hdog_regions view(int64_t cap) {
-    if ((int64_t)cx.size() < cap) {
-      for (auto* v : {&cx, &cy, &cz, &st, &bl, &mx}) v->resize(cap);
-      sz.resize(cap);
-    }
+  std::shared_ptr<void> block;
+  int64_t cap = 0;
+  float *cx = nullptr, *cy = nullptr, *cz = nullptr, *st = nullptr, *bl = nullptr,
+        *mx = nullptr;
+  uint16_t* sz = nullptr;
+  bool reserve(hdog_ctx* ctx, int64_t need) {
+    if (need <= cap) return true;
+    const int64_t n = need + need / 4 + 1024;  // the next substacks' counts vary a little
+    const size_t fb = ((size_t)n * sizeof(float) + 255) / 256 * 256;
+    void* p = nullptr;
+    if (hdog_host_alloc(ctx, fb * 6 + (size_t)n * sizeof(uint16_t), &p) != HDOG_OK) return false;
+    block = std::shared_ptr<void>(p, [ctx](void* q) { hdog_host_free(ctx, q); });
+    char* c = static_cast<char*>(p);
+    float** f[6] = {&cx, &cy, &cz, &st, &bl, &mx};
+    for (int k = 0; k < 6; ++k) *f[k] = reinterpret_cast<float*>(c + k * fb);
+    sz = reinterpret_cast<uint16_t*>(c + 6 * fb);
+    cap = n;
+    return true;
+  }
+  hdog_regions view() const {
     hdog_regions r{};
-    r.capacity = (int64_t)cx.size();
-    r.centroid_x = cx.data();
-    r.centroid_y = cy.data();
-    r.centroid_z = cz.data();
-    r.structureness = st.data();
-    r.blobness = bl.data();
-    r.max_normalized = mx.data();
-    r.size = sz.data();
+    r.capacity = cap;
+    r.centroid_x = cx;
+    r.centroid_y = cy;
+    r.centroid_z = cz;
+    r.structureness = st;
+    r.blobness = bl;
+    r.max_normalized = mx;
+    r.size = sz;
     return r;
   }
 };
@@ -126,15 +147,24 @@ int main(int argc, char** argv) {
 
   const size_t plane = (size_t)p.width * p.height;
   // every substack of every stack, in order: one pipeline over the whole parameter file
+  // keep: leading slices shared with the previous substack of the same stack (they are
+  // copied from its device slot, from its slice `from`), so only d - keep are read
   struct Item {
     size_t is;
-    int z0, d;
+    int z0, d, from, keep;
   };
   std::vector<Item> items;
   for (size_t is = 0; is < p.src_paths.size(); ++is) {
     const int L = (int)p.src_paths[is].size();
-    for (int z0 = 0; z0 < L - p.depth_margin; z0 += step)
-      items.push_back({is, z0, std::min(p.depth, L - z0)});
+    for (int z0 = 0; z0 < L - p.depth_margin; z0 += step) {
+      Item it{is, z0, std::min(p.depth, L - z0), 0, 0};
+      if (!items.empty() && items.back().is == is) {
+        const Item& pr = items.back();
+        it.from = z0 - pr.z0;
+        it.keep = std::max(0, std::min(pr.d - it.from, it.d));
+      }
+      items.push_back(it);
+    }
   }
   // three page-locked substack buffers: one being read from disk, one being copied to the
   // device, one whose copy the running substack consumed
@@ -182,9 +212,8 @@ int main(int argc, char** argv) {
         out << "length: " << p.src_paths[j.is].size() << "\n";
       } else {
         auto t = Clock::now();
-        bool ok = save_features(p.dst_paths[j.is], j.R, j.reg.cx.data(), j.reg.cy.data(),
-                                j.reg.cz.data(), j.reg.st.data(), j.reg.bl.data(),
-                                j.reg.mx.data(), j.reg.sz.data(), j.d, j.z0, p) >= 0;
+        bool ok = save_features(p.dst_paths[j.is], j.R, j.reg.cx, j.reg.cy, j.reg.cz,
+                                j.reg.st, j.reg.bl, j.reg.mx, j.reg.sz, j.d, j.z0, p) >= 0;
         if (!ok) out << "Unable to open " << p.dst_paths[j.is] << "\n";
         if (j.d == p.depth)
           out << "[save] middle of the stack : " << j.z0 + p.depth_margin << " - "
@@ -238,8 +267,8 @@ int main(int argc, char** argv) {
     std::vector<std::thread> pool;
     for (unsigned w = 0; w < nthreads; ++w)
       pool.emplace_back([&, w] {
-        for (int i = (int)w; i < it.d; i += (int)nthreads)
-          if (!load_slice(src[it.z0 + i], dst + plane * i, plane, errs[w])) return;
+        for (int i = (int)w; i < it.d - it.keep; i += (int)nthreads)
+          if (!load_slice(src[it.z0 + it.keep + i], dst + plane * i, plane, errs[w])) return;
       });
     for (auto& th : pool) th.join();
     for (auto& e : errs)
@@ -247,7 +276,11 @@ int main(int argc, char** argv) {
     ld->took = ms_since(t);
   };
   auto upload = [&](size_t k) {
-    return hdog_substack_upload(ctx, &hp, buf[k % 3], items[k].d, (int)(k & 1));
+    const Item& it = items[k];
+    if (it.keep > 0)  // the previous substack is in the other slot
+      return hdog_substack_upload_overlap(ctx, &hp, it.from, it.keep, buf[k % 3],
+                                          it.d - it.keep, (int)(k & 1));
+    return hdog_substack_upload(ctx, &hp, buf[k % 3], it.d, (int)(k & 1));
   };
   auto say = [&](const std::string& s) {
     std::lock_guard<std::mutex> lk(mu);
@@ -299,8 +332,12 @@ int main(int argc, char** argv) {
     hdog_regions probe{};
     int e = hdog_substack_run(ctx, nullptr, &hp, (int)(k & 1), &probe);
     if (e == HDOG_ERR_CAPACITY || e == HDOG_OK) {
-      hdog_regions v = j.reg.view(probe.count);
-      e = hdog_regions_download(ctx, nullptr, &v);
+      if (j.reg.reserve(ctx, probe.count)) {
+        hdog_regions v = j.reg.view();
+        e = hdog_regions_download(ctx, nullptr, &v);
+      } else {
+        e = HDOG_ERR_HIP;
+      }
     }
     if (e != HDOG_OK) {
       say(std::string("HDoG failed: ") + hdog_last_error(ctx) + "\n");
@@ -339,6 +376,7 @@ int main(int argc, char** argv) {
   cv.notify_all();
   writer.join();
   if (write_failed && rc == 0) rc = 2;
+  spare.clear();  // page-locked region arrays, before the context goes
   for (auto& b : buf) hdog_host_free(ctx, b);
   hdog_destroy(ctx);
   return rc;

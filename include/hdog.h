@@ -186,6 +186,18 @@ int hdog_substack_upload(hdog_ctx* ctx, const hdog_params* p, const uint16_t* h_
 int hdog_substack_run(hdog_ctx* ctx, void* stream, const hdog_params* p, int slot,
                       hdog_regions* out);
 
+/* hdog_substack_upload for a substack that overlaps the one uploaded into the other slot:
+ * its slices 0 .. keep-1 are slices from .. from+keep-1 of that upload (copied on the
+ * device, which needs from + keep no more than that upload's depth_valid), slices keep ..
+ * keep+n_new-1 come from h_img (n_new host slices), and the rest up to p->depth is
+ * zero-filled.  Substacks advance by depth - 2*margin slices, so consecutive substacks of
+ * a stack share all but that many: the reference reads and copies every slice of every
+ * substack again (src/3D/HDoG3D_main.cpp:155-170); here only the new ones cross the link.
+ * Same slot rules as hdog_substack_upload; HDOG_ERR_STATE when the other slot does not
+ * hold enough valid slices of this shape. */
+int hdog_substack_upload_overlap(hdog_ctx* ctx, const hdog_params* p, int from, int keep,
+                                 const uint16_t* h_img, int n_new, int slot);
+
 /* Page-locked host memory (hipHostMalloc): host-to-device copies from it run at the full
  * link rate and asynchronously.  No counterpart in the reference, which reads into
  * cudaHostAlloc buffers it allocates itself (src/3D/HDoG3D_main.cpp:58-109). */

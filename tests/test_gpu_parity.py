@@ -444,6 +444,77 @@ def test_split_upload_and_run_match_the_oracle(hd):
             C.hdog_host_free(hd._h, ptr)
 
 
+def test_overlap_upload_matches_the_oracle(hd):
+    # substacks of one 40-slice stack (depth 24, step 8) filled from the previous upload's
+    # slices on the device plus the new host slices only: the second and third share 16
+    # slices with their predecessor; the last two (16 and 8 slices, zero-filled to 24) lie
+    # wholly inside their predecessor and read nothing from the host
+    import ctypes
+    from hdog import _capi as C
+    p = dict(C0P, truncate=2.0)
+    vol = synth.make_volume("c0", shape=(40, 48, 64), n_blobs=40, seed=77)
+    depth, step = 24, 8
+    shape = (depth, 48, 64)
+    params = hd.make_params(shape, p["rxy"], p["rz"], p["min_intensity"], p["sigma_xy1"],
+                            p["sigma_z1"], p["sigma_xy2"], p["sigma_z2"], 2.0)
+    items = []
+    for z0 in range(0, 40 - 4, step):
+        d = min(depth, 40 - z0)
+        keep = max(0, min(items[-1][1] - (z0 - items[-1][0]), d)) if items else 0
+        items.append((z0, d, keep))
+    assert items == [(0, 24, 0), (8, 24, 16), (16, 24, 16), (24, 16, 16), (32, 8, 8)]
+    n = 48 * 64 * depth
+    bufs = []
+    for _ in range(3):
+        ptr = ctypes.c_void_p()
+        assert C.hdog_host_alloc(hd._h, n * 2, ctypes.byref(ptr)) == 0
+        bufs.append((ptr, np.ctypeslib.as_array((ctypes.c_uint16 * n).from_address(ptr.value))))
+    try:
+        def up(k):
+            z0, d, keep = items[k]
+            h = bufs[k % 3][1].reshape(shape)
+            h[:d - keep] = vol[z0 + keep:z0 + d]  # the new slices only
+            if keep:
+                rc = C.hdog_substack_upload_overlap(hd._h, ctypes.byref(params), step, keep,
+                                                    bufs[k % 3][0], d - keep, k & 1)
+            else:
+                rc = C.hdog_substack_upload(hd._h, ctypes.byref(params), bufs[k % 3][0], d, k & 1)
+            assert rc == 0, C.hdog_last_error(hd._h)
+        up(0)
+        for k, (z0, d, keep) in enumerate(items):
+            if k + 1 < len(items):
+                up(k + 1)
+            padded = np.zeros(shape, np.uint16)
+            padded[:d] = vol[z0:z0 + d]
+            o = oracle.substack(padded, p)["regions"]
+            R = len(o["label"])
+            lab = np.zeros(R, np.uint32)
+            cnt = np.zeros(R, np.uint64)
+            reg = C.Regions()
+            reg.capacity = R
+            reg.label, reg.voxels = lab.ctypes.data, cnt.ctypes.data
+            rc = C.hdog_substack_run(hd._h, hd._stream(), ctypes.byref(params), k & 1,
+                                     ctypes.byref(reg))
+            assert rc == 0, C.hdog_last_error(hd._h)
+            assert reg.count == R, k
+            np.testing.assert_array_equal(lab[1:], o["label"][1:], err_msg=str(k))
+            np.testing.assert_array_equal(cnt[1:], o["count"][1:], err_msg=str(k))
+        # slot 0 holds the last substack: 8 valid slices, so slices 4 .. 11 are refused
+        assert C.hdog_substack_upload_overlap(hd._h, ctypes.byref(params), 4, 8, bufs[0][0],
+                                              0, 1) == C.HDOG_ERR_STATE
+        # as are slices of another shape, and an empty or too deep substack
+        other = hd.make_params((depth, 48, 32), p["rxy"], p["rz"], p["min_intensity"],
+                               p["sigma_xy1"], p["sigma_z1"], p["sigma_xy2"], p["sigma_z2"], 2.0)
+        assert C.hdog_substack_upload_overlap(hd._h, ctypes.byref(other), 0, 4, bufs[0][0],
+                                              0, 1) == C.HDOG_ERR_STATE
+        for keep, n_new in ((0, 0), (20, 8)):
+            assert C.hdog_substack_upload_overlap(hd._h, ctypes.byref(params), 0, keep,
+                                                  bufs[0][0], n_new, 1) == C.HDOG_ERR_ARG
+    finally:
+        for ptr, _ in bufs:
+            C.hdog_host_free(hd._h, ptr)
+
+
 def test_errors_raise(hd):
     from hdog import HDoG, HDoGError
     with pytest.raises(HDoGError):
