@@ -264,9 +264,31 @@ struct LdsTable {
     }
   }
 
+  // sums of some voxels of one region: count, coordinate offsets from the origin, raw
+  // intensity sum and max, max key (or sum) of the normalized value, Hessian sums
+  struct Part {
+    uint32_t cnt, dx, dy, dz, isum, imax, nkey;
+    double ns, h[6];
+  };
+
   __device__ void add(RegionAcc* acc, uint32_t r, uint32_t x, uint32_t y, uint32_t z,
                       const float* hv, float nv, uint32_t iv, int is_ave, bool has_img,
                       bool owned = false) {
+    Part q;
+    q.cnt = 1u;
+    q.dx = x - ox;
+    q.dy = y - oy;
+    q.dz = z - oz;
+    q.isum = q.imax = iv;
+    q.nkey = is_ave ? 0u : fkey(nv);
+    q.ns = is_ave ? (double)nv : 0.0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) q.h[k] = (double)hv[k];
+    add_part(acc, r, q, is_ave, has_img, owned);
+  }
+
+  __device__ void add_part(RegionAcc* acc, uint32_t r, const Part& q, int is_ave, bool has_img,
+                           bool owned) {
     uint32_t slot = (r * 2654435761u) >> (32 - __builtin_ctz(S));
     bool placed = false;
     for (int probes = 0; probes < S; ++probes) {
@@ -279,27 +301,31 @@ struct LdsTable {
     }
     if (!placed) {
       RunSums v;
-      v.cnt = 1u; v.sx = x; v.sy = y; v.sz = z;
-      v.isum = iv; v.imax = iv;
-      v.nkey = is_ave ? 0u : fkey(nv);
-      v.ns = is_ave ? (double)nv : 0.0;
+      v.cnt = q.cnt;
+      v.sx = q.dx + (unsigned long long)q.cnt * ox;
+      v.sy = q.dy + (unsigned long long)q.cnt * oy;
+      v.sz = q.dz + (unsigned long long)q.cnt * oz;
+      v.isum = q.isum;
+      v.imax = q.imax;
+      v.nkey = q.nkey;
+      v.ns = q.ns;
 #pragma unroll
-      for (int k = 0; k < 6; ++k) v.h[k] = (double)hv[k];
+      for (int k = 0; k < 6; ++k) v.h[k] = q.h[k];
       flush_global(acc + r, v, is_ave, has_img);
       spilled = 1u;
       return;
     }
     if (owned) own[slot] = 1u;  // the same value from every voxel of the region
-    atomicAdd(&cxy[slot], 1ull | ((unsigned long long)(x - ox) << 16) |
-                              ((unsigned long long)(y - oy) << 40));
-    atomicAdd(&zis[slot], (unsigned long long)(z - oz) | ((unsigned long long)iv << 24));
+    atomicAdd(&cxy[slot], (unsigned long long)q.cnt | ((unsigned long long)q.dx << 16) |
+                              ((unsigned long long)q.dy << 40));
+    atomicAdd(&zis[slot], (unsigned long long)q.dz | ((unsigned long long)q.isum << 24));
 #pragma unroll
-    for (int k = 0; k < 6; ++k) unsafeAtomicAdd(&h[k][slot], (double)hv[k]);
+    for (int k = 0; k < 6; ++k) unsafeAtomicAdd(&h[k][slot], q.h[k]);
     if (is_ave)
-      unsafeAtomicAdd(&ns[slot], (double)nv);
+      unsafeAtomicAdd(&ns[slot], q.ns);
     else
-      atomicMax(&nk[slot], fkey(nv));
-    if (has_img) atomicMax(&im[slot], iv);
+      atomicMax(&nk[slot], q.nkey);
+    if (has_img) atomicMax(&im[slot], q.imax);
   }
 
   __device__ void flush(RegionAcc* acc, int is_ave, bool has_img) {
@@ -384,6 +410,15 @@ __global__ __launch_bounds__(NT) void k_accum(const uint32_t* __restrict__ L,
 static_assert(kAccGY * kAccGZ * kCandTile < (1 << 16) && kHessTX <= 256 &&
                   kAccGY * kHessTY <= 256 && kAccGZ * kHessTZ <= 256,
               "packed table sums: count below 2^16, offsets below 256");
+// v of lane i + 1 (lane 63 keeps its own) and of lane i - 1 (lane 0 keeps its own); one
+// DPP move each.  The whole wave must be active.
+__device__ __forceinline__ uint32_t lane_next(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x130, 0xf, 0xf, false);
+}
+__device__ __forceinline__ uint32_t lane_prev(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x138, 0xf, 0xf, false);
+}
+
 template <typename TN>
 __global__ __launch_bounds__(NT) void k_accum_cand(const float* __restrict__ cand,
                                                    const uint32_t* __restrict__ cand_cnt,
@@ -398,6 +433,7 @@ __global__ __launch_bounds__(NT) void k_accum_cand(const float* __restrict__ can
   // work-group p takes the tiles (bx, kAccGY gy + a, kAccGZ gz + b) of the Hessian tiling;
   // a region within them is final here, others pay one set of global atomics
   constexpr int TPW = kAccGY * kAccGZ;
+  const int lane = (int)(threadIdx.x & 63);
   const unsigned p = xcd_tile(blockIdx.x, gridDim.x);
   const unsigned ngy = (nby + kAccGY - 1) / kAccGY;
   const unsigned bx = p % nbx, gy = (p / nbx) % ngy, gz = p / nbx / ngy;
@@ -424,21 +460,88 @@ __global__ __launch_bounds__(NT) void k_accum_cand(const float* __restrict__ can
     const uint32_t* rl = rlab + (size_t)tiles[q] * kCandTile;  // each record's root label
     // tile origin: records hold the voxel's position in the tile
     const uint32_t tx0 = bx * kHessTX, ty0 = tby[q] * kHessTY, tz0 = tbz[q] * kHessTZ;
-    for (uint32_t i = threadIdx.x; i < n[q]; i += NT) {
-      const uint32_t v = __float_as_uint(t[i]);
-      const uint32_t x = tx0 + (v % kHessTX), y = ty0 + (v / kHessTX) % kHessTY,
-                     z = tz0 + v / (kHessTX * kHessTY);
-      const uint32_t idx = z * WH + y * W + x;
-      float hv[6];
+    // the whole wave walks the records together (the group sums below move values across
+    // lanes); lanes past the tile's records load nothing and add nothing
+    for (uint32_t b = threadIdx.x - lane; b < n[q]; b += NT) {
+      const uint32_t i = b + lane;
+      const bool ok = i < n[q];
+      uint32_t r = 0u, x = 0u, y = 0u, z = 0u, iv = 0u;
+      float hv[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f}, nv = 0.0f;
+      bool owned = false;
+      if (ok) {
+        const uint32_t v = __float_as_uint(t[i]);
+        x = tx0 + (v % kHessTX);
+        y = ty0 + (v / kHessTX) % kHessTY;
+        z = tz0 + v / (kHessTX * kHessTY);
+        const uint32_t idx = z * WH + y * W + x;
 #pragma unroll
-      for (int k = 0; k < 6; ++k) hv[k] = t[(size_t)kCandTile * (1 + k) + i];
-      const uint32_t root = rl[i];
-      const uint32_t r = rid[root];
-      // the labeling marks every region that reaches beyond one work-group's tiles
-      const bool owned = !((multi[root >> 6] >> (root & 63)) & 1ull);
-      const float nv = ld(norm, (i64)idx);
-      const uint32_t iv = has_img ? (uint32_t)img[idx] : 0u;
-      if (r < cap) tab.add(acc, r, x, y, z, hv, nv, iv, is_ave, has_img, owned);
+        for (int k = 0; k < 6; ++k) hv[k] = t[(size_t)kCandTile * (1 + k) + i];
+        const uint32_t root = rl[i];
+        r = rid[root];
+        // the labeling marks every region that reaches beyond one work-group's tiles
+        owned = !((multi[root >> 6] >> (root & 63)) & 1ull);
+        nv = ld(norm, (i64)idx);
+        iv = has_img ? (uint32_t)img[idx] : 0u;
+      }
+      // Consecutive records are mostly x-neighbours in one row of one region.  Runs of
+      // lanes with the same region are cut into groups of at most four; the first lane of
+      // a group adds the group's sums, so the table sees one set of atomics per group.
+      // (the shifts run in every lane, outside any condition: a lane whose source lane is
+      // inactive would get its own value back)
+      const uint32_t r_prev = lane_prev(r);
+      const uint32_t ok_prev = lane_prev(ok ? 1u : 0u);
+      const bool run_head = (lane == 0) | !ok | (ok_prev == 0u) | (r_prev != r);
+      const unsigned long long heads = __ballot(run_head);
+      const unsigned long long upto = lane == 63 ? ~0ull : (2ull << lane) - 1ull;
+      const int start = 63 - __builtin_clzll(heads & upto);  // first lane of the run
+      const unsigned long long later = heads & ~upto;
+      const int next = later ? __builtin_ctzll(later) : 64;  // first lane of the next run
+      const int last = min(next - 1, lane + 3);  // last lane of a group that starts here
+      typename LdsTable<256>::Part g;
+      g.cnt = 1u;
+      g.dx = x - tab.ox;
+      g.dy = y - tab.oy;
+      g.dz = z - tab.oz;
+      g.isum = g.imax = iv;
+      g.nkey = is_ave ? 0u : fkey(nv);
+      g.ns = is_ave ? (double)nv : 0.0;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) g.h[k] = (double)hv[k];
+      // the values of lanes +1, +2, +3 arrive by one-lane shifts; those past `last` are
+      // left out (a shift keeps lane 63's own value, which is never inside its group)
+      uint32_t sdx = g.dx, sdy = g.dy, sdz = g.dz, siv = iv, snk = g.nkey;
+      float snv = nv, sh[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) sh[k] = hv[k];
+#pragma unroll
+      for (int o = 1; o <= 3; ++o) {
+        sdx = lane_next(sdx);
+        sdy = lane_next(sdy);
+        sdz = lane_next(sdz);
+        siv = lane_next(siv);
+        if (is_ave)
+          snv = __uint_as_float(lane_next(__float_as_uint(snv)));
+        else
+          snk = lane_next(snk);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) sh[k] = __uint_as_float(lane_next(__float_as_uint(sh[k])));
+        if (lane + o <= last) {
+          g.cnt += 1u;
+          g.dx += sdx;
+          g.dy += sdy;
+          g.dz += sdz;
+          g.isum += siv;
+          g.imax = max(g.imax, siv);
+          if (is_ave)
+            g.ns += (double)snv;
+          else
+            g.nkey = max(g.nkey, snk);
+#pragma unroll
+          for (int k = 0; k < 6; ++k) g.h[k] += (double)sh[k];
+        }
+      }
+      const bool head = ((lane - start) & 3) == 0;
+      if (ok && head && r < cap) tab.add_part(acc, r, g, is_ave, has_img, owned);
     }
   }
   __syncthreads();
