@@ -266,6 +266,99 @@ __global__ __launch_bounds__(NT) void k_soft_argminmax(const float* __restrict__
   }
 }
 
+// The x passes with the work-group's row segment x0 - r .. x0 + NT - 1 + r staged in LDS
+// (clamped to the row): the same taps, in the same order, as the line kernels, each read
+// from LDS instead of global memory.  For radii up to kXR; others use the line kernels.
+constexpr int kXR = 64;
+
+// stages the clamped row segment of the work-group; returns the row's linear base, x0
+template <typename T>
+__device__ __forceinline__ i64 stage_row(const T* __restrict__ in, T* sv, int W, int r,
+                                         unsigned nxb, int& x0) {
+  const unsigned row = blockIdx.x / nxb;
+  x0 = (int)(blockIdx.x - row * nxb) * NT;
+  const i64 base = (i64)row * W;
+  for (int e = (int)threadIdx.x; e < NT + 2 * r; e += NT) sv[e] = in[base + clampi(x0 - r + e, 0, W - 1)];
+  __syncthreads();
+  return base;
+}
+
+// first (x) pass of the windowed (value, index) minimum and maximum: value v[p], index p
+__global__ __launch_bounds__(NT) void k_soft_argminmax_x(const float* __restrict__ v,
+                                                         float* __restrict__ omn,
+                                                         uint32_t* __restrict__ oimn,
+                                                         float* __restrict__ omx,
+                                                         uint32_t* __restrict__ oimx, int W,
+                                                         int r, unsigned nxb) {
+  __shared__ float sv[NT + 2 * kXR];
+  int x0;
+  const i64 base = stage_row(v, sv, W, r, nxb, x0);
+  const int x = x0 + (int)threadIdx.x;
+  if (x >= W) return;
+  float bmn = INFINITY, bmx = -INFINITY;
+  uint32_t jmn = 0xFFFFFFFFu, jmx = 0xFFFFFFFFu;
+  for (int t = -r; t <= r; ++t) {
+    const int c = clampi(x + t, 0, W - 1);
+    const float a = sv[x + t - x0 + r];  // the value at c
+    const uint32_t j = (uint32_t)(base + c);
+    if (a < bmn || (a == bmn && j < jmn)) { bmn = a; jmn = j; }
+    if (a > bmx || (a == bmx && j < jmx)) { bmx = a; jmx = j; }
+  }
+  omn[base + x] = bmn; oimn[base + x] = jmn;
+  omx[base + x] = bmx; oimx[base + x] = jmx;
+}
+
+// clamped box mean of the (E, D) pair along x (FP32 sums in window order)
+__global__ __launch_bounds__(NT) void k_soft_box_x(const float2* __restrict__ in,
+                                                   float2* __restrict__ out, int W, int r,
+                                                   unsigned nxb) {
+  __shared__ float2 sv[NT + 2 * kXR];
+  const float k = 1.0f / (float)(2 * r + 1);
+  int x0;
+  const i64 base = stage_row(in, sv, W, r, nxb, x0);
+  const int x = x0 + (int)threadIdx.x;
+  if (x >= W) return;
+  float se = 0.0f, sd = 0.0f;
+  for (int t = -r; t <= r; ++t) {
+    const float2 v = sv[x + t - x0 + r];
+    se += v.x;
+    sd += v.y;
+  }
+  out[base + x] = make_float2(se * k, sd * k);
+}
+
+// transpose of the clamped box mean along x (k_soft_box_adj's sums, from LDS)
+__global__ __launch_bounds__(NT) void k_soft_box_adj_x(const float2* __restrict__ in,
+                                                       float2* __restrict__ out, int W, int r,
+                                                       unsigned nxb) {
+  __shared__ float2 sv[NT + 2 * kXR];
+  const float k = 1.0f / (float)(2 * r + 1);
+  int x0;
+  const i64 base = stage_row(in, sv, W, r, nxb, x0);
+  const int p = x0 + (int)threadIdx.x, L = W;
+  if (p >= W) return;
+  const int o = r - x0;  // LDS slot of position c: c + o
+  float se = 0.0f, sd = 0.0f;
+  for (int c = max(0, p - r); c <= min(L - 1, p + r); ++c) {
+    const float2 v = sv[c + o];
+    se += v.x;
+    sd += v.y;
+  }
+  if (p == 0)
+    for (int c = 0; c <= min(L - 1, r - 1); ++c) {
+      const float2 v = sv[c + o];
+      se += v.x * (float)(r - c);
+      sd += v.y * (float)(r - c);
+    }
+  if (p == L - 1)
+    for (int c = max(0, L - r); c <= L - 1; ++c) {
+      const float2 v = sv[c + o];
+      se += v.x * (float)(c + r - L + 1);
+      sd += v.y * (float)(c + r - L + 1);
+    }
+  out[base + p] = make_float2(se * k, sd * k);
+}
+
 // The same pass along y or z (second and third passes, indices given) as a column walker
 // for radius R: one thread per line (x-consecutive threads), van Herk blocks of 2R+1 over
 // the clamp-extended line g[k] = f[clamp(k - R)] held in place in registers (slot j keeps
@@ -549,6 +642,10 @@ hipError_t launch_soft_normalize(const float* v, float* norm, float2* ed, uint32
       hipLaunchKernelGGL((k_soft_argminmax_walk<10>), dim3((unsigned)(xblocks(W) * n_other)),
                          dim3(NT), 0, s, vmn, imn, vmx, imx, dst, (uint32_t*)(dst + N), dst + 2 * N,
                          (uint32_t*)(dst + 3 * N), W, L, sa, so, xblocks(W));
+    } else if (axis == 0 && rad[0] <= kXR) {  // x, the first pass: indices are positions
+      hipLaunchKernelGGL(k_soft_argminmax_x, dim3(grid_rows(W, H, D)), dim3(NT), 0, s, v, dst,
+                         (uint32_t*)(dst + N), dst + 2 * N, (uint32_t*)(dst + 3 * N), W, rad[0],
+                         xblocks(W));
     } else {
       hipLaunchKernelGGL(k_soft_argminmax, dim3(grid_rows(W, H, D)), dim3(NT), 0, s, vmn, imn,
                          vmx, imx, dst, (uint32_t*)(dst + N), dst + 2 * N,
@@ -569,7 +666,10 @@ hipError_t launch_soft_normalize(const float* v, float* norm, float2* ed, uint32
   const i64 WH = (i64)W * H;
   for (int axis = 0; axis < 3; ++axis) {
     // x: p0 -> ed, y: ed -> p0, z: p0 -> ed; y and z by the normalization's box passes
-    if (axis == 0) {
+    if (axis == 0 && rad[0] <= kXR) {
+      hipLaunchKernelGGL(k_soft_box_x, dim3(grid_rows(W, H, D)), dim3(NT), 0, s, bufs[0],
+                         bufs[1], W, rad[0], xblocks(W));
+    } else if (axis == 0) {
       hipLaunchKernelGGL(k_soft_box, dim3(grid_rows(W, H, D)), dim3(NT), 0, s, bufs[0], bufs[1],
                          W, H, D, axis, rad[axis], xblocks(W));
     } else {
@@ -602,6 +702,9 @@ hipError_t launch_soft_normalize_adjoint(const float* v, const float2* ed, const
       hipLaunchKernelGGL((k_soft_box_adj_walk<10>), dim3((unsigned)(xblocks(W) * n_other)),
                          dim3(NT), 0, s, bufs[k & 1], bufs[(k + 1) & 1], W, axis == 1 ? H : D,
                          axis == 1 ? (i64)W : WH, axis == 1 ? WH : (i64)W, xblocks(W));
+    } else if (axis == 0 && rad[0] <= kXR) {
+      hipLaunchKernelGGL(k_soft_box_adj_x, dim3(grid_rows(W, H, D)), dim3(NT), 0, s,
+                         bufs[k & 1], bufs[(k + 1) & 1], W, rad[0], xblocks(W));
     } else {
       hipLaunchKernelGGL(k_soft_box_adj, dim3(grid_rows(W, H, D)), dim3(NT), 0, s, bufs[k & 1],
                          bufs[(k + 1) & 1], W, H, D, axis, rad[axis], xblocks(W));
