@@ -277,18 +277,25 @@ __global__ __launch_bounds__(NTR) void k_cand_local(const float* __restrict__ ca
     R.each_rolled(rec, n, [&](uint32_t, uint32_t pv) __attribute__((always_inline)) {
       const int v = (int)pv;
       const int vx = v & (RTX - 1), vy = (v / RTX) & (RTY - 1), vz = v / (RTX * RTY);
-      // bit q: forward neighbour q is a candidate inside the clipped tile (branch-free)
-      const bool xl = vx > 0, xr = vx + 1 < T.ex, yl = vy > 0, yr = vy + 1 < T.ey,
-                 zr = vz + 1 < T.ez;
-      uint32_t fg = 0u;
-#pragma unroll
-      for (int q = 0; q < 13; ++q) {
-        const bool in = (fdx(q) < 0 ? xl : fdx(q) > 0 ? xr : true) &&
-                        (fdy(q) < 0 ? yl : fdy(q) > 0 ? yr : true) && (fdz(q) > 0 ? zr : true);
-        const int row = in ? (vy + fdy(q)) + RTY * (vz + fdz(q)) : 0;
-        const uint32_t b = (uint32_t)(bits[row] >> ((vx + fdx(q)) & (RTX - 1))) & 1u;
-        fg |= (in ? b : 0u) << q;
-      }
+      // bit q: forward neighbour q is a candidate inside the clipped tile (branch-free).
+      // The 13 neighbours lie in five mask rows: x-1, x, x+1 of rows (y+1, z), (y-1, z+1),
+      // (y, z+1), (y+1, z+1) in three bits each, and x+1 of (y, z).  Mask bits outside the
+      // clipped tile are 0 (no candidates there), so only rows outside it need a test; a
+      // column outside the 64-bit row shifts in as 0.
+      const bool yl = vy > 0, yr = vy + 1 < T.ey, zr = vz + 1 < T.ez;
+      auto tri = [&](int row, bool ok) -> uint32_t {  // bits of x-1, x, x+1 (0 outside)
+        const unsigned long long w = bits[ok ? row : 0];
+        const uint32_t t = vx ? (uint32_t)(w >> (vx - 1)) & 7u : (uint32_t)(w << 1) & 7u;
+        return ok ? t : 0u;
+      };
+      const int r0 = vy + RTY * vz;
+      const uint32_t fg = tri(r0 - 1 + RTY, yl && zr) | tri(r0 + RTY, zr) << 3 |
+                          tri(r0 + 1 + RTY, yr && zr) << 6 | tri(r0 + 1, yr) << 9 |
+                          ((uint32_t)(bits[r0] >> vx >> 1) & 1u) << 12;
+      static_assert(fdx(0) == -1 && fdy(0) == -1 && fdz(0) == 1 && fdx(8) == 1 &&
+                        fdy(8) == 1 && fdx(9) == -1 && fdy(9) == 1 && fdz(9) == 0 &&
+                        fdx(12) == 1 && fdy(12) == 0 && fdz(12) == 0,
+                    "bit q of the five rows is forward neighbour q");
       // the links: neighbours not adjacent to one already known to be in v's set
       uint32_t known = 0u, link = 0u;
 #pragma unroll
