@@ -7,7 +7,9 @@ This is a CPU model of that rule over the same tile-local and merge link sets (t
 clipped to the volume, wrapped linear offsets for links that leave the tile), checked
 against the oracle's labels on random volumes with ragged tiles, wrap and small widths.
 A model that drops a needed link (for example one that counts a neighbour as known before
-its set is joined) fails it."""
+its set is joined) fails it.  The tile-local kernel's gathering of the 13 forward
+neighbours' mask bits from five rows is checked against the definition (a neighbour inside
+the clipped tile that is a candidate)."""
 import numpy as np
 import pytest
 
@@ -125,3 +127,42 @@ def test_a_wrong_rule_is_caught():
         got, _ = label_with_rule(m, tile, wrong=True)
         bad += not np.array_equal(got, oracle.ccl3d(m))
     assert bad > 0
+
+
+def five_row_mask(bits, vx, vy, vz, ey, ez, RTY=8):
+    """k_cand_local's forward-neighbour bits from five mask rows: x-1, x, x+1 of rows
+    (y-1, z+1), (y, z+1), (y+1, z+1), (y+1, z) in three bits each, then x+1 of (y, z)."""
+    M = (1 << 64) - 1
+
+    def tri(row, ok):
+        w = bits[row if ok else 0]
+        t = ((w >> (vx - 1)) & 7) if vx else ((w << 1) & M) & 7
+        return t if ok else 0
+
+    yl, yr, zr = vy > 0, vy + 1 < ey, vz + 1 < ez
+    r0 = vy + RTY * vz
+    return (tri(r0 - 1 + RTY, yl and zr) | tri(r0 + RTY, zr) << 3 |
+            tri(r0 + 1 + RTY, yr and zr) << 6 | tri(r0 + 1, yr) << 9 |
+            ((bits[r0] >> vx >> 1) & 1) << 12)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_five_row_neighbour_mask_equals_the_definition(seed):
+    # bit q set exactly when forward neighbour q (OFF[q]) is a candidate inside the tile
+    # clipped to (ex, ey, ez); candidates exist only inside it, as the records guarantee
+    rng = np.random.default_rng(seed)
+    RTX, RTY, RTZ = 64, 8, 16
+    for _ in range(40):
+        ex, ey, ez = (int(rng.integers(1, n + 1)) for n in (RTX, RTY, RTZ))
+        cand = np.zeros((RTZ, RTY, RTX), bool)
+        cand[:ez, :ey, :ex] = rng.random((ez, ey, ex)) < 0.4
+        bits = [int(sum(1 << x for x in range(RTX) if cand[z, y, x]))
+                for z in range(RTZ) for y in range(RTY)]
+        for _ in range(200):
+            vx, vy, vz = int(rng.integers(0, ex)), int(rng.integers(0, ey)), int(rng.integers(0, ez))
+            want = 0
+            for q, (dx, dy, dz) in enumerate(OFF):
+                x, y, z = vx + dx, vy + dy, vz + dz
+                if 0 <= x < ex and 0 <= y < ey and 0 <= z < ez and cand[z, y, x]:
+                    want |= 1 << q
+            assert five_row_mask(bits, vx, vy, vz, ey, ez) == want, (vx, vy, vz, ex, ey, ez)
