@@ -394,15 +394,16 @@ __global__ __launch_bounds__(1024) void k_minmax_zy_walk(const uint16_t* __restr
     for (int j = 0; j < NLD; ++j)
       q[slot][j] = __builtin_amdgcn_raw_buffer_load_b16(r, (int)(off[j] * 2u), 0, 0);
   };
-  // slice k's packed rows into buffer k & 1, the loads of slice k + PF into the freed slot
-  auto stage = [&](int k, int slot) -> const uint32_t* {
+  // slice k's packed rows into buffer k & 1, the loads of slice k + PF into the freed slot.
+  // Step k stores slice k + 1 and reads slice k's rows, then one barrier: every wave's
+  // stores of slice k + 1 are then done before step k + 1 reads them, and its reads of
+  // slice k before step k + 1 stores slice k + 2 over them (one wait per slice, not two).
+  auto stage = [&](int k, int slot) {
     uint32_t* Sb = S[k & 1];
 #pragma unroll
     for (int j = 0; j < NLD; ++j)
       Sb[(wv + NWV * j) * 64 + lane] = pack(q[slot][j]);
     fetch(k + PF, slot);
-    __syncthreads();
-    return Sb;
   };
   const int ty = KR * wv;  // first output row of this thread in the tile
   // y window minima of rows ty, ty + 1: staged rows ty .. ty + 2RY + 1 of column lane
@@ -440,13 +441,16 @@ __global__ __launch_bounds__(1024) void k_minmax_zy_walk(const uint16_t* __restr
   };
 #pragma unroll
   for (int k = 0; k < PF; ++k) fetch(k, k);
+  stage(0, 0);
+  __syncthreads();
   // block 0 (slices k = 0 .. WB-1) only fills the slots; its suffix minimum S[0] is
   // output 0
 #pragma unroll
   for (int j = 0; j < WB; ++j) {
-    const uint32_t* Sb = stage(j, j % PF);
+    stage(j + 1, (j + 1) % PF);
     uint32_t v[KR];
-    ymin(Sb, v);
+    ymin(S[j & 1], v);
+    __syncthreads();
 #pragma unroll
     for (int i = 0; i < KR; ++i) zs[i][j] = v[i];
   }
@@ -465,9 +469,10 @@ __global__ __launch_bounds__(1024) void k_minmax_zy_walk(const uint16_t* __restr
     for (int j = 0; j < WB; ++j) {
       const int k = k0 + j;
       if (k >= kend) return;
-      const uint32_t* Sb = stage(k, j % PF);  // (k0 + j) % PF == j % PF
+      stage(k + 1, (j + 1) % PF);  // (k0 + j) % PF == j % PF
       uint32_t v[KR], o[KR];
-      ymin(Sb, v);
+      ymin(S[k & 1], v);
+      __syncthreads();
 #pragma unroll
       for (int i = 0; i < KR; ++i) {
         P[i] = j == 0 ? v[i] : pkmin(P[i], v[i]);
