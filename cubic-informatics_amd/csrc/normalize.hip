@@ -344,8 +344,8 @@ __global__ __launch_bounds__(NT) void k_minmax_walk(const void* __restrict__ src
 // Min / max along y and z in one pass: a work-group of 16 waves owns a 64-column x 32-row
 // tile of every plane and walks z.  Per slice:
 //   1. the packed rows y0-RY .. y0+31+RY of the slice (clamped to the plane; raw u16
-//      loaded PF slices ahead) enter an LDS buffer (two buffers, alternating: one barrier
-//      per slice);
+//      loaded PF slices ahead) enter an LDS buffer (two buffers, alternating: the next
+//      slice is stored while this one is read, one barrier per slice);
 //   2. each thread (lane = column, wave = row pair) forms the y window minima of its two
 //      rows from the 2RY+2 values of its column: a shared window middle plus one end value
 //      each;
